@@ -1,0 +1,257 @@
+/*
+ * rfhip.h — C ABI of the MI355X-native TV-L1 optical-flow and joint flow/depth solver.
+ *
+ * This is the drop-in boundary for the reference's CudaFlow / ReconFlow classes
+ * (reconflow/Cudaflow.h:17-361, reconflow/Reconflow.h:9-159). Every entry point is
+ * extern "C", takes plain pointers and sizes, and returns an int status (0 = success,
+ * negative = error; rf_last_error() describes the most recent failure on the calling
+ * thread). Nothing here exits the process.
+ *
+ * Layout conventions kept from the reference:
+ *   - images and fields are row-major float arrays of stride x height elements, with
+ *     stride = width rounded up to the stride alignment (Cudaflow.cu:985-994, default 32);
+ *   - host images handed to rf_copy_images are already padded to the stride
+ *     (test_reconflow/main.cpp:57-59); padding columns of outputs are undefined;
+ *   - flow maps frame-0 pixels into frame 1 (I0(x) ~ I1(x + u(x)), warpImage.cu:25-28);
+ *   - the pose of camera 1 is X1 = R X0 + t (solveUcSk.cu:25-27), 3x3 and 3x1 row-major
+ *     doubles; intrinsics are 3x3 row-major doubles.
+ *
+ * Device pointers (rf_k_* and the *_device calls) are HIP device addresses; `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).
+ */
+#ifndef RFHIP_H
+#define RFHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- method identifiers (Cudaflow.h:20-28, Reconflow.h:46-52) ---- */
+#define RF_METHOD_TVL1 0
+#define RF_METHOD_TVCHARB 1
+#define RF_METHOD_TVCHARBGRAD 2
+#define RF_METHOD_TVL1PATCH 3
+#define RF_METHODR_TVL1_MS 14
+#define RF_METHODR_TVL1_MS_FN 17
+#define RF_METHODR_TVL1_MS_FNSPARSE 20
+
+/* ---- input pixel types (the cvType argument of CudaFlow::initialize, Cudaflow.cu:99) ---- */
+#define RF_INPUT_32F 0   /* CV_32F gray, copied (rgbToGray.cu:66-77) */
+#define RF_INPUT_8UC3 1  /* CV_8UC3, ((r+g+b)/3)/256 (rgbToGray.cu:3-19) */
+#define RF_INPUT_8U 2    /* CV_8U, /256 (rgbToGray.cu:21-34) */
+#define RF_INPUT_16U 3   /* CV_16U, /65536 (rgbToGray.cu:36-49) */
+
+/* ---- execution paths ---- */
+#define RF_PATH_PLAIN 0  /* one kernel per reference step, reference buffer rotation */
+#define RF_PATH_FUSED 1  /* one fused kernel per solver iteration, same arithmetic */
+
+/* ---- storage of images, derivatives and dual fields ---- */
+#define RF_STORAGE_F32 0
+#define RF_STORAGE_F16 1 /* FP16 storage, every update computed in FP32 */
+
+/* Status codes. */
+#define RF_OK 0
+#define RF_ERR_ARG -1
+#define RF_ERR_STATE -2
+#define RF_ERR_HIP -3
+#define RF_ERR_ALLOC -4
+#define RF_ERR_UNSUPPORTED -5
+
+/* Solver parameters: the arguments of CudaFlow::initialize (Cudaflow.cu:99-127) and
+ * ReconFlow::_initializeR / initializeR (Reconflow.cu:15-41). */
+typedef struct rf_params {
+    int width, height;
+    int channels;          /* kept for interface parity; the pixel type decides the layout */
+    int input_type;        /* RF_INPUT_* */
+    int n_levels;
+    float scale;           /* pyramid divisor: 2.0 halves each level (Cudaflow.cu:167-168) */
+    int method;            /* RF_METHOD_* / RF_METHODR_* */
+    float lambda, lambdagrad, theta, tau;
+    int n_warp_iters, n_solver_iters;
+    int with_visualization;
+    /* ReconFlow (joint) parameters; theta is replaced by 1/alpha_tv (Reconflow.cu:32) */
+    float lambdaf, lambdams, alpha_tv, alpha_proj, alpha_fn;
+    /* launch geometry of the reference constructor (Cudaflow.cu:10-14) */
+    int stride_alignment;
+    /* this implementation's execution options */
+    int path;              /* RF_PATH_* */
+    int storage;           /* RF_STORAGE_* */
+    int use_graph;         /* capture the static solve schedule in a hipGraph and replay it */
+    int fix_coarse_k;      /* 1: K[2][2] = 1 on every pyramid level (the reference leaves it 0 on
+                              levels >= 1, Reconflow.cu:99-110, which breaks the coarse triangulation) */
+    int tri_f64;           /* 1: evaluate the triangulation (Solve3dMinimalArea) in FP64 */
+} rf_params;
+
+/* Fills p with the reference defaults (Cudaflow.cu:99-101, Reconflow.cu:27-30) for
+ * width x height: 1 warp, 100 iterations, lambda 100, theta 0.33, tau 0.25, alignment 32,
+ * plain FP32 path without a graph. */
+void rf_params_default(rf_params* p, int width, int height);
+
+typedef struct rf_context rf_context;
+
+/* Last error message of the calling thread ("" if none). */
+const char* rf_last_error(void);
+
+/* Number of levels the reference demo derives for a height (main.cpp:50-56 and
+ * CudaFlow::computePyramidLevels, Cudaflow.cu:1023-1031). */
+int rf_compute_pyramid_levels(int width, int height, int min_width, float scale);
+
+/* ---- lifecycle: CudaFlow(), initialize(...), ~CudaFlow() ---- */
+/* CudaFlow::CudaFlow(BlockWidth, BlockHeight, StrideAlignment) (Cudaflow.cu:10-14). The block
+ * shape is recorded for interface parity; kernels use gfx950 wave64 tiles. */
+int rf_create(rf_context** out, int block_width, int block_height, int stride_alignment);
+/* ~CudaFlow / ~ReconFlow (Cudaflow.cu:873-978, Reconflow.cu:479-508). */
+int rf_destroy(rf_context* ctx);
+/* CudaFlow::initialize / ReconFlow::_initializeR. Re-initialising releases the previous
+ * buffers (the reference leaks them). Allocates every device buffer once. */
+int rf_initialize(rf_context* ctx, const rf_params* params);
+/* ReconFlow::initializeR with a prior flow (Reconflow.cu:15-25): initialize + prior buffers. */
+int rf_initialize_large_displacement(rf_context* ctx);
+/* Pyramid geometry chosen at initialize (Cudaflow.cu:146-171). Arrays hold n_levels ints. */
+int rf_get_pyramid(const rf_context* ctx, int* widths, int* heights, int* strides);
+int rf_get_stride(const rf_context* ctx);
+/* The stream every kernel of this context runs on (hipStream_t as void*). */
+void* rf_get_stream(const rf_context* ctx);
+
+/* ---- inputs ---- */
+/* CudaFlow::copyImagesToDevice (Cudaflow.cu:288-311): stride x height pixels of the
+ * initialize() input type from host memory. Synchronous. */
+int rf_copy_images(rf_context* ctx, const void* i0, const void* i1);
+/* Same from device memory (stride x height float gray each), enqueued on the stream. */
+int rf_copy_images_device(rf_context* ctx, const float* d_i0, const float* d_i1);
+/* ReconFlow::setCameraMatrices(K0, K1) (Reconflow.cu:78-116). */
+int rf_set_camera(rf_context* ctx, const double K0[9], const double K1[9]);
+/* CudaFlow::copyInitialOpticalFlowToDevice (Cudaflow.cu:326-330) and
+ * copySparseOpticalFlowToDevice (Cudaflow.cu:332-343); mask may be NULL (all ones). */
+int rf_copy_initial_flow(rf_context* ctx, const float* u, const float* v);
+int rf_copy_sparse_flow(rf_context* ctx, const float* u, const float* v, const float* mask);
+
+/* ---- solve ---- */
+/* CudaFlow::solveOpticalFlow(flowScale) (Cudaflow.cu:395-621). Asynchronous on the stream. */
+int rf_solve_flow(rf_context* ctx, float flow_scale);
+/* ReconFlow::solveR(R0, t0, R1, t1, flowScale) (Reconflow.cu:134-466). Pass R0 = t0 = NULL
+ * for camera 0 at the origin (solveR(R1, t1), Reconflow.cu:130-144). Asynchronous. */
+int rf_solve_recon(rf_context* ctx, const double* R0, const double* t0, const double* R1, const double* t1,
+                   float flow_scale);
+/* Blocks until the context's stream is idle. */
+int rf_synchronize(rf_context* ctx);
+
+/* ---- outputs ---- */
+/* CudaFlow::copyOpticalFlowToHost(u, v[, uvrgb]) (Cudaflow.cu:313-324): stride x height
+ * floats each; uvrgb (3 floats per pixel, BGR) may be NULL. Synchronous. */
+int rf_copy_flow(rf_context* ctx, float* u, float* v, float* uvrgb);
+/* ReconFlow::copy3dToHost(X, Y, Z) (Reconflow.cu:65-70): camera-0 frame points. */
+int rf_copy_3d(rf_context* ctx, float* X, float* Y, float* Z);
+/* Device addresses of the current result fields (valid until the next solve). */
+int rf_get_result_device(const rf_context* ctx, const float** u, const float** v, const float** X,
+                         const float** Y, const float** Z);
+/* ReconFlow::removeEdgeArtifacts3D(surfaceWidth, maxSurfaceArea) (Reconflow.cu:468-477). */
+int rf_remove_edge_artifacts_3d(rf_context* ctx, int surface_width, float max_surface_area);
+
+/* ---- kernel timing (no counterpart in the reference, which has no timers) ----
+ * Arms HIP-event timing of one kernel kind at one pyramid level for the next solves that run
+ * without a graph: events are recorded on the context's stream around every launch of that
+ * kind (around each uninterrupted run of launches for the fused iteration kernel).
+ * rf_profile_read returns the summed kernel time and the number of launches timed since the
+ * last arm, and stops timing. */
+#define RF_KID_FUSED_ITER 0
+#define RF_KID_DATA 1
+#define RF_KID_DUAL 2
+#define RF_KID_UVPROJ 3
+#define RF_KID_SOLVE3D 4
+#define RF_KID_MEDIAN3D 5
+#define RF_KID_CLEANUP 6
+#define RF_KID_SOLVE_UC 7
+#define RF_KID_UPDATE_SK 8
+int rf_profile_arm(rf_context* ctx, int kernel_id, int level);
+int rf_profile_read(rf_context* ctx, double* total_ms, int* launches);
+
+/* ---- batch API: many pairs of uneven sizes per launch ---- */
+typedef struct rf_batch rf_batch;
+/* Creates a batch of n pairs that share params (except width/height/input_type, taken
+ * from the arrays). Pairs are solved together level by level: every kernel launch covers
+ * the tiles of all pairs at that step. */
+int rf_batch_create(rf_batch** out, const rf_params* params, int n, const int* widths, const int* heights);
+int rf_batch_destroy(rf_batch* b);
+int rf_batch_stride(const rf_batch* b, int i);
+/* Device float gray images (stride_i x height_i) of pair i; enqueued on the batch stream. */
+int rf_batch_set_images_device(rf_batch* b, int i, const float* d_i0, const float* d_i1);
+int rf_batch_set_camera(rf_batch* b, int i, const double K0[9], const double K1[9], const double R1[9],
+                        const double t1[3]);
+int rf_batch_solve(rf_batch* b);
+int rf_batch_synchronize(rf_batch* b);
+int rf_batch_get_result_device(const rf_batch* b, int i, const float** u, const float** v, const float** X,
+                               const float** Y, const float** Z);
+void* rf_batch_get_stream(const rf_batch* b);
+
+/* ---- per-kernel entry points on device pointers (the public kernel wrappers of
+ * Cudaflow.h:101-193 and Reconflow.h:86-158), enqueued on `stream` ---- */
+int rf_k_rgb_to_gray(const uint8_t* rgb, float* gray, int w, int h, int s, void* stream);       /* rgbToGray.cu:80-86 */
+int rf_k_cv8u_to_gray(const uint8_t* in, float* gray, int w, int h, int s, void* stream);      /* rgbToGray.cu:88-94 */
+int rf_k_cv16u_to_gray(const uint16_t* in, float* gray, int w, int h, int s, void* stream);    /* rgbToGray.cu:96-102 */
+int rf_k_cv32f_to_gray(const float* in, float* gray, int w, int h, int s, void* stream);       /* rgbToGray.cu:112-118 */
+int rf_k_downscale(const float* src, int W, int H, int S, int w, int h, int s, float* out,
+                   void* stream);                                                            /* downscale.cu:50-67 */
+int rf_k_downscale_scaled(const float* src, int W, int H, int S, int w, int h, int s, float scale,
+                          float* out, void* stream);                                         /* downscale.cu:69-85 */
+int rf_k_upscale(const float* src, int w, int h, int s, int W, int H, int S, float scale, float* out,
+                 void* stream);                                                              /* upscale.cu:42-59 */
+int rf_k_warp_image(const float* src, int w, int h, int s, const float* u, const float* v, float* out,
+                    void* stream);                                                           /* warpImage.cu:33-50 */
+int rf_k_compute_derivatives(const float* I0, const float* I1, int w, int h, int s, float* Ix, float* Iy,
+                             float* Iz, void* stream);                                       /* computeDerivatives.cu:60-84 */
+int rf_k_solve_data_l1(const float* duhat0, const float* dvhat0, const float* pu1, const float* pu2,
+                       const float* pv1, const float* pv2, const float* Ix, const float* Iy, const float* Iz,
+                       int w, int h, int s, float lambda, float theta, float* duhat1, float* dvhat1,
+                       void* stream);                                                        /* solveDataL1.cu:94-114 */
+int rf_k_solve_smooth_dual_tv_global(const float* duhat, const float* dvhat, const float* pu1, const float* pu2,
+                                     const float* pv1, const float* pv2, int w, int h, int s, float tau,
+                                     float theta, float* pu1s, float* pu2s, float* pv1s, float* pv2s,
+                                     void* stream);                                          /* solveSmoothDualTVGlobal.cu:80-96 */
+int rf_k_median_filter(const float* u, const float* v, int w, int h, int s, float* ou, float* ov,
+                       int kernel_size, void* stream);                                       /* medianFilter.cu:145-161 */
+int rf_k_median_filter_3d(const float* X, const float* Y, const float* Z, int w, int h, int s, float* X1,
+                          float* Y1, float* Z1, int kernel_size, void* stream);              /* medianFilter.cu:246-258 */
+int rf_k_add(const float* a, const float* b, int count, float* sum, void* stream);           /* addKernel.cu:13-18 */
+int rf_k_solve_recon_data_l1(const float* u, const float* v, const float* duhat0, const float* dvhat0,
+                             const float* uproj, const float* vproj, const float* sku, const float* skv,
+                             const float* pu1, const float* pu2, const float* pv1, const float* pv2,
+                             const float* Ix, const float* Iy, const float* Iz, int w, int h, int s,
+                             float lambda, float alpha_tv, float alpha_proj, float* duhat1, float* dvhat1,
+                             void* stream);                                                  /* solveReconDataL1.cu:105-131 */
+int rf_k_solve_recon_data_l1_fn(const float* u, const float* v, const float* duhat0, const float* dvhat0,
+                                const float* uproj, const float* vproj, const float* ufn, const float* vfn,
+                                const float* sku, const float* skv, const float* pu1, const float* pu2,
+                                const float* pv1, const float* pv2, const float* Ix, const float* Iy,
+                                const float* Iz, int w, int h, int s, float lambda, float alpha_tv,
+                                float alpha_proj, float alpha_fn, const float* fnmask, float* duhat1,
+                                float* dvhat1, void* stream);                                /* solveReconDataL1.cu:243-274, 381-409; fnmask NULL = dense */
+int rf_k_solve_uvproj(const float* u, const float* v, const float* du, const float* dv, const float* uc,
+                      const float* vc, const float* sku, const float* skv, int w, int h, int s, float lambdaf,
+                      float alpha_proj, float* uproj, float* vproj, void* stream);          /* solveUVProj.cu:25-45 */
+int rf_k_solve_3d_minimal_area(const float* uproj, const float* vproj, const float* X0, const float* Y0,
+                               const float* Z0, const double P[12], const double Q[12], float lambdaf,
+                               float lambdams, int w, int h, int s, float* X, float* Y, float* Z,
+                               void* stream);                                                /* solve3dMinimalArea.cu:197-241 */
+int rf_k_cleanup_3d(const float* X0, const float* Y0, const float* Z0, const float* Xm, const float* Ym,
+                    const float* Zm, int w, int h, int s, float* X1, float* Y1, float* Z1,
+                    void* stream);                                                           /* solve3dMinimalArea.cu:299-313 */
+int rf_k_solve_uc(const float* X, const float* Y, const float* Z, const double R[9], const double t[3],
+                  const double K1[9], int w, int h, int s, float* uc, float* vc, void* stream); /* solveUcSk.cu:37-74 */
+int rf_k_update_sk(const float* sku0, const float* skv0, const float* u, const float* v, const float* du,
+                   const float* dv, const float* uproj, const float* vproj, int w, int h, int s, float* sku1,
+                   float* skv1, void* stream);                                               /* solveUcSk.cu:99-115 */
+int rf_k_flow_to_hsv(const float* u, const float* v, int w, int h, int s, float* uvrgb, float flow_scale,
+                     void* stream);                                                          /* flowToHSV.cu:304-311 */
+int rf_k_remove_edge_artifacts_3d(const float* X0, const float* Y0, const float* Z0, int surface_width,
+                                  float max_surface_area, int w, int h, int s, float* X1, float* Y1, float* Z1,
+                                  void* stream);                                             /* solve3dMinimalArea.cu:380-393 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RFHIP_H */

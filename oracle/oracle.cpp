@@ -1,0 +1,1067 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the Reconflow coarse-to-fine TV-L1 optical-flow path and its
+// epipolar-constrained joint flow/depth path, written from the text of the reference
+// kernels. Every routine is a template over the arithmetic type T:
+//   * T = double  -> the FP64 oracle that judges the GPU path (tolerances);
+//   * T = float   -> an FP32 restatement with the reference's exact operation order,
+//                    built with -ffp-contract=off, used to show bit-level agreement.
+// Only tests, the smoke check and the bench's CPU-baseline leg load this library.
+// Nothing in the product links or calls it.
+//
+// Semantics fixed here where the reference leaves them to hardware or to undefined
+// behaviour (each also stated where it is used):
+//   * texture sampling S(I, tx, ty): bilinear with exact weights, texel-centre
+//     coordinates, edge-duplicating mirror addressing over the bound w x h region;
+//   * fresh device buffers are zero; one zero guard row follows every level-0-sized
+//     buffer (it stands for the read past the allocation in Solve3dMinimalArea);
+//   * a NaN rho in the L1 data steps gives NaN du/dv (the reference leaves them
+//     unassigned);
+//   * a warp coordinate that is not finite or exceeds 2^24 in magnitude samples NaN.
+//
+// All file:line citations are relative to the reference repository root.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------------
+// Texture emulation. Mirror addressing on normalized coordinates with linear
+// filtering (e.g. warpImage.cu:40-47): texel index i maps into [0, n) with period 2n,
+// duplicating the edge texel (-1 -> 0, n -> n-1).
+static inline long refl(long i, long n) {
+    long p = 2 * n;
+    long m = i % p;
+    if (m < 0) m += p;
+    return m < n ? m : p - 1 - m;
+}
+
+static inline long floor_div(long a, long b) {  // b > 0
+    long q = a / b;
+    if ((a % b != 0) && (a < 0)) --q;
+    return q;
+}
+
+// Bilinear sample at texel coordinate (x0 + fx, y0 + fy), weights as in the public
+// CUDA texture-fetch formula (1-a)(1-b)T00 + a(1-b)T10 + (1-a)bT01 + abT11.
+template <typename T>
+static inline T bilin(const T* img, long w, long h, long s, long x0, T fx, long y0, T fy) {
+    long xa = refl(x0, w), xb = refl(x0 + 1, w);
+    long ya = refl(y0, h), yb = refl(y0 + 1, h);
+    T one = T(1);
+    T w00 = (one - fx) * (one - fy);
+    T w10 = fx * (one - fy);
+    T w01 = (one - fx) * fy;
+    T w11 = fx * fy;
+    T r = w00 * img[xa + ya * s];
+    r = r + w10 * img[xb + ya * s];
+    r = r + w01 * img[xa + yb * s];
+    r = r + w11 * img[xb + yb * s];
+    return r;
+}
+
+// Sample at the rational texel coordinate (nx/dx, ny/dy) with integer numerators.
+template <typename T>
+static inline T sample_ratio(const T* img, long w, long h, long s, long nx, long dx, long ny, long dy) {
+    long x0 = floor_div(nx, dx), y0 = floor_div(ny, dy);
+    T fx = T(nx - x0 * dx) / T(dx);
+    T fy = T(ny - y0 * dy) / T(dy);
+    return bilin(img, w, h, s, x0, fx, y0, fy);
+}
+
+// ---------------------------------------------------------------------------------
+// A2: rgbToGray (rgbToGray.cu:3-19): g = ((float)r + (float)g + (float)b)/3; g /= 256.
+template <typename T>
+void gray_rgb(const uint8_t* rgb, T* out, int w, int h, int s) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T g = ((T)rgb[3 * p] + (T)rgb[3 * p + 1] + (T)rgb[3 * p + 2]) / T(3);
+            out[p] = g / T(256);
+        }
+}
+
+// A2: Cv32fToGray (rgbToGray.cu:66-77): copy of the valid region.
+template <typename T>
+void gray_f32(const float* in, T* out, int w, int h, int s) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) out[ix + (long)iy * s] = (T)in[ix + (long)iy * s];
+}
+
+// f2: Cv8uToGray / Cv16uToGray (rgbToGray.cu:21-49).
+template <typename T>
+void gray_u8(const uint8_t* in, T* out, int w, int h, int s) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) out[ix + (long)iy * s] = (T)in[ix + (long)iy * s] / T(256);
+}
+template <typename T>
+void gray_u16(const uint16_t* in, T* out, int w, int h, int s) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) out[ix + (long)iy * s] = (T)in[ix + (long)iy * s] / T(65536);
+}
+
+// A3: DownscaleKernel (downscale.cu:6-26). Coarse pixel centre in fine texels is
+// (ix+1/2)W/w - 1/2; the four taps sit a quarter coarse pixel to each side.
+// scale != 1 gives DownscaleScalingKernel (downscale.cu:28-48).
+template <typename T>
+void downscale(const T* src, int W, int H, int S, int w, int h, int s, T scale, bool scaled, T* out) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long dxn = 4L * w, dyn = 4L * h;
+            long xc = (4L * ix + 2) * W - 2L * w;
+            long yc = (4L * iy + 2) * H - 2L * h;
+            T a = sample_ratio(src, W, H, S, xc - W, dxn, yc, dyn);
+            T b = sample_ratio(src, W, H, S, xc + W, dxn, yc, dyn);
+            T c = sample_ratio(src, W, H, S, xc, dxn, yc - H, dyn);
+            T d = sample_ratio(src, W, H, S, xc, dxn, yc + H, dyn);
+            T sum = ((a + b) + c) + d;
+            out[ix + (long)iy * s] = scaled ? (scale * T(0.25)) * sum : T(0.25) * sum;
+        }
+}
+
+// A10: UpscaleKernel (upscale.cu:14-27): fine pixel centre in coarse texels is
+// (ix+1/2)w/W - 1/2; result times scale.
+template <typename T>
+void upscale(const T* src, int w, int h, int s, int W, int H, int S, T scale, T* out) {
+#pragma omp parallel for
+    for (int iy = 0; iy < H; ++iy)
+        for (int ix = 0; ix < W; ++ix) {
+            T v = sample_ratio(src, w, h, s, (2L * ix + 1) * w - W, 2L * W, (2L * iy + 1) * h - H, 2L * H);
+            out[ix + (long)iy * S] = v * scale;
+        }
+}
+
+// A4: WarpingKernel (warpImage.cu:15-29): out = S(I1, ix + u, iy + v).
+template <typename T>
+void warp(const T* src, int w, int h, int s, const T* u, const T* v, T* out) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T uu = u[p], vv = v[p];
+            if (!(std::fabs(uu) <= T(16777216)) || !(std::fabs(vv) <= T(16777216))) {
+                out[p] = std::numeric_limits<T>::quiet_NaN();
+                continue;
+            }
+            T fu = std::floor(uu), fv = std::floor(vv);
+            out[p] = bilin(src, w, h, s, ix + (long)fu, uu - fu, iy + (long)fv, vv - fv);
+        }
+}
+
+// A5: ComputeDerivativesKernel (computeDerivatives.cu:8-57). Taps sit on exact texel
+// centres, so each is a mirror-indexed read; operation order as at :26-56.
+template <typename T>
+void derivatives(const T* I0, const T* I1, int w, int h, int s, T* Ix, T* Iy, T* Iz) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            auto at = [&](const T* I, long x, long y) { return I[refl(x, w) + refl(y, h) * s]; };
+            T t0, t1;
+            t0 = at(I0, ix - 2, iy);
+            t0 -= at(I0, ix - 1, iy) * T(8);
+            t0 += at(I0, ix + 1, iy) * T(8);
+            t0 -= at(I0, ix + 2, iy);
+            t0 /= T(12);
+            t1 = at(I1, ix - 2, iy);
+            t1 -= at(I1, ix - 1, iy) * T(8);
+            t1 += at(I1, ix + 1, iy) * T(8);
+            t1 -= at(I1, ix + 2, iy);
+            t1 /= T(12);
+            Ix[p] = (t0 + t1) * T(0.5);
+            Iz[p] = at(I1, ix, iy) - at(I0, ix, iy);
+            t0 = at(I0, ix, iy - 2);
+            t0 -= at(I0, ix, iy - 1) * T(8);
+            t0 += at(I0, ix, iy + 1) * T(8);
+            t0 -= at(I0, ix, iy + 2);
+            t0 /= T(12);
+            t1 = at(I1, ix, iy - 2);
+            t1 -= at(I1, ix, iy - 1) * T(8);
+            t1 += at(I1, ix, iy + 1) * T(8);
+            t1 -= at(I1, ix, iy + 2);
+            t1 /= T(12);
+            Iy[p] = (t0 + t1) * T(0.5);
+        }
+}
+
+// Divergence of the dual field with the reference's left/top border rule
+// (solveDataL1.cu:64-88): backward differences, missing neighbours dropped.
+template <typename T>
+static inline T div_p(const T* p1, const T* p2, int ix, int iy, int s) {
+    long p = ix + (long)iy * s;
+    if (ix - 1 < 0) {
+        if (iy - 1 < 0) return p1[p] + p2[p];
+        return (p1[p] + p2[p]) - p2[p - s];
+    }
+    if (iy - 1 < 0) return (p1[p] - p1[p - 1]) + p2[p];
+    return ((p1[p] - p1[p - 1]) + p2[p]) - p2[p - s];
+}
+
+// A6: SolveDataL1Kernel (solveDataL1.cu:4-91): pointwise L1 threshold + primal step.
+template <typename T>
+void data_l1(const T* duhat0, const T* dvhat0, const T* pu1, const T* pu2, const T* pv1, const T* pv2,
+             const T* Ix, const T* Iy, const T* It, int w, int h, int s, T lambda, T theta, T* duhat1,
+             T* dvhat1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T dix = Ix[p], diy = Iy[p], dit = It[p];
+            T duhat = duhat0[p], dvhat = dvhat0[p];
+            T rho = (dix * duhat + diy * dvhat) + dit;
+            T lt = lambda * theta;
+            T g2 = dix * dix + diy * diy;
+            T upper = lt * g2;
+            T lower = ((-lambda) * theta) * g2;
+            T du = std::numeric_limits<T>::quiet_NaN(), dv = du;
+            if ((rho <= upper) && (rho >= lower)) {
+                T magi = dix * dix + diy * diy;
+                if (magi != T(0)) {
+                    du = duhat - (rho * dix) / magi;
+                    dv = dvhat - (rho * diy) / magi;
+                } else {
+                    du = duhat;
+                    dv = dvhat;
+                }
+            } else if (rho < lower) {
+                du = duhat + lt * dix;
+                dv = dvhat + lt * diy;
+            } else if (rho > upper) {
+                du = duhat - lt * dix;
+                dv = dvhat - lt * diy;
+            }
+            duhat1[p] = du + theta * div_p(pu1, pu2, ix, iy, s);
+            dvhat1[p] = dv + theta * div_p(pv1, pv2, ix, iy, s);
+        }
+}
+
+// A7: SolveSmoothDualTVGlobalKernel (solveSmoothDualTVGlobal.cu:4-65).
+template <typename T>
+void dual_tv(const T* duhat, const T* dvhat, const T* pu1, const T* pu2, const T* pv1, const T* pv2, int w,
+             int h, int s, T tau, T theta, T* pu1s, T* pu2s, T* pv1s, T* pv2s) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T dux, dvx, duy, dvy;
+            if (ix + 1 >= w) {
+                dux = 0;
+                dvx = 0;
+            } else {
+                dux = duhat[p + 1] - duhat[p];
+                dvx = dvhat[p + 1] - dvhat[p];
+            }
+            if (iy + 1 >= h) {
+                duy = 0;
+                dvy = 0;
+            } else {
+                duy = duhat[p + s] - duhat[p];
+                dvy = dvhat[p + s] - dvhat[p];
+            }
+            T magdu = std::sqrt(dux * dux + duy * duy);
+            T magdv = std::sqrt(dvx * dvx + dvy * dvy);
+            T fac = tau / theta;
+            pu1s[p] = (pu1[p] + fac * dux) / (T(1) + fac * magdu);
+            pu2s[p] = (pu2[p] + fac * duy) / (T(1) + fac * magdu);
+            pv1s[p] = (pv1[p] + fac * dvx) / (T(1) + fac * magdv);
+            pv2s[p] = (pv2[p] + fac * dvy) / (T(1) + fac * magdv);
+        }
+}
+
+// Partial exchange sort of the reference median kernels: for j < nsel, for i > j,
+// swap when m[j] > m[i]; the answer is m[nsel-1].
+template <typename T, int N>
+static inline T exchange_select(T (&m)[N], int nsel) {
+    for (int j = 0; j < nsel; ++j)
+        for (int i = j + 1; i < N; ++i)
+            if (m[j] > m[i]) std::swap(m[j], m[i]);
+    return m[nsel - 1];
+}
+
+// A8: MedianFilterKernel5 (medianFilter.cu:3-75). Taps start at 0; fill rule at :31-50
+// (a tap outside in both row and column keeps its 0).
+template <typename T>
+void median5(const T* u, const T* v, int w, int h, int s, T* ou, T* ov) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            T mu[25], mv[25];
+            for (int k = 0; k < 25; ++k) mu[k] = mv[k] = T(0);
+            for (int j = 0; j < 5; ++j)
+                for (int i = 0; i < 5; ++i) {
+                    int col = ix + i - 2, row = iy + j - 2;
+                    long q = -1;
+                    if (col >= 0 && col < w && row >= 0 && row < h) q = col + (long)s * row;
+                    else if (col < 0 && row >= 0 && row < h) q = (long)s * row;
+                    else if (col >= w && row >= 0 && row < h) q = w - 1 + (long)s * row;
+                    else if (col >= 0 && col < w && row < 0) q = col;
+                    else if (col >= 0 && col < w && row >= h) q = col + (long)s * (h - 1);
+                    if (q >= 0) {
+                        mu[j * 5 + i] = u[q];
+                        mv[j * 5 + i] = v[q];
+                    }
+                }
+            long p = ix + (long)iy * s;
+            ou[p] = exchange_select(mu, 13);
+            ov[p] = exchange_select(mv, 13);
+        }
+}
+
+// A15: MedianFilter3DKernel3 (medianFilter.cu:166-243). The right and bottom
+// replacements test col > width / row > height, so taps at col == w or row == h stay 0.
+template <typename T>
+void median3d(const T* X, const T* Y, const T* Z, int w, int h, int s, T* X1, T* Y1, T* Z1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            T mX[9], mY[9], mZ[9];
+            for (int k = 0; k < 9; ++k) mX[k] = mY[k] = mZ[k] = T(0);
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    int col = ix + i - 1, row = iy + j - 1;
+                    long q = -1;
+                    if (col >= 0 && col < w && row >= 0 && row < h) q = col + (long)s * row;
+                    else if (col < 0 && row >= 0 && row < h) q = (long)s * row;
+                    else if (col > w && row >= 0 && row < h) q = w - 1 + (long)s * row;
+                    else if (col >= 0 && col < w && row < 0) q = col;
+                    else if (col >= 0 && col < w && row > h) q = col + (long)s * (h - 1);
+                    if (q >= 0) {
+                        mX[j * 3 + i] = X[q];
+                        mY[j * 3 + i] = Y[q];
+                        mZ[j * 3 + i] = Z[q];
+                    }
+                }
+            long p = ix + (long)iy * s;
+            X1[p] = exchange_select(mX, 5);
+            Y1[p] = exchange_select(mY, 5);
+            Z1[p] = exchange_select(mZ, 5);
+        }
+}
+
+// A9: AddKernel (addKernel.cu:3-11) over count elements (padding included).
+template <typename T>
+void add(const T* a, const T* b, long count, T* out) {
+#pragma omp parallel for
+    for (long i = 0; i < count; ++i) out[i] = a[i] + b[i];
+}
+
+// A12: SolveReconDataL1Kernel (solveReconDataL1.cu:4-102).
+template <typename T>
+void recon_data_l1(const T* u0, const T* v0, const T* duhat0, const T* dvhat0, const T* uproj0,
+                   const T* vproj0, const T* sku0, const T* skv0, const T* pu1, const T* pu2, const T* pv1,
+                   const T* pv2, const T* Ix, const T* Iy, const T* It, int w, int h, int s, T lambda,
+                   T alphaTv, T alphaProj, T* duhat1, T* dvhat1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T dix = Ix[p], diy = Iy[p], dit = It[p];
+            T A = alphaTv + alphaProj;
+            T dusub = (alphaTv * duhat0[p] + alphaProj * ((uproj0[p] - u0[p]) - sku0[p])) / A;
+            T dvsub = (alphaTv * dvhat0[p] + alphaProj * ((vproj0[p] - v0[p]) - skv0[p])) / A;
+            T rho = (dix * dusub + diy * dvsub) + dit;
+            T g2 = dix * dix + diy * diy;
+            T upper = (lambda * g2) / A;
+            T lower = ((-lambda) * g2) / A;
+            T du = std::numeric_limits<T>::quiet_NaN(), dv = du;
+            if ((rho <= upper) && (rho >= lower)) {
+                T magi = dix * dix + diy * diy;
+                if (magi != T(0)) {
+                    du = dusub - (rho * dix) / magi;
+                    dv = dvsub - (rho * diy) / magi;
+                } else {
+                    du = dusub;
+                    dv = dvsub;
+                }
+            } else if (rho < lower) {
+                du = dusub + (lambda * dix) / A;
+                dv = dvsub + (lambda * diy) / A;
+            } else if (rho > upper) {
+                du = dusub - (lambda * dix) / A;
+                dv = dvsub - (lambda * diy) / A;
+            }
+            duhat1[p] = du + div_p(pu1, pu2, ix, iy, s) / A;
+            dvhat1[p] = dv + div_p(pv1, pv2, ix, iy, s) / A;
+        }
+}
+
+// f1: SolveReconDataL1FlownetKernel, dense and sparse (solveReconDataL1.cu:139-240,
+// 276-378): the data blend gains an alphaFn pull towards the prior flow; the sparse
+// form weights alphaFn by the mask.
+template <typename T>
+void recon_data_l1_fn(const T* u0, const T* v0, const T* duhat0, const T* dvhat0, const T* uproj0,
+                      const T* vproj0, const T* ufn0, const T* vfn0, const T* sku0, const T* skv0,
+                      const T* pu1, const T* pu2, const T* pv1, const T* pv2, const T* Ix, const T* Iy,
+                      const T* It, int w, int h, int s, T lambda, T alphaTv, T alphaProj, T alphaFn0,
+                      const T* mask, T* duhat1, T* dvhat1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T alphaFn = mask ? alphaFn0 * mask[p] : alphaFn0;
+            T dix = Ix[p], diy = Iy[p], dit = It[p];
+            T A = (alphaTv + alphaProj) + alphaFn;
+            T dusub = ((alphaFn * (ufn0[p] - u0[p]) + alphaTv * duhat0[p]) +
+                       alphaProj * ((uproj0[p] - u0[p]) - sku0[p])) / A;
+            T dvsub = ((alphaFn * (vfn0[p] - v0[p]) + alphaTv * dvhat0[p]) +
+                       alphaProj * ((vproj0[p] - v0[p]) - skv0[p])) / A;
+            T rho = (dix * dusub + diy * dvsub) + dit;
+            T g2 = dix * dix + diy * diy;
+            T upper = (lambda * g2) / A;
+            T lower = ((-lambda) * g2) / A;
+            T du = std::numeric_limits<T>::quiet_NaN(), dv = du;
+            if ((rho <= upper) && (rho >= lower)) {
+                T magi = dix * dix + diy * diy;
+                if (magi != T(0)) {
+                    du = dusub - (rho * dix) / magi;
+                    dv = dvsub - (rho * diy) / magi;
+                } else {
+                    du = dusub;
+                    dv = dvsub;
+                }
+            } else if (rho < lower) {
+                du = dusub + (lambda * dix) / A;
+                dv = dvsub + (lambda * diy) / A;
+            } else if (rho > upper) {
+                du = dusub - (lambda * dix) / A;
+                dv = dvsub - (lambda * diy) / A;
+            }
+            duhat1[p] = du + div_p(pu1, pu2, ix, iy, s) / A;
+            dvhat1[p] = dv + div_p(pv1, pv2, ix, iy, s) / A;
+        }
+}
+
+// A13: SolveUVProjKernel (solveUVProj.cu:4-22).
+template <typename T>
+void uvproj(const T* u0, const T* v0, const T* du0, const T* dv0, const T* uc0, const T* vc0, const T* sku0,
+            const T* skv0, int w, int h, int s, T lambdaf, T alphaProj, T* uproj1, T* vproj1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T den = lambdaf + alphaProj;
+            uproj1[p] = (lambdaf * uc0[p] + alphaProj * ((u0[p] + du0[p]) + sku0[p])) / den;
+            vproj1[p] = (lambdaf * vc0[p] + alphaProj * ((v0[p] + dv0[p]) + skv0[p])) / den;
+        }
+}
+
+// A14: Solve3dMinimalAreaKernel (solve3dMinimalArea.cu:3-193). The guards ix+1 > width
+// and iy+1 > height (:65, :76, :96, :112) are never true, so column w and row h are read;
+// the caller's buffers carry the flat level-0 layout plus a zero guard row.
+template <typename T>
+void solve3d(const T* uproj0, const T* vproj0, const T* X0, const T* Y0, const T* Z0, const double* P,
+             const double* Q, T lambdaf, T lambdams, int w, int h, int s, T* X, T* Y, T* Z) {
+    const T P11 = (T)P[0], P12 = (T)P[1], P13 = (T)P[2], P14 = (T)P[3];
+    const T P21 = (T)P[4], P22 = (T)P[5], P23 = (T)P[6], P24 = (T)P[7];
+    const T P31 = (T)P[8], P32 = (T)P[9], P33 = (T)P[10], P34 = (T)P[11];
+    const T Q11 = (T)Q[0], Q12 = (T)Q[1], Q13 = (T)Q[2], Q14 = (T)Q[3];
+    const T Q21 = (T)Q[4], Q22 = (T)Q[5], Q23 = (T)Q[6], Q24 = (T)Q[7];
+    const T Q31 = (T)Q[8], Q32 = (T)Q[9], Q33 = (T)Q[10], Q34 = (T)Q[11];
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long pos = ix + (long)iy * s;
+            T x1 = (T)ix, y1 = (T)iy;
+            T x2 = (T)ix + uproj0[pos];
+            T y2 = (T)iy + vproj0[pos];
+            T p1 = x1 * P31 - P11, p2 = x1 * P32 - P12, p3 = x1 * P33 - P13, p4 = x1 * P34 - P14;
+            T q1 = y1 * P31 - P21, q2 = y1 * P32 - P22, q3 = y1 * P33 - P23, q4 = y1 * P34 - P24;
+            T r1 = x2 * Q31 - Q11, r2 = x2 * Q32 - Q12, r3 = x2 * Q33 - Q13, r4 = x2 * Q34 - Q14;
+            T s1 = y2 * Q31 - Q21, s2 = y2 * Q32 - Q22, s3 = y2 * Q33 - Q23, s4 = y2 * Q34 - Q24;
+            auto dot4 = [](T a1, T b1, T a2, T b2, T a3, T b3, T a4, T b4) {
+                return ((a1 * b1 + a2 * b2) + a3 * b3) + a4 * b4;
+            };
+            T ax = lambdaf * dot4(p1, p1, q1, q1, r1, r1, s1, s1);
+            T bx = lambdaf * dot4(p1, p2, q1, q2, r1, r2, s1, s2);
+            T cx = lambdaf * dot4(p1, p3, q1, q3, r1, r3, s1, s3);
+            T dx = lambdaf * dot4(p1, p4, q1, q4, r1, r4, s1, s4);
+            T ay = lambdaf * dot4(p2, p1, q2, q1, r2, r1, s2, s1);
+            T by = lambdaf * dot4(p2, p2, q2, q2, r2, r2, s2, s2);
+            T cy = lambdaf * dot4(p2, p3, q2, q3, r2, r3, s2, s3);
+            T dy = lambdaf * dot4(p2, p4, q2, q4, r2, r4, s2, s4);
+            T az = lambdaf * dot4(p3, p1, q3, q1, r3, r1, s3, s1);
+            T bz = lambdaf * dot4(p3, p2, q3, q2, r3, r2, s3, s2);
+            T cz = lambdaf * dot4(p3, p3, q3, q3, r3, r3, s3, s3);
+            T dz = lambdaf * dot4(p3, p4, q3, q4, r3, r4, s3, s4);
+
+            // forward differences; the border branches of the reference never fire
+            T Xx = X0[pos + 1] - X0[pos], Yx = Y0[pos + 1] - Y0[pos], Zx = Z0[pos + 1] - Z0[pos];
+            T Xy = X0[pos + s] - X0[pos], Yy = Y0[pos + s] - Y0[pos], Zy = Z0[pos + s] - Z0[pos];
+
+            T c1 = Yx * Zy - Zx * Yy;
+            T c2 = Zx * Xy - Xx * Zy;
+            T c3 = Xx * Yy - Yx * Xy;
+            T lambetams = lambdams / std::sqrt(((c1 * c1 + c2 * c2) + c3 * c3) + T(0.001f));
+
+            T Xibar, Yibar, Zibar, Xjbar, Yjbar, Zjbar;
+            if (ix - 1 < 0) {
+                Xibar = X0[pos + 1];
+                Yibar = Y0[pos + 1];
+                Zibar = Z0[pos + 1];
+            } else {
+                Xibar = T(0.5) * X0[pos - 1] + T(0.5) * X0[pos + 1];
+                Yibar = T(0.5) * Y0[pos - 1] + T(0.5) * Y0[pos + 1];
+                Zibar = T(0.5) * Z0[pos - 1] + T(0.5) * Z0[pos + 1];
+            }
+            if (iy - 1 < 0) {
+                Xjbar = X0[pos + s];
+                Yjbar = Y0[pos + s];
+                Zjbar = Z0[pos + s];
+            } else {
+                Xjbar = T(0.5) * X0[pos - s] + T(0.5) * X0[pos + s];
+                Yjbar = T(0.5) * Y0[pos - s] + T(0.5) * Y0[pos + s];
+                Zjbar = T(0.5) * Z0[pos - s] + T(0.5) * Z0[pos + s];
+            }
+
+            T a11 = ax + lambetams * (((Yx * Yx + Zx * Zx) + Yy * Yy) + Zy * Zy);
+            T a12 = bx - lambetams * (Xx * Yx + Xy * Yy);
+            T a13 = cx - lambetams * (Xx * Zx + Xy * Zy);
+            T a21 = ay - lambetams * (Xx * Yx + Xy * Yy);
+            T a22 = by + lambetams * (((Xx * Xx + Zx * Zx) + Xy * Xy) + Zy * Zy);
+            T a23 = cy - lambetams * (Yx * Zx + Yy * Zy);
+            T a31 = az - lambetams * (Xx * Zx + Xy * Zy);
+            T a32 = bz - lambetams * (Yx * Zx + Yy * Zy);
+            T a33 = cz + lambetams * (((Xx * Xx + Yx * Yx) + Xy * Xy) + Yy * Yy);
+
+            T detA = (a11 * (a22 * a33 - a23 * a32) - a12 * (a21 * a33 - a23 * a31)) +
+                     a13 * (a21 * a32 - a22 * a31);
+
+            T E11 = a22 * a33 - a23 * a32, E12 = a13 * a32 - a12 * a33, E13 = a12 * a23 - a13 * a22;
+            T E21 = a23 * a31 - a21 * a33, E22 = a11 * a33 - a13 * a31, E23 = a13 * a21 - a11 * a23;
+            T E31 = a21 * a32 - a22 * a31, E32 = a12 * a31 - a11 * a32, E33 = a11 * a22 - a12 * a21;
+
+            T b1 = lambetams * ((((((Zy * Zy + Yy * Yy) * Xibar + (Zx * Zx + Yx * Yx) * Xjbar) -
+                                  (Xy * Yy) * Yibar) - (Xx * Yx) * Yjbar) - (Xy * Zy) * Zibar) -
+                                (Xx * Zx) * Zjbar) - dx;
+            T b2 = lambetams * (((((-(Xy * Yy) * Xibar - (Xx * Yx) * Xjbar) + (Xy * Xy + Zy * Zy) * Yibar) +
+                                  (Xx * Xx + Zx * Zx) * Yjbar) - (Yy * Zy) * Zibar) -
+                                (Yx * Zx) * Zjbar) - dy;
+            T b3 = lambetams * (((((-(Xy * Zy) * Xibar - (Xx * Zx) * Xjbar) - (Yy * Zy) * Yibar) -
+                                  (Yx * Zx) * Yjbar) + (Xy * Xy + Yy * Yy) * Zibar) +
+                                (Xx * Xx + Yx * Yx) * Zjbar) - dz;
+
+            T Xsub = ((E11 * b1 + E12 * b2) + E13 * b3) / detA;
+            T Ysub = ((E21 * b1 + E22 * b2) + E23 * b3) / detA;
+            T Zsub = ((E31 * b1 + E32 * b2) + E33 * b3) / detA;
+
+            if (std::isfinite(Xsub) && std::isfinite(Ysub) && std::isfinite(Zsub)) {
+                X[pos] = Xsub;
+                Y[pos] = Ysub;
+                Z[pos] = Zsub;
+            } else {
+                X[pos] = X0[pos];
+                Y[pos] = Y0[pos];
+                Z[pos] = Z0[pos];
+            }
+        }
+}
+
+// A16: CleanUp3DKernel (solve3dMinimalArea.cu:248-297).
+template <typename T>
+void cleanup3d(const T* X0, const T* Y0, const T* Z0, const T* Xm, const T* Ym, const T* Zm, int w, int h,
+               int s, T* X1, T* Y1, T* Z1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T X = X0[p], Y = Y0[p], Z = Z0[p], xm = Xm[p], ym = Ym[p], zm = Zm[p];
+            if (std::isfinite(X) && std::isfinite(Y) && std::isfinite(Z) && std::isfinite(xm) &&
+                std::isfinite(ym) && std::isfinite(zm)) {
+                X1[p] = (std::fabs(xm - X) > T(5)) ? xm : X;
+                Y1[p] = (std::fabs(ym - Y) > T(5)) ? ym : Y;
+                Z1[p] = (std::fabs(zm - Z) > T(5)) ? zm : Z;
+            } else {
+                X1[p] = 0;
+                Y1[p] = 0;
+                Z1[p] = 0;
+            }
+        }
+}
+
+// A17: SolveUcKernel (solveUcSk.cu:3-35): the focal length K1[0] serves both axes.
+template <typename T>
+void solve_uc(const T* X0, const T* Y0, const T* Z0, const double* R, const double* t, const double* K1,
+              int w, int h, int s, T* uc, T* vc) {
+    const T R11 = (T)R[0], R12 = (T)R[1], R13 = (T)R[2];
+    const T R21 = (T)R[3], R22 = (T)R[4], R23 = (T)R[5];
+    const T R31 = (T)R[6], R32 = (T)R[7], R33 = (T)R[8];
+    const T t1 = (T)t[0], t2 = (T)t[1], t3 = (T)t[2];
+    const T f = (T)K1[0], cx = (T)K1[2], cy = (T)K1[5];
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T X = X0[p], Y = Y0[p], Z = Z0[p];
+            T X2 = ((R11 * X + R12 * Y) + R13 * Z) + t1;
+            T Y2 = ((R21 * X + R22 * Y) + R23 * Z) + t2;
+            T Z2 = ((R31 * X + R32 * Y) + R33 * Z) + t3;
+            T xproj = (f * X2) / Z2 + cx;
+            T yproj = (f * Y2) / Z2 + cy;
+            uc[p] = xproj - (T)ix;
+            vc[p] = yproj - (T)iy;
+        }
+}
+
+// A18: UpdateSkKernel (solveUcSk.cu:80-97).
+template <typename T>
+void update_sk(const T* sku0, const T* skv0, const T* u0, const T* v0, const T* du0, const T* dv0,
+               const T* uproj0, const T* vproj0, int w, int h, int s, T* sku1, T* skv1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            sku1[p] = ((sku0[p] + u0[p]) + du0[p]) - uproj0[p];
+            skv1[p] = ((skv0[p] + v0[p]) + dv0[p]) - vproj0[p];
+        }
+}
+
+// ---------------------------------------------------------------------------------
+// Host orchestration.
+
+// Parameters of CudaFlow::initialize / ReconFlow::_initializeR (Cudaflow.cu:99-127;
+// Reconflow.cu:27-41). Layout mirrored by the Python wrapper.
+struct Params {
+    int width, height, n_levels;
+    float scale;
+    int method;
+    float lambda, lambdagrad, theta, tau;
+    int n_warp_iters, n_solver_iters;
+    float lambdaf, lambdams, alpha_tv, alpha_proj, alpha_fn;
+    int stride_alignment;
+    int input_type;    // 0: float gray, 1: 8-bit RGB (uchar3), 2: 8-bit gray, 3: 16-bit gray
+    int fix_coarse_k;  // 0: reference intrinsics pyramid (K[2][2] = 0 on levels >= 1); 1: K[2][2] = 1
+    int tri_f64;       // 1: the triangulation step runs in FP64 whatever T is
+};
+
+// Solve3dMinimalArea evaluated in FP64 on T-typed fields (inputs widened, outputs rounded).
+template <typename T>
+void solve3d_wide(const T* up, const T* vp, const T* X0, const T* Y0, const T* Z0, const double* P,
+                  const double* Q, T lambdaf, T lambdams, int w, int h, int s, long n, T* X, T* Y, T* Z) {
+    std::vector<double> a(n), b(n), c(n), d(n), e(n), o1(n), o2(n), o3(n);
+    for (long i = 0; i < n; ++i) {
+        a[i] = up[i];
+        b[i] = vp[i];
+        c[i] = X0[i];
+        d[i] = Y0[i];
+        e[i] = Z0[i];
+    }
+    solve3d<double>(a.data(), b.data(), c.data(), d.data(), e.data(), P, Q, (double)lambdaf, (double)lambdams, w,
+                    h, s, o1.data(), o2.data(), o3.data());
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            X[p] = (T)o1[p];
+            Y[p] = (T)o2[p];
+            Z[p] = (T)o3[p];
+        }
+}
+
+// iAlignUp (Cudaflow.cu:985-994).
+static int align_up(int n, int m) {
+    int mod = n % m;
+    return mod ? n + m - mod : n;
+}
+
+// Pyramid geometry (Cudaflow.cu:146-171): newW = (int)(W / fScale) in float.
+static void pyramid(const Params& P, std::vector<int>& W, std::vector<int>& H, std::vector<int>& S) {
+    W.resize(P.n_levels);
+    H.resize(P.n_levels);
+    S.resize(P.n_levels);
+    int nh = P.height, nw = P.width, ns = align_up(P.width, P.stride_alignment);
+    for (int l = 0; l < P.n_levels; ++l) {
+        W[l] = nw;
+        H[l] = nh;
+        S[l] = ns;
+        nh = (int)((float)nh / P.scale);
+        nw = (int)((float)nw / P.scale);
+        ns = align_up(nw, P.stride_alignment);
+    }
+}
+
+// Flat device-buffer model: every buffer holds s0*h0 elements plus one zero guard row.
+template <typename T>
+struct Pool {
+    long n, guard;
+    std::vector<std::vector<T>> store;
+    Pool(long n_, long guard_) : n(n_), guard(guard_) {}
+    T* make() {
+        store.emplace_back(n + guard, T(0));
+        return store.back().data();
+    }
+};
+
+template <typename T>
+static void zero(T* p, long n) {
+    std::fill(p, p + n, T(0));
+}
+
+template <typename T>
+static void gray_input(const Params& P, const void* in, T* out, int s0) {
+    if (P.input_type == 1) gray_rgb((const uint8_t*)in, out, P.width, P.height, s0);
+    else if (P.input_type == 2) gray_u8((const uint8_t*)in, out, P.width, P.height, s0);
+    else if (P.input_type == 3) gray_u16((const uint16_t*)in, out, P.width, P.height, s0);
+    else gray_f32((const float*)in, out, P.width, P.height, s0);
+}
+
+// A11: CudaFlow::_solveOpticalFlow for METHOD_TVL1 (Cudaflow.cu:403-621), with the
+// pointer swaps mirrored. Images are stride x height, outputs u, v likewise.
+template <typename T>
+int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, float* v_out) {
+    std::vector<int> pW, pH, pS;
+    pyramid(P, pW, pH, pS);
+    const int w0 = pW[0], h0 = pH[0], s0 = pS[0];
+    const long n = (long)s0 * h0;
+    Pool<T> pool(n, s0);
+    std::vector<T*> pI0(P.n_levels), pI1(P.n_levels);
+    for (int l = 0; l < P.n_levels; ++l) {
+        pI0[l] = pool.make();
+        pI1[l] = pool.make();
+    }
+    T *i1warp = pool.make(), *dumed = pool.make(), *dvmed = pool.make(), *dumeds = pool.make(),
+      *dvmeds = pool.make();
+    T *pu1 = pool.make(), *pu2 = pool.make(), *pv1 = pool.make(), *pv2 = pool.make();
+    T *pu1s = pool.make(), *pu2s = pool.make(), *pv1s = pool.make(), *pv2s = pool.make();
+    T *Ix = pool.make(), *Iy = pool.make(), *Iz = pool.make();
+    T *u = pool.make(), *v = pool.make(), *us = pool.make(), *vs = pool.make();
+    (void)w0;
+
+    gray_input(P, i0, pI0[0], s0);
+    gray_input(P, i1, pI1[0], s0);
+    for (int l = 1; l < P.n_levels; ++l) {
+        downscale(pI0[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI0[l]);
+        downscale(pI1[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI1[l]);
+    }
+    zero(u, n);
+    zero(v, n);
+    const T lambda = P.lambda, theta = P.theta, tau = P.tau;
+    for (int level = P.n_levels - 1; level >= 0; --level) {
+        const int w = pW[level], h = pH[level], s = pS[level];
+        for (int wi = 0; wi < P.n_warp_iters; ++wi) {
+            for (T* b : {dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
+            warp(pI1[level], w, h, s, u, v, i1warp);
+            derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            for (int it = 0; it < P.n_solver_iters; ++it) {
+                data_l1(dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta, dumeds, dvmeds);
+                std::swap(dumed, dumeds);
+                std::swap(dvmed, dvmeds);
+                dual_tv(dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, theta, pu1s, pu2s, pv1s, pv2s);
+                std::swap(pu1, pu1s);
+                std::swap(pu2, pu2s);
+                std::swap(pv1, pv1s);
+                std::swap(pv2, pv2s);
+            }
+            median5(dumed, dvmed, w, h, s, dumeds, dvmeds);
+            std::swap(dumed, dumeds);
+            std::swap(dvmed, dvmeds);
+            add(u, dumed, (long)h * s, u);
+            add(v, dvmed, (long)h * s, v);
+        }
+        if (level > 0) {
+            T sc = P.scale;
+            upscale(u, w, h, s, pW[level - 1], pH[level - 1], pS[level - 1], sc, us);
+            upscale(v, w, h, s, pW[level - 1], pH[level - 1], pS[level - 1], sc, vs);
+            std::swap(u, us);
+            std::swap(v, vs);
+        }
+    }
+    for (long i = 0; i < n; ++i) {
+        u_out[i] = (float)u[i];
+        v_out[i] = (float)v[i];
+    }
+    return 0;
+}
+
+// 3x3 (row-major) times [R|t] (3x4): the K*[R|t] product of Reconflow.cu:225-226.
+static void kmul(const double* K, const double* R, const double* t, double* out) {
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0;
+            for (int k = 0; k < 3; ++k) acc += K[i * 3 + k] * R[k * 3 + j];
+            out[i * 4 + j] = acc;
+        }
+        double acc = 0;
+        for (int k = 0; k < 3; ++k) acc += K[i * 3 + k] * t[k];
+        out[i * 4 + 3] = acc;
+    }
+}
+
+// Intrinsics pyramid of ReconFlow::setCameraMatrices (Reconflow.cu:78-116): levels >= 1
+// keep only fx, fy, cx, cy (divided by fScale); K[2][2] stays 0 there.
+// fix_coarse_k = 1 sets K[2][2] = 1 on every level (an option, not the reference).
+static void k_pyramid(const double* K, int n_levels, float scale, int fix_coarse_k,
+                      std::vector<std::vector<double>>& out) {
+    out.assign(n_levels, std::vector<double>(9, 0.0));
+    for (int i = 0; i < 9; ++i) out[0][i] = K[i];
+    for (int l = 1; l < n_levels; ++l) {
+        out[l][0] = out[l - 1][0] / (double)scale;
+        out[l][2] = out[l - 1][2] / (double)scale;
+        out[l][4] = out[l - 1][4] / (double)scale;
+        out[l][5] = out[l - 1][5] / (double)scale;
+        if (fix_coarse_k) out[l][8] = 1.0;
+    }
+}
+
+// A19: ReconFlow::solveReconFlow for METHODR_TVL1_MS (Reconflow.cu:147-466) and, with a
+// prior flow, METHODR_TVL1_MS_FN / _FNSPARSE (Reconflow.cu:232-241, 301-329). Pointer
+// swaps mirrored, so the buffer Solve3d reads is always the one zeroed at solve start.
+template <typename T>
+int solve_recon(const Params& P, const double* K0, const double* K1, const double* R0, const double* t0,
+                const double* R1, const double* t1, const void* i0, const void* i1, const float* ufn_in,
+                const float* vfn_in, const float* mask_in, float* u_out, float* v_out, float* X_out,
+                float* Y_out, float* Z_out) {
+    std::vector<int> pW, pH, pS;
+    pyramid(P, pW, pH, pS);
+    const int h0 = pH[0], s0 = pS[0];
+    const long n = (long)s0 * h0;
+    Pool<T> pool(n, s0);
+    std::vector<T*> pI0(P.n_levels), pI1(P.n_levels);
+    for (int l = 0; l < P.n_levels; ++l) {
+        pI0[l] = pool.make();
+        pI1[l] = pool.make();
+    }
+    T *i1warp = pool.make(), *dumed = pool.make(), *dvmed = pool.make(), *dumeds = pool.make(),
+      *dvmeds = pool.make();
+    T *pu1 = pool.make(), *pu2 = pool.make(), *pv1 = pool.make(), *pv2 = pool.make();
+    T *pu1s = pool.make(), *pu2s = pool.make(), *pv1s = pool.make(), *pv2s = pool.make();
+    T *Ix = pool.make(), *Iy = pool.make(), *Iz = pool.make();
+    T *u = pool.make(), *v = pool.make(), *us = pool.make(), *vs = pool.make();
+    T *X = pool.make(), *Y = pool.make(), *Z = pool.make(), *Xs = pool.make(), *Ys = pool.make(),
+      *Zs = pool.make(), *Xm = pool.make(), *Ym = pool.make(), *Zm = pool.make();
+    T *uproj = pool.make(), *vproj = pool.make(), *uc = pool.make(), *vc = pool.make();
+    T *sku = pool.make(), *skv = pool.make(), *skus = pool.make(), *skvs = pool.make();
+    const bool fn = (P.method == 17 || P.method == 20) && ufn_in && vfn_in;
+    T *ufn = nullptr, *vfn = nullptr, *ufn_l = nullptr, *vfn_l = nullptr, *mask = nullptr, *mask_l = nullptr;
+    if (fn) {
+        ufn = pool.make();
+        vfn = pool.make();
+        ufn_l = pool.make();
+        vfn_l = pool.make();
+        mask = pool.make();
+        mask_l = pool.make();
+        for (long i = 0; i < n; ++i) {
+            ufn[i] = ufn_in[i];
+            vfn[i] = vfn_in[i];
+            mask[i] = mask_in ? mask_in[i] : 1.0f;
+        }
+    }
+
+    std::vector<std::vector<double>> pK0, pK1;
+    k_pyramid(K0, P.n_levels, P.scale, P.fix_coarse_k, pK0);
+    k_pyramid(K1, P.n_levels, P.scale, P.fix_coarse_k, pK1);
+
+    gray_input(P, i0, pI0[0], s0);
+    gray_input(P, i1, pI1[0], s0);
+    for (int l = 1; l < P.n_levels; ++l) {
+        downscale(pI0[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI0[l]);
+        downscale(pI1[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI1[l]);
+    }
+    for (T* b : {u, v, uproj, vproj, uc, vc, X, Y, Z, sku, skv}) zero(b, n);
+
+    const T lambda = P.lambda, tau = P.tau, alphaTv = P.alpha_tv, alphaProj = P.alpha_proj;
+    const T thetaTv = (T)(1.0f / P.alpha_tv);  // Reconflow.cu:32
+    const T lambdaf = P.lambdaf, lambdams = P.lambdams, alphaFn = P.alpha_fn;
+    for (int level = P.n_levels - 1; level >= 0; --level) {
+        const int w = pW[level], h = pH[level], s = pS[level];
+        double Pm[12], Qm[12];
+        kmul(pK0[level].data(), R0, t0, Pm);
+        kmul(pK1[level].data(), R1, t1, Qm);
+        if (fn) {
+            float sc = (float)pW[level] / (float)pW[0];
+            downscale(ufn, pW[0], pH[0], pS[0], w, h, s, (T)sc, true, ufn_l);
+            downscale(vfn, pW[0], pH[0], pS[0], w, h, s, (T)sc, true, vfn_l);
+            downscale(mask, pW[0], pH[0], pS[0], w, h, s, T(1), false, mask_l);
+        }
+        for (int wi = 0; wi < P.n_warp_iters; ++wi) {
+            for (T* b : {dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
+            warp(pI1[level], w, h, s, u, v, i1warp);
+            derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            for (int it = 0; it < P.n_solver_iters; ++it) {
+                if (fn)
+                    recon_data_l1_fn(u, v, dumed, dvmed, uproj, vproj, ufn_l, vfn_l, sku, skv, pu1, pu2, pv1,
+                                     pv2, Ix, Iy, Iz, w, h, s, lambda, alphaTv, alphaProj, alphaFn,
+                                     P.method == 20 ? mask_l : nullptr, dumeds, dvmeds);
+                else
+                    recon_data_l1(u, v, dumed, dvmed, uproj, vproj, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz,
+                                  w, h, s, lambda, alphaTv, alphaProj, dumeds, dvmeds);
+                std::swap(dumed, dumeds);
+                std::swap(dvmed, dvmeds);
+                dual_tv(dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, thetaTv, pu1s, pu2s, pv1s, pv2s);
+                std::swap(pu1, pu1s);
+                std::swap(pu2, pu2s);
+                std::swap(pv1, pv1s);
+                std::swap(pv2, pv2s);
+                uvproj(u, v, dumed, dvmed, uc, vc, sku, skv, w, h, s, lambdaf, alphaProj, uproj, vproj);
+                if (P.tri_f64)
+                    solve3d_wide(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, n + s0, Xs, Ys, Zs);
+                else
+                    solve3d(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, Xs, Ys, Zs);
+                std::swap(X, Xs);
+                std::swap(Y, Ys);
+                std::swap(Z, Zs);
+                median3d(X, Y, Z, w, h, s, Xm, Ym, Zm);
+                cleanup3d(X, Y, Z, Xm, Ym, Zm, w, h, s, Xs, Ys, Zs);
+                std::swap(X, Xs);
+                std::swap(Y, Ys);
+                std::swap(Z, Zs);
+                solve_uc(X, Y, Z, R1, t1, pK1[level].data(), w, h, s, uc, vc);
+                update_sk(sku, skv, u, v, dumed, dvmed, uproj, vproj, w, h, s, skus, skvs);
+                std::swap(sku, skus);
+                std::swap(skv, skvs);
+            }
+            median5(dumed, dvmed, w, h, s, dumeds, dvmeds);
+            std::swap(dumed, dumeds);
+            std::swap(dvmed, dvmeds);
+            add(u, dumed, (long)h * s, u);
+            add(v, dvmed, (long)h * s, v);
+            median5(u, v, w, h, s, dumeds, dvmeds);
+            std::swap(u, dumeds);
+            std::swap(v, dvmeds);
+        }
+        if (level > 0) {
+            const int W = pW[level - 1], H = pH[level - 1], S = pS[level - 1];
+            T sc = P.scale;
+            upscale(u, w, h, s, W, H, S, sc, us);
+            upscale(v, w, h, s, W, H, S, sc, vs);
+            std::swap(u, us);
+            std::swap(v, vs);
+            upscale(uc, w, h, s, W, H, S, sc, us);
+            upscale(vc, w, h, s, W, H, S, sc, vs);
+            std::swap(uc, us);
+            std::swap(vc, vs);
+            upscale(uproj, w, h, s, W, H, S, sc, us);
+            upscale(vproj, w, h, s, W, H, S, sc, vs);
+            std::swap(uproj, us);
+            std::swap(vproj, vs);
+            upscale(sku, w, h, s, W, H, S, sc, us);
+            upscale(skv, w, h, s, W, H, S, sc, vs);
+            std::swap(sku, us);
+            std::swap(skv, vs);
+        }
+    }
+    for (long i = 0; i < n; ++i) {
+        u_out[i] = (float)u[i];
+        v_out[i] = (float)v[i];
+        X_out[i] = (float)X[i];
+        Y_out[i] = (float)Y[i];
+        Z_out[i] = (float)Z[i];
+    }
+    return 0;
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------------
+// C ABI for the Python test harness (ctypes). Suffix _f64 / _f32 selects T.
+extern "C" {
+
+void orc_gray_rgb_f64(const uint8_t* rgb, double* out, int w, int h, int s) { orc::gray_rgb(rgb, out, w, h, s); }
+void orc_gray_rgb_f32(const uint8_t* rgb, float* out, int w, int h, int s) { orc::gray_rgb(rgb, out, w, h, s); }
+void orc_gray_u8_f64(const uint8_t* in, double* out, int w, int h, int s) { orc::gray_u8(in, out, w, h, s); }
+void orc_gray_u8_f32(const uint8_t* in, float* out, int w, int h, int s) { orc::gray_u8(in, out, w, h, s); }
+void orc_gray_u16_f64(const uint16_t* in, double* out, int w, int h, int s) { orc::gray_u16(in, out, w, h, s); }
+void orc_gray_u16_f32(const uint16_t* in, float* out, int w, int h, int s) { orc::gray_u16(in, out, w, h, s); }
+
+#define DEF2(T, SUF)                                                                                        \
+    void orc_downscale_##SUF(const T* src, int W, int H, int S, int w, int h, int s, T* out) {               \
+        orc::downscale<T>(src, W, H, S, w, h, s, T(1), false, out);                                          \
+    }                                                                                                        \
+    void orc_downscale_scaled_##SUF(const T* src, int W, int H, int S, int w, int h, int s, float scale,     \
+                                    T* out) {                                                                \
+        orc::downscale<T>(src, W, H, S, w, h, s, (T)scale, true, out);                                       \
+    }                                                                                                        \
+    void orc_upscale_##SUF(const T* src, int w, int h, int s, int W, int H, int S, float scale, T* out) {    \
+        orc::upscale<T>(src, w, h, s, W, H, S, (T)scale, out);                                               \
+    }                                                                                                        \
+    void orc_warp_##SUF(const T* src, int w, int h, int s, const T* u, const T* v, T* out) {                 \
+        orc::warp<T>(src, w, h, s, u, v, out);                                                               \
+    }                                                                                                        \
+    void orc_derivatives_##SUF(const T* I0, const T* I1, int w, int h, int s, T* Ix, T* Iy, T* Iz) {         \
+        orc::derivatives<T>(I0, I1, w, h, s, Ix, Iy, Iz);                                                    \
+    }                                                                                                        \
+    void orc_data_l1_##SUF(const T* du, const T* dv, const T* pu1, const T* pu2, const T* pv1, const T* pv2, \
+                           const T* Ix, const T* Iy, const T* Iz, int w, int h, int s, float lambda,         \
+                           float theta, T* du1, T* dv1) {                                                    \
+        orc::data_l1<T>(du, dv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, (T)lambda, (T)theta, du1, dv1);     \
+    }                                                                                                        \
+    void orc_dual_tv_##SUF(const T* du, const T* dv, const T* pu1, const T* pu2, const T* pv1, const T* pv2, \
+                           int w, int h, int s, float tau, float theta, T* a, T* b, T* c, T* d) {            \
+        orc::dual_tv<T>(du, dv, pu1, pu2, pv1, pv2, w, h, s, (T)tau, (T)theta, a, b, c, d);                  \
+    }                                                                                                        \
+    void orc_median5_##SUF(const T* u, const T* v, int w, int h, int s, T* ou, T* ov) {                      \
+        orc::median5<T>(u, v, w, h, s, ou, ov);                                                              \
+    }                                                                                                        \
+    void orc_median3d_##SUF(const T* X, const T* Y, const T* Z, int w, int h, int s, T* a, T* b, T* c) {     \
+        orc::median3d<T>(X, Y, Z, w, h, s, a, b, c);                                                         \
+    }                                                                                                        \
+    void orc_add_##SUF(const T* a, const T* b, long count, T* out) { orc::add<T>(a, b, count, out); }        \
+    void orc_recon_data_l1_##SUF(const T* u, const T* v, const T* du, const T* dv, const T* up, const T* vp, \
+                                 const T* sku, const T* skv, const T* pu1, const T* pu2, const T* pv1,       \
+                                 const T* pv2, const T* Ix, const T* Iy, const T* Iz, int w, int h, int s,   \
+                                 float lambda, float aTv, float aP, T* du1, T* dv1) {                        \
+        orc::recon_data_l1<T>(u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s,      \
+                              (T)lambda, (T)aTv, (T)aP, du1, dv1);                                           \
+    }                                                                                                        \
+    void orc_recon_data_l1_fn_##SUF(const T* u, const T* v, const T* du, const T* dv, const T* up,           \
+                                    const T* vp, const T* ufn, const T* vfn, const T* sku, const T* skv,     \
+                                    const T* pu1, const T* pu2, const T* pv1, const T* pv2, const T* Ix,     \
+                                    const T* Iy, const T* Iz, int w, int h, int s, float lambda, float aTv,  \
+                                    float aP, float aFn, const T* mask, T* du1, T* dv1) {                    \
+        orc::recon_data_l1_fn<T>(u, v, du, dv, up, vp, ufn, vfn, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz,  \
+                                 w, h, s, (T)lambda, (T)aTv, (T)aP, (T)aFn, mask, du1, dv1);                 \
+    }                                                                                                        \
+    void orc_uvproj_##SUF(const T* u, const T* v, const T* du, const T* dv, const T* uc, const T* vc,        \
+                          const T* sku, const T* skv, int w, int h, int s, float lf, float aP, T* up, T* vp) {\
+        orc::uvproj<T>(u, v, du, dv, uc, vc, sku, skv, w, h, s, (T)lf, (T)aP, up, vp);                      \
+    }                                                                                                        \
+    void orc_solve3d_##SUF(const T* up, const T* vp, const T* X0, const T* Y0, const T* Z0, const double* P,  \
+                           const double* Q, float lf, float lms, int w, int h, int s, T* X, T* Y, T* Z) {    \
+        orc::solve3d<T>(up, vp, X0, Y0, Z0, P, Q, (T)lf, (T)lms, w, h, s, X, Y, Z);                          \
+    }                                                                                                        \
+    void orc_cleanup3d_##SUF(const T* X0, const T* Y0, const T* Z0, const T* Xm, const T* Ym, const T* Zm,   \
+                             int w, int h, int s, T* X1, T* Y1, T* Z1) {                                     \
+        orc::cleanup3d<T>(X0, Y0, Z0, Xm, Ym, Zm, w, h, s, X1, Y1, Z1);                                      \
+    }                                                                                                        \
+    void orc_solve_uc_##SUF(const T* X, const T* Y, const T* Z, const double* R, const double* t,            \
+                            const double* K1, int w, int h, int s, T* uc, T* vc) {                           \
+        orc::solve_uc<T>(X, Y, Z, R, t, K1, w, h, s, uc, vc);                                                \
+    }                                                                                                        \
+    void orc_update_sk_##SUF(const T* sku, const T* skv, const T* u, const T* v, const T* du, const T* dv,   \
+                             const T* up, const T* vp, int w, int h, int s, T* sku1, T* skv1) {              \
+        orc::update_sk<T>(sku, skv, u, v, du, dv, up, vp, w, h, s, sku1, skv1);                              \
+    }                                                                                                        \
+    int orc_solve_flow_##SUF(const orc::Params* P, const void* i0, const void* i1, float* u, float* v) {     \
+        return orc::solve_flow<T>(*P, i0, i1, u, v);                                                         \
+    }                                                                                                        \
+    int orc_solve_recon_##SUF(const orc::Params* P, const double* K0, const double* K1, const double* R0,    \
+                              const double* t0, const double* R1, const double* t1, const void* i0,          \
+                              const void* i1, const float* ufn, const float* vfn, const float* mask,         \
+                              float* u, float* v, float* X, float* Y, float* Z) {                            \
+        return orc::solve_recon<T>(*P, K0, K1, R0, t0, R1, t1, i0, i1, ufn, vfn, mask, u, v, X, Y, Z);      \
+    }
+
+DEF2(double, f64)
+DEF2(float, f32)
+
+void orc_pyramid(const orc::Params* P, int* W, int* H, int* S) {
+    std::vector<int> pW, pH, pS;
+    orc::pyramid(*P, pW, pH, pS);
+    for (int l = 0; l < P->n_levels; ++l) {
+        W[l] = pW[l];
+        H[l] = pH[l];
+        S[l] = pS[l];
+    }
+}
+
+int orc_params_size(void) { return (int)sizeof(orc::Params); }
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+"""In-tree build of the native libraries.
+
+* ``reconflow_amd/_build/librfhip.so``   — the HIP solver (gfx950) behind ``include/rfhip.h``;
+* ``reconflow_amd/_build/librfsynth.so`` — the host-side seeded pair generator.
+
+Both are compiled every time :func:`build_all` runs. HIP code is built with contraction off and
+IEEE division / square root so that the fused and the per-step kernels agree bit for bit, and
+with packed FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+OUT = PKG / "_build"
+
+HIP_SOURCES = ["rf_kernels.hip", "rf_fused.hip", "rf_solver.hip", "rf_capi_kernels.hip", "rf_batch.hip"]
+ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-result",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found; the HIP solver cannot be built")
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> None:
+    proc = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
+
+
+def build_hip(jobs: int = 8) -> Path:
+    """Compile every HIP source for gfx950 and link librfhip.so."""
+    OUT.mkdir(exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    procs = []
+    for src in HIP_SOURCES:
+        obj = OUT / (Path(src).stem + ".o")
+        objs.append(obj)
+        cmd = [hipcc, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        if len(procs) >= jobs:
+            _drain(procs)
+    _drain(procs)
+    lib = OUT / "librfhip.so"
+    tmp = OUT / "librfhip.so.tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    tmp.replace(lib)
+    return lib
+
+
+def _drain(procs: list) -> None:
+    while procs:
+        cmd, p = procs.pop(0)
+        out, err = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"hipcc failed: {' '.join(cmd)}\n{out}\n{err}")
+
+
+def build_synth() -> Path:
+    """Compile the host-only pair generator."""
+    OUT.mkdir(exist_ok=True)
+    lib = OUT / "librfsynth.so"
+    tmp = OUT / "librfsynth.so.tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", str(CSRC / "synth.cpp"), "-o", str(tmp)])
+    tmp.replace(lib)
+    return lib
+
+
+def build_all() -> None:
+    build_synth()
+    build_hip()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", OUT)

@@ -1,0 +1,349 @@
+// Per-pixel device arithmetic of the TV-L1 / joint flow-depth solver for gfx950.
+//
+// Each function evaluates one reference kernel's formula for one pixel, in the reference's
+// operation order; the library is compiled with -ffp-contract=off and IEEE division and
+// square root, so the plain and the fused kernels (which share these functions) produce
+// identical FP32 results. Citations are to the reference kernels each one follows.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rf {
+
+// Mirror addressing with edge duplication (texture Mirror mode, e.g. warpImage.cu:40-43):
+// period 2n, -1 -> 0, n -> n-1.
+__device__ __forceinline__ int refl(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;
+    int p = 2 * n;
+    int m = i % p;
+    if (m < 0) m += p;
+    return m < n ? m : p - 1 - m;
+}
+
+__device__ __forceinline__ int floor_div(int a, int b) {  // b > 0
+    int q = a / b;
+    if ((a % b != 0) && (a < 0)) --q;
+    return q;
+}
+
+// Bilinear texel blend with the texture-fetch weight formula
+// (1-a)(1-b)T00 + a(1-b)T10 + (1-a)bT01 + abT11, summed left to right.
+__device__ __forceinline__ float blend(float t00, float t10, float t01, float t11, float fx, float fy) {
+    float w00 = (1.0f - fx) * (1.0f - fy);
+    float w10 = fx * (1.0f - fy);
+    float w01 = (1.0f - fx) * fy;
+    float w11 = fx * fy;
+    float r = w00 * t00;
+    r = r + w10 * t10;
+    r = r + w01 * t01;
+    r = r + w11 * t11;
+    return r;
+}
+
+template <typename TI>
+__device__ __forceinline__ float ld(const TI* p, long i) {
+    return (float)p[i];
+}
+
+template <typename TI>
+__device__ __forceinline__ float bilin(const TI* img, int w, int h, int s, int x0, float fx, int y0, float fy) {
+    int xa = refl(x0, w), xb = refl(x0 + 1, w), ya = refl(y0, h), yb = refl(y0 + 1, h);
+    return blend(ld(img, xa + (long)ya * s), ld(img, xb + (long)ya * s), ld(img, xa + (long)yb * s),
+                 ld(img, xb + (long)yb * s), fx, fy);
+}
+
+// Sample at the rational texel coordinate (nx/dx, ny/dy): exact integer split, weight
+// fractions rounded once.
+template <typename TI>
+__device__ __forceinline__ float sample_ratio(const TI* img, int w, int h, int s, int nx, int dx, int ny, int dy) {
+    int x0 = floor_div(nx, dx), y0 = floor_div(ny, dy);
+    float fx = (float)(nx - x0 * dx) / (float)dx;
+    float fy = (float)(ny - y0 * dy) / (float)dy;
+    return bilin(img, w, h, s, x0, fx, y0, fy);
+}
+
+// Warp sample S(I, ix + u, iy + v) (warpImage.cu:25-28): the fraction u - floor(u) is exact.
+template <typename TI>
+__device__ __forceinline__ float warp_sample(const TI* img, int w, int h, int s, int ix, int iy, float u, float v) {
+    if (!(fabsf(u) <= 16777216.0f) || !(fabsf(v) <= 16777216.0f)) return __builtin_nanf("");
+    float fu = floorf(u), fv = floorf(v);
+    return bilin(img, w, h, s, ix + (int)fu, u - fu, iy + (int)fv, v - fv);
+}
+
+// 5-point derivative (computeDerivatives.cu:26-30): ((a - 8b) + 8c - d) / 12.
+__device__ __forceinline__ float d5(float m2, float m1, float p1, float p2) {
+    float t = m2;
+    t -= m1 * 8.0f;
+    t += p1 * 8.0f;
+    t -= p2;
+    t /= 12.0f;
+    return t;
+}
+
+// Divergence with the reference's left/top border rule (solveDataL1.cu:64-88). pl = p1 at
+// x-1, pd = p2 at y-1 (ignored on the first column / row).
+__device__ __forceinline__ float div_p(float p1, float p1l, float p2, float p2d, bool has_left, bool has_up) {
+    if (!has_left) {
+        if (!has_up) return p1 + p2;
+        return (p1 + p2) - p2d;
+    }
+    if (!has_up) return (p1 - p1l) + p2;
+    return ((p1 - p1l) + p2) - p2d;
+}
+
+// L1 data threshold (solveDataL1.cu:27-51 with lt = lambda*theta; solveReconDataL1.cu:38-63
+// with the division by A applied to upper/lower/steps). A NaN rho yields NaN.
+__device__ __forceinline__ void l1_threshold(float dix, float diy, float dit, float bu, float bv, float lambda,
+                                             float theta, float& du, float& dv) {
+    float rho = (dix * bu + diy * bv) + dit;
+    float lt = lambda * theta;
+    float g2 = dix * dix + diy * diy;
+    float upper = lt * g2;
+    float lower = ((-lambda) * theta) * g2;
+    du = __builtin_nanf("");
+    dv = du;
+    if ((rho <= upper) && (rho >= lower)) {
+        float magi = dix * dix + diy * diy;
+        if (magi != 0.0f) {
+            du = bu - (rho * dix) / magi;
+            dv = bv - (rho * diy) / magi;
+        } else {
+            du = bu;
+            dv = bv;
+        }
+    } else if (rho < lower) {
+        du = bu + lt * dix;
+        dv = bv + lt * diy;
+    } else if (rho > upper) {
+        du = bu - lt * dix;
+        dv = bv - lt * diy;
+    }
+}
+
+__device__ __forceinline__ void l1_threshold_div(float dix, float diy, float dit, float bu, float bv, float lambda,
+                                                 float A, float& du, float& dv) {
+    float rho = (dix * bu + diy * bv) + dit;
+    float g2 = dix * dix + diy * diy;
+    float upper = (lambda * g2) / A;
+    float lower = ((-lambda) * g2) / A;
+    du = __builtin_nanf("");
+    dv = du;
+    if ((rho <= upper) && (rho >= lower)) {
+        float magi = dix * dix + diy * diy;
+        if (magi != 0.0f) {
+            du = bu - (rho * dix) / magi;
+            dv = bv - (rho * diy) / magi;
+        } else {
+            du = bu;
+            dv = bv;
+        }
+    } else if (rho < lower) {
+        du = bu + (lambda * dix) / A;
+        dv = bv + (lambda * diy) / A;
+    } else if (rho > upper) {
+        du = bu - (lambda * dix) / A;
+        dv = bv - (lambda * diy) / A;
+    }
+}
+
+// Joint data blend (solveReconDataL1.cu:35-36): (aTv*uhat + aP*((uproj - u) - sk)) / A.
+__device__ __forceinline__ float recon_blend(float aTv, float aP, float A, float uhat, float uproj, float u, float sk) {
+    return (aTv * uhat + aP * ((uproj - u) - sk)) / A;
+}
+
+// With a prior flow (solveReconDataL1.cu:172-173).
+__device__ __forceinline__ float recon_blend_fn(float aTv, float aP, float aFn, float A, float uhat, float uproj,
+                                                float u, float sk, float ufn) {
+    return ((aFn * (ufn - u) + aTv * uhat) + aP * ((uproj - u) - sk)) / A;
+}
+
+// Dual TV step for one component pair (solveSmoothDualTVGlobal.cu:24-56).
+__device__ __forceinline__ void dual_px(float ux, float uy, float p1, float p2, float fac, float& o1, float& o2) {
+    float mag = sqrtf(ux * ux + uy * uy);
+    o1 = (p1 + fac * ux) / (1.0f + fac * mag);
+    o2 = (p2 + fac * uy) / (1.0f + fac * mag);
+}
+
+// UVProj blend (solveUVProj.cu:19-20).
+__device__ __forceinline__ float uvproj_px(float lf, float aP, float uc, float u, float du, float sk) {
+    return (lf * uc + aP * ((u + du) + sk)) / (lf + aP);
+}
+
+// Sk update (solveUcSk.cu:93-94).
+__device__ __forceinline__ float sk_px(float sk, float u, float du, float uproj) {
+    return ((sk + u) + du) - uproj;
+}
+
+// Pinhole reprojection into camera 1 minus the pixel (solveUcSk.cu:21-33).
+struct UcParams {
+    float R11, R12, R13, R21, R22, R23, R31, R32, R33, t1, t2, t3, f, cx, cy;
+};
+__device__ __forceinline__ void uc_px(const UcParams& c, float X, float Y, float Z, int ix, int iy, float& uc,
+                                      float& vc) {
+    float X2 = ((c.R11 * X + c.R12 * Y) + c.R13 * Z) + c.t1;
+    float Y2 = ((c.R21 * X + c.R22 * Y) + c.R23 * Z) + c.t2;
+    float Z2 = ((c.R31 * X + c.R32 * Y) + c.R33 * Z) + c.t3;
+    float xproj = (c.f * X2) / Z2 + c.cx;
+    float yproj = (c.f * Y2) / Z2 + c.cy;
+    uc = xproj - (float)ix;
+    vc = yproj - (float)iy;
+}
+
+// Camera matrices of Solve3dMinimalArea (solve3dMinimalArea.cu:204-228): cast to float for
+// the reference arithmetic, kept in double for the FP64 triangulation.
+template <typename R>
+struct PQT {
+    R P[12], Q[12];
+};
+using PQ = PQT<float>;
+using PQd = PQT<double>;
+
+// Solve3dMinimalAreaKernel for one pixel (solve3dMinimalArea.cu:21-185), evaluated in R (float
+// as the reference, or double). The neighbour values are passed in: r = X0 at ix+1, d = X0 at
+// iy+1, l = X0 at ix-1 (used when has_left), u = X0 at iy-1 (used when has_up). Column w and
+// row h are legitimately read by the caller (the reference's border guards never fire).
+template <typename R>
+__device__ __forceinline__ void solve3d_px(const PQT<R>& m, R lambdaf, R lambdams, int ix, int iy, float uproj_f,
+                                           float vproj_f, float X0f, float Y0f, float Z0f, float Xrf, float Yrf,
+                                           float Zrf, float Xdf, float Ydf, float Zdf, float Xlf, float Ylf, float Zlf,
+                                           float Xuf, float Yuf, float Zuf, bool has_left, bool has_up, float& oX,
+                                           float& oY, float& oZ) {
+    const R* P = m.P;
+    const R* Q = m.Q;
+    const R X0 = X0f, Y0 = Y0f, Z0 = Z0f, Xr = Xrf, Yr = Yrf, Zr = Zrf, Xd = Xdf, Yd = Ydf, Zd = Zdf;
+    const R Xl = Xlf, Yl = Ylf, Zl = Zlf, Xu = Xuf, Yu = Yuf, Zu = Zuf;
+    R x1 = (R)ix, y1 = (R)iy;
+    R x2 = (R)ix + (R)uproj_f;
+    R y2 = (R)iy + (R)vproj_f;
+    R p1 = x1 * P[8] - P[0], p2 = x1 * P[9] - P[1], p3 = x1 * P[10] - P[2], p4 = x1 * P[11] - P[3];
+    R q1 = y1 * P[8] - P[4], q2 = y1 * P[9] - P[5], q3 = y1 * P[10] - P[6], q4 = y1 * P[11] - P[7];
+    R r1 = x2 * Q[8] - Q[0], r2 = x2 * Q[9] - Q[1], r3 = x2 * Q[10] - Q[2], r4 = x2 * Q[11] - Q[3];
+    R s1 = y2 * Q[8] - Q[4], s2 = y2 * Q[9] - Q[5], s3 = y2 * Q[10] - Q[6], s4 = y2 * Q[11] - Q[7];
+#define RF_DOT4(a1, b1, a2, b2, a3, b3, a4, b4) ((((a1) * (b1) + (a2) * (b2)) + (a3) * (b3)) + (a4) * (b4))
+    R ax = lambdaf * RF_DOT4(p1, p1, q1, q1, r1, r1, s1, s1);
+    R bx = lambdaf * RF_DOT4(p1, p2, q1, q2, r1, r2, s1, s2);
+    R cx = lambdaf * RF_DOT4(p1, p3, q1, q3, r1, r3, s1, s3);
+    R dx = lambdaf * RF_DOT4(p1, p4, q1, q4, r1, r4, s1, s4);
+    R ay = lambdaf * RF_DOT4(p2, p1, q2, q1, r2, r1, s2, s1);
+    R by = lambdaf * RF_DOT4(p2, p2, q2, q2, r2, r2, s2, s2);
+    R cy = lambdaf * RF_DOT4(p2, p3, q2, q3, r2, r3, s2, s3);
+    R dy = lambdaf * RF_DOT4(p2, p4, q2, q4, r2, r4, s2, s4);
+    R az = lambdaf * RF_DOT4(p3, p1, q3, q1, r3, r1, s3, s1);
+    R bz = lambdaf * RF_DOT4(p3, p2, q3, q2, r3, r2, s3, s2);
+    R cz = lambdaf * RF_DOT4(p3, p3, q3, q3, r3, r3, s3, s3);
+    R dz = lambdaf * RF_DOT4(p3, p4, q3, q4, r3, r4, s3, s4);
+#undef RF_DOT4
+    R Xx = Xr - X0, Yx = Yr - Y0, Zx = Zr - Z0;
+    R Xy = Xd - X0, Yy = Yd - Y0, Zy = Zd - Z0;
+    R c1 = Yx * Zy - Zx * Yy;
+    R c2 = Zx * Xy - Xx * Zy;
+    R c3 = Xx * Yy - Yx * Xy;
+    R lambetams = lambdams / sqrt(((c1 * c1 + c2 * c2) + c3 * c3) + (R)0.001f);
+    R Xibar, Yibar, Zibar, Xjbar, Yjbar, Zjbar;
+    const R half = (R)0.5;
+    if (!has_left) {
+        Xibar = Xr;
+        Yibar = Yr;
+        Zibar = Zr;
+    } else {
+        Xibar = half * Xl + half * Xr;
+        Yibar = half * Yl + half * Yr;
+        Zibar = half * Zl + half * Zr;
+    }
+    if (!has_up) {
+        Xjbar = Xd;
+        Yjbar = Yd;
+        Zjbar = Zd;
+    } else {
+        Xjbar = half * Xu + half * Xd;
+        Yjbar = half * Yu + half * Yd;
+        Zjbar = half * Zu + half * Zd;
+    }
+    R a11 = ax + lambetams * (((Yx * Yx + Zx * Zx) + Yy * Yy) + Zy * Zy);
+    R a12 = bx - lambetams * (Xx * Yx + Xy * Yy);
+    R a13 = cx - lambetams * (Xx * Zx + Xy * Zy);
+    R a21 = ay - lambetams * (Xx * Yx + Xy * Yy);
+    R a22 = by + lambetams * (((Xx * Xx + Zx * Zx) + Xy * Xy) + Zy * Zy);
+    R a23 = cy - lambetams * (Yx * Zx + Yy * Zy);
+    R a31 = az - lambetams * (Xx * Zx + Xy * Zy);
+    R a32 = bz - lambetams * (Yx * Zx + Yy * Zy);
+    R a33 = cz + lambetams * (((Xx * Xx + Yx * Yx) + Xy * Xy) + Yy * Yy);
+    R detA = (a11 * (a22 * a33 - a23 * a32) - a12 * (a21 * a33 - a23 * a31)) + a13 * (a21 * a32 - a22 * a31);
+    R E11 = a22 * a33 - a23 * a32, E12 = a13 * a32 - a12 * a33, E13 = a12 * a23 - a13 * a22;
+    R E21 = a23 * a31 - a21 * a33, E22 = a11 * a33 - a13 * a31, E23 = a13 * a21 - a11 * a23;
+    R E31 = a21 * a32 - a22 * a31, E32 = a12 * a31 - a11 * a32, E33 = a11 * a22 - a12 * a21;
+    R b1 = lambetams * ((((((Zy * Zy + Yy * Yy) * Xibar + (Zx * Zx + Yx * Yx) * Xjbar) - (Xy * Yy) * Yibar) -
+                          (Xx * Yx) * Yjbar) - (Xy * Zy) * Zibar) - (Xx * Zx) * Zjbar) - dx;
+    R b2 = lambetams * (((((-(Xy * Yy) * Xibar - (Xx * Yx) * Xjbar) + (Xy * Xy + Zy * Zy) * Yibar) +
+                          (Xx * Xx + Zx * Zx) * Yjbar) - (Yy * Zy) * Zibar) - (Yx * Zx) * Zjbar) - dy;
+    R b3 = lambetams * (((((-(Xy * Zy) * Xibar - (Xx * Zx) * Xjbar) - (Yy * Zy) * Yibar) - (Yx * Zx) * Yjbar) +
+                          (Xy * Xy + Yy * Yy) * Zibar) + (Xx * Xx + Yx * Yx) * Zjbar) - dz;
+    R Xs = ((E11 * b1 + E12 * b2) + E13 * b3) / detA;
+    R Ys = ((E21 * b1 + E22 * b2) + E23 * b3) / detA;
+    R Zs = ((E31 * b1 + E32 * b2) + E33 * b3) / detA;
+    if (isfinite(Xs) && isfinite(Ys) && isfinite(Zs)) {
+        oX = (float)Xs;
+        oY = (float)Ys;
+        oZ = (float)Zs;
+    } else {
+        oX = X0f;
+        oY = Y0f;
+        oZ = Z0f;
+    }
+}
+
+// CleanUp3DKernel for one pixel (solve3dMinimalArea.cu:258-295).
+__device__ __forceinline__ void cleanup_px(float X, float Y, float Z, float xm, float ym, float zm, float& oX,
+                                           float& oY, float& oZ) {
+    if (isfinite(X) && isfinite(Y) && isfinite(Z) && isfinite(xm) && isfinite(ym) && isfinite(zm)) {
+        oX = (fabsf(xm - X) > 5.0f) ? xm : X;
+        oY = (fabsf(ym - Y) > 5.0f) ? ym : Y;
+        oZ = (fabsf(zm - Z) > 5.0f) ? zm : Z;
+    } else {
+        oX = 0.0f;
+        oY = 0.0f;
+        oZ = 0.0f;
+    }
+}
+
+// Partial exchange sort of the reference median kernels (medianFilter.cu:52-71, 218-239):
+// for j < NSEL, for i > j, swap when m[j] > m[i]; the answer is m[NSEL-1]. Fully unrolled so
+// the array lives in registers.
+template <int N, int NSEL>
+__device__ __forceinline__ float exchange_select(float (&m)[N]) {
+#pragma unroll
+    for (int j = 0; j < NSEL; ++j) {
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) {
+            float a = m[j], b = m[i];
+            bool sw = a > b;
+            m[j] = sw ? b : a;
+            m[i] = sw ? a : b;
+        }
+    }
+    return m[NSEL - 1];
+}
+
+// Tap index of the 5x5 flow median (medianFilter.cu:31-50); -1 = tap keeps its 0.
+__device__ __forceinline__ long median5_tap(int col, int row, int w, int h, int s) {
+    if (col >= 0 && col < w && row >= 0 && row < h) return col + (long)s * row;
+    if (col < 0 && row >= 0 && row < h) return (long)s * row;
+    if (col >= w && row >= 0 && row < h) return w - 1 + (long)s * row;
+    if (col >= 0 && col < w && row < 0) return col;
+    if (col >= 0 && col < w && row >= h) return col + (long)s * (h - 1);
+    return -1;
+}
+
+// Tap index of the 3x3 XYZ median (medianFilter.cu:193-214): the right/bottom replacements
+// test col > w / row > h, so col == w and row == h keep 0.
+__device__ __forceinline__ long median3_tap(int col, int row, int w, int h, int s) {
+    if (col >= 0 && col < w && row >= 0 && row < h) return col + (long)s * row;
+    if (col < 0 && row >= 0 && row < h) return (long)s * row;
+    if (col > w && row >= 0 && row < h) return w - 1 + (long)s * row;
+    if (col >= 0 && col < w && row < 0) return col;
+    if (col >= 0 && col < w && row > h) return col + (long)s * (h - 1);
+    return -1;
+}
+
+}  // namespace rf

@@ -1,0 +1,515 @@
+// Per-step gfx950 kernels: one kernel per reference kernel, one thread per pixel.
+//
+// Workgroups are 64 x 4 threads: one wave64 spans 64 consecutive pixels of a row, so every
+// field access of a wave is one contiguous 256-byte segment. These kernels are the plain
+// path that the fused path is checked against; each evaluates the per-pixel functions of
+// rf_device.hpp, so both paths share one arithmetic definition.
+
+#include <math.h>
+
+#include "rf_kernels.hpp"
+
+namespace rf {
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 grid2(int w, int h) { return dim3((w + BX - 1) / BX, (h + BY - 1) / BY); }
+
+#define RF_PIX                                          \
+    const int ix = blockIdx.x * BX + threadIdx.x;       \
+    const int iy = blockIdx.y * BY + threadIdx.y;       \
+    if (ix >= w || iy >= h) return;                     \
+    const long pos = ix + (long)iy * s;
+
+struct FillArgs {
+    float* bufs[16];
+};
+
+__global__ void __launch_bounds__(256) k_fill_zero(FillArgs a, int nbuf, long n) {
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    long stride = (long)gridDim.x * 256;
+    for (int b = 0; b < nbuf; ++b)
+        for (long j = i; j < n; j += stride) a.bufs[b][j] = 0.0f;
+}
+
+// rgbToGray.cu:3-19
+__global__ void __launch_bounds__(256) k_gray_rgb(const uint8_t* rgb, float* out, int w, int h, int s) {
+    RF_PIX
+    float g = ((float)rgb[3 * pos] + (float)rgb[3 * pos + 1] + (float)rgb[3 * pos + 2]) / 3.0f;
+    out[pos] = g / 256.0f;
+}
+// rgbToGray.cu:21-34
+__global__ void __launch_bounds__(256) k_gray_u8(const uint8_t* in, float* out, int w, int h, int s) {
+    RF_PIX
+    out[pos] = (float)in[pos] / 256.0f;
+}
+// rgbToGray.cu:36-49
+__global__ void __launch_bounds__(256) k_gray_u16(const uint16_t* in, float* out, int w, int h, int s) {
+    RF_PIX
+    out[pos] = (float)in[pos] / 65536.0f;
+}
+// rgbToGray.cu:66-77
+__global__ void __launch_bounds__(256) k_gray_f32(const float* in, float* out, int w, int h, int s) {
+    RF_PIX
+    out[pos] = in[pos];
+}
+
+// downscale.cu:6-48. (w, h, s) is the coarse output; (W, H, S) the fine source.
+__global__ void __launch_bounds__(256) k_downscale(const float* src, int W, int H, int S, int w, int h, int s,
+                                                    int scaled, float scale, float* out) {
+    RF_PIX
+    int dxn = 4 * w, dyn = 4 * h;
+    int xc = (4 * ix + 2) * W - 2 * w;
+    int yc = (4 * iy + 2) * H - 2 * h;
+    float a = sample_ratio(src, W, H, S, xc - W, dxn, yc, dyn);
+    float b = sample_ratio(src, W, H, S, xc + W, dxn, yc, dyn);
+    float c = sample_ratio(src, W, H, S, xc, dxn, yc - H, dyn);
+    float d = sample_ratio(src, W, H, S, xc, dxn, yc + H, dyn);
+    float sum = ((a + b) + c) + d;
+    out[pos] = scaled ? (scale * 0.25f) * sum : 0.25f * sum;
+}
+
+// upscale.cu:14-27. (w, h, s) is the fine output; (cw, ch, cs) the coarse source.
+__global__ void __launch_bounds__(256) k_upscale(const float* src, int cw, int ch, int cs, int w, int h, int s,
+                                                  float scale, float* out) {
+    RF_PIX
+    float v = sample_ratio(src, cw, ch, cs, (2 * ix + 1) * cw - w, 2 * w, (2 * iy + 1) * ch - h, 2 * h);
+    out[pos] = v * scale;
+}
+
+// warpImage.cu:15-29
+__global__ void __launch_bounds__(256) k_warp(const float* src, int w, int h, int s, const float* u,
+                                               const float* v, float* out) {
+    RF_PIX
+    out[pos] = warp_sample(src, w, h, s, ix, iy, u[pos], v[pos]);
+}
+
+// computeDerivatives.cu:8-57
+__global__ void __launch_bounds__(256) k_derivatives(const float* I0, const float* I1, int w, int h, int s,
+                                                      float* Ix, float* Iy, float* Iz) {
+    RF_PIX
+    const long r = (long)iy * s;
+    int xm2 = refl(ix - 2, w), xm1 = refl(ix - 1, w), xp1 = refl(ix + 1, w), xp2 = refl(ix + 2, w);
+    int ym2 = refl(iy - 2, h), ym1 = refl(iy - 1, h), yp1 = refl(iy + 1, h), yp2 = refl(iy + 2, h);
+    float t0 = d5(I0[r + xm2], I0[r + xm1], I0[r + xp1], I0[r + xp2]);
+    float t1 = d5(I1[r + xm2], I1[r + xm1], I1[r + xp1], I1[r + xp2]);
+    Ix[pos] = (t0 + t1) * 0.5f;
+    Iz[pos] = I1[pos] - I0[pos];
+    t0 = d5(I0[ix + (long)ym2 * s], I0[ix + (long)ym1 * s], I0[ix + (long)yp1 * s], I0[ix + (long)yp2 * s]);
+    t1 = d5(I1[ix + (long)ym2 * s], I1[ix + (long)ym1 * s], I1[ix + (long)yp1 * s], I1[ix + (long)yp2 * s]);
+    Iy[pos] = (t0 + t1) * 0.5f;
+}
+
+// solveDataL1.cu:4-91
+__global__ void __launch_bounds__(256) k_data_l1(const float* du0, const float* dv0, const float* pu1,
+                                                  const float* pu2, const float* pv1, const float* pv2,
+                                                  const float* Ix, const float* Iy, const float* Iz, int w, int h,
+                                                  int s, float lambda, float theta, float* du1, float* dv1) {
+    RF_PIX
+    float du, dv;
+    l1_threshold(Ix[pos], Iy[pos], Iz[pos], du0[pos], dv0[pos], lambda, theta, du, dv);
+    bool hl = ix > 0, hu = iy > 0;
+    float divu = div_p(pu1[pos], hl ? pu1[pos - 1] : 0.0f, pu2[pos], hu ? pu2[pos - s] : 0.0f, hl, hu);
+    float divv = div_p(pv1[pos], hl ? pv1[pos - 1] : 0.0f, pv2[pos], hu ? pv2[pos - s] : 0.0f, hl, hu);
+    du1[pos] = du + theta * divu;
+    dv1[pos] = dv + theta * divv;
+}
+
+// solveSmoothDualTVGlobal.cu:4-65
+__global__ void __launch_bounds__(256) k_dual_tv(const float* du, const float* dv, const float* pu1,
+                                                  const float* pu2, const float* pv1, const float* pv2, int w, int h,
+                                                  int s, float tau, float theta, float* pu1s, float* pu2s,
+                                                  float* pv1s, float* pv2s) {
+    RF_PIX
+    float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
+    if (ix + 1 < w) {
+        dux = du[pos + 1] - du[pos];
+        dvx = dv[pos + 1] - dv[pos];
+    }
+    if (iy + 1 < h) {
+        duy = du[pos + s] - du[pos];
+        dvy = dv[pos + s] - dv[pos];
+    }
+    float fac = tau / theta;
+    float a, b;
+    dual_px(dux, duy, pu1[pos], pu2[pos], fac, a, b);
+    pu1s[pos] = a;
+    pu2s[pos] = b;
+    dual_px(dvx, dvy, pv1[pos], pv2[pos], fac, a, b);
+    pv1s[pos] = a;
+    pv2s[pos] = b;
+}
+
+// medianFilter.cu:3-75
+__global__ void __launch_bounds__(256) k_median5(const float* u, const float* v, int w, int h, int s, float* ou,
+                                                  float* ov) {
+    RF_PIX
+    float mu[25], mv[25];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            long q = median5_tap(ix + i - 2, iy + j - 2, w, h, s);
+            mu[j * 5 + i] = q >= 0 ? u[q] : 0.0f;
+            mv[j * 5 + i] = q >= 0 ? v[q] : 0.0f;
+        }
+    ou[pos] = exchange_select<25, 13>(mu);
+    ov[pos] = exchange_select<25, 13>(mv);
+}
+
+// medianFilter.cu:166-243
+__global__ void __launch_bounds__(256) k_median3d(const float* X, const float* Y, const float* Z, int w, int h,
+                                                   int s, float* X1, float* Y1, float* Z1) {
+    RF_PIX
+    float mX[9], mY[9], mZ[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            long q = median3_tap(ix + i - 1, iy + j - 1, w, h, s);
+            mX[j * 3 + i] = q >= 0 ? X[q] : 0.0f;
+            mY[j * 3 + i] = q >= 0 ? Y[q] : 0.0f;
+            mZ[j * 3 + i] = q >= 0 ? Z[q] : 0.0f;
+        }
+    X1[pos] = exchange_select<9, 5>(mX);
+    Y1[pos] = exchange_select<9, 5>(mY);
+    Z1[pos] = exchange_select<9, 5>(mZ);
+}
+
+// addKernel.cu:3-11
+__global__ void __launch_bounds__(256) k_add(const float* a, const float* b, long count, float* out) {
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) out[i] = a[i] + b[i];
+}
+
+// solveReconDataL1.cu:4-102 (prior == false) and :139-378 (prior == true; mask may be null)
+template <bool PRIOR>
+__global__ void __launch_bounds__(256) k_recon_data_l1(const float* u, const float* v, const float* du0,
+                                                        const float* dv0, const float* up, const float* vp,
+                                                        const float* sku, const float* skv, const float* pu1,
+                                                        const float* pu2, const float* pv1, const float* pv2,
+                                                        const float* Ix, const float* Iy, const float* Iz, int w,
+                                                        int h, int s, float lambda, float aTv, float aP,
+                                                        const float* ufn, const float* vfn, float aFn0,
+                                                        const float* mask, float* du1, float* dv1) {
+    RF_PIX
+    float A, bu, bv;
+    if (PRIOR) {
+        float aFn = mask ? aFn0 * mask[pos] : aFn0;
+        A = (aTv + aP) + aFn;
+        bu = recon_blend_fn(aTv, aP, aFn, A, du0[pos], up[pos], u[pos], sku[pos], ufn[pos]);
+        bv = recon_blend_fn(aTv, aP, aFn, A, dv0[pos], vp[pos], v[pos], skv[pos], vfn[pos]);
+    } else {
+        A = aTv + aP;
+        bu = recon_blend(aTv, aP, A, du0[pos], up[pos], u[pos], sku[pos]);
+        bv = recon_blend(aTv, aP, A, dv0[pos], vp[pos], v[pos], skv[pos]);
+    }
+    float du, dv;
+    l1_threshold_div(Ix[pos], Iy[pos], Iz[pos], bu, bv, lambda, A, du, dv);
+    bool hl = ix > 0, hu = iy > 0;
+    float divu = div_p(pu1[pos], hl ? pu1[pos - 1] : 0.0f, pu2[pos], hu ? pu2[pos - s] : 0.0f, hl, hu);
+    float divv = div_p(pv1[pos], hl ? pv1[pos - 1] : 0.0f, pv2[pos], hu ? pv2[pos - s] : 0.0f, hl, hu);
+    du1[pos] = du + divu / A;
+    dv1[pos] = dv + divv / A;
+}
+
+// solveUVProj.cu:4-22
+__global__ void __launch_bounds__(256) k_uvproj(const float* u, const float* v, const float* du, const float* dv,
+                                                 const float* uc, const float* vc, const float* sku,
+                                                 const float* skv, int w, int h, int s, float lf, float aP,
+                                                 float* up, float* vp) {
+    RF_PIX
+    up[pos] = uvproj_px(lf, aP, uc[pos], u[pos], du[pos], sku[pos]);
+    vp[pos] = uvproj_px(lf, aP, vc[pos], v[pos], dv[pos], skv[pos]);
+}
+
+// solve3dMinimalArea.cu:3-193. Reads column w and row h of X0 (see solve3d_px).
+// R = double evaluates the triangulation in FP64 (inputs widened, results rounded once).
+template <typename R>
+__global__ void __launch_bounds__(256) k_solve3d(const float* up, const float* vp, const float* X0,
+                                                  const float* Y0, const float* Z0, PQT<R> pqv, const PQT<R>* pqp,
+                                                  float lf, float lms, int w, int h, int s, float* X, float* Y,
+                                                  float* Z) {
+    RF_PIX
+    const PQT<R> pq = pqp ? *pqp : pqv;
+    bool hl = ix > 0, hu = iy > 0;
+    long pl = hl ? pos - 1 : pos, pu = hu ? pos - s : pos;
+    float oX, oY, oZ;
+    solve3d_px<R>(pq, (R)lf, (R)lms, ix, iy, up[pos], vp[pos], X0[pos], Y0[pos], Z0[pos], X0[pos + 1], Y0[pos + 1],
+               Z0[pos + 1], X0[pos + s], Y0[pos + s], Z0[pos + s], X0[pl], Y0[pl], Z0[pl], X0[pu], Y0[pu], Z0[pu],
+               hl, hu, oX, oY, oZ);
+    X[pos] = oX;
+    Y[pos] = oY;
+    Z[pos] = oZ;
+}
+
+// solve3dMinimalArea.cu:248-297
+__global__ void __launch_bounds__(256) k_cleanup3d(const float* X0, const float* Y0, const float* Z0,
+                                                    const float* Xm, const float* Ym, const float* Zm, int w, int h,
+                                                    int s, float* X1, float* Y1, float* Z1) {
+    RF_PIX
+    float a, b, c;
+    cleanup_px(X0[pos], Y0[pos], Z0[pos], Xm[pos], Ym[pos], Zm[pos], a, b, c);
+    X1[pos] = a;
+    Y1[pos] = b;
+    Z1[pos] = c;
+}
+
+// solveUcSk.cu:3-35
+__global__ void __launch_bounds__(256) k_solve_uc(const float* X, const float* Y, const float* Z, UcParams cv,
+                                                   const UcParams* cp, int w, int h, int s, float* uc, float* vc) {
+    RF_PIX
+    const UcParams c = cp ? *cp : cv;
+    float a, b;
+    uc_px(c, X[pos], Y[pos], Z[pos], ix, iy, a, b);
+    uc[pos] = a;
+    vc[pos] = b;
+}
+
+// solveUcSk.cu:80-97
+__global__ void __launch_bounds__(256) k_update_sk(const float* sku, const float* skv, const float* u,
+                                                    const float* v, const float* du, const float* dv,
+                                                    const float* up, const float* vp, int w, int h, int s,
+                                                    float* sku1, float* skv1) {
+    RF_PIX
+    sku1[pos] = sk_px(sku[pos], u[pos], du[pos], up[pos]);
+    skv1[pos] = sk_px(skv[pos], v[pos], dv[pos], vp[pos]);
+}
+
+// flowToHSV.cu:194-302 (Middlebury colour wheel). The reference indexes its 55-entry tables
+// with k0, k1 in [1, 55]; index 55 lies past the table there and wraps to 0 here.
+__constant__ float c_wheel[3][55] = {
+    {255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 213, 170, 128,
+     85,  43,  0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   19,
+     39,  58,  78,  98,  117, 137, 156, 176, 196, 215, 235, 255, 255, 255, 255, 255, 255},
+    {0,   17,  34,  51,  68,  85,  102, 119, 136, 153, 170, 187, 204, 221, 238, 255, 255, 255, 255,
+     255, 255, 255, 255, 255, 255, 255, 232, 209, 186, 163, 140, 116, 93,  70,  47,  24,  0,   0,
+     0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0},
+    {0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,
+     0,   0,   0,   63,  127, 191, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255,
+     255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 213, 170, 128, 85,  43}};
+
+__global__ void __launch_bounds__(256) k_flow_to_hsv(const float* u, const float* v, int w, int h, int s,
+                                                      float* rgb, float flowscale) {
+    RF_PIX
+    float du = u[pos] / flowscale, dv = v[pos] / flowscale;
+    const int ncols = 55;
+    float rad = sqrtf(du * du + dv * dv);
+    float a = atan2f(-dv, -du) / 3.14159f;
+    float fk = (a + 1.0f) / 2.0f * ((float)ncols - 1.0f) + 1.0f;
+    int k0 = (int)floorf(fk);
+    int k1 = k0 + 1;
+    if (k1 == ncols + 1) k1 = 1;
+    float f = fk - (float)k0;
+    k0 = ((k0 % ncols) + ncols) % ncols;
+    k1 = ((k1 % ncols) + ncols) % ncols;
+    float col[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float cc = (1.0f - f) * (c_wheel[c][k0] / 255.0f) + f * (c_wheel[c][k1] / 255.0f);
+        col[c] = rad <= 1.0f ? 1.0f - rad * (1.0f - cc) : cc * 0.75f;
+    }
+    rgb[3 * pos + 0] = col[2];  // B
+    rgb[3 * pos + 1] = col[1];  // G
+    rgb[3 * pos + 2] = col[0];  // R
+}
+
+// solve3dMinimalArea.cu:320-378. The forward difference at ix + range == w (or row h) reads
+// the padding column (or the next row / guard row), as in the reference.
+__global__ void __launch_bounds__(256) k_remove_edge(const float* X0, const float* Y0, const float* Z0, int range,
+                                                      float maxarea, int w, int h, int s, float* X1, float* Y1,
+                                                      float* Z1) {
+    RF_PIX
+    float X = X0[pos], Y = Y0[pos], Z = Z0[pos];
+    float Xx, Yx, Zx, Xy, Yy, Zy;
+    if (ix + range > w) {
+        long q = pos - range;
+        Xx = -X0[q] + X;
+        Yx = -Y0[q] + Y;
+        Zx = -Z0[q] + Z;
+    } else {
+        long q = pos + range;
+        Xx = X0[q] - X;
+        Yx = Y0[q] - Y;
+        Zx = Z0[q] - Z;
+    }
+    if (iy + range > h) {
+        long q = pos - (long)range * s;
+        Xy = -X0[q] + X;
+        Yy = -Y0[q] + Y;
+        Zy = -Z0[q] + Z;
+    } else {
+        long q = pos + (long)range * s;
+        Xy = X0[q] - X;
+        Yy = Y0[q] - Y;
+        Zy = Z0[q] - Z;
+    }
+    float c1 = Yx * Zy - Zx * Yy, c2 = Zx * Xy - Xx * Zy, c3 = Xx * Yy - Yx * Xy;
+    float area = sqrtf((c1 * c1 + c2 * c2) + c3 * c3);
+    bool drop = area > maxarea;
+    X1[pos] = drop ? 0.0f : X;
+    Y1[pos] = drop ? 0.0f : Y;
+    Z1[pos] = drop ? 0.0f : Z;
+}
+
+}  // namespace
+
+PQ make_pq(const double* P, const double* Q) {
+    PQ m;
+    for (int i = 0; i < 12; ++i) {
+        m.P[i] = (float)P[i];
+        m.Q[i] = (float)Q[i];
+    }
+    return m;
+}
+
+PQd make_pqd(const double* P, const double* Q) {
+    PQd m;
+    for (int i = 0; i < 12; ++i) {
+        m.P[i] = P[i];
+        m.Q[i] = Q[i];
+    }
+    return m;
+}
+
+UcParams make_uc(const double* R, const double* t, const double* K1) {
+    UcParams c;
+    c.R11 = (float)R[0];
+    c.R12 = (float)R[1];
+    c.R13 = (float)R[2];
+    c.R21 = (float)R[3];
+    c.R22 = (float)R[4];
+    c.R23 = (float)R[5];
+    c.R31 = (float)R[6];
+    c.R32 = (float)R[7];
+    c.R33 = (float)R[8];
+    c.t1 = (float)t[0];
+    c.t2 = (float)t[1];
+    c.t3 = (float)t[2];
+    c.f = (float)K1[0];
+    c.cx = (float)K1[2];
+    c.cy = (float)K1[5];
+    return c;
+}
+
+#define RF_L2(K, ...)                                                       \
+    do {                                                                    \
+        if (w <= 0 || h <= 0) return hipSuccess;                            \
+        hipLaunchKernelGGL(K, grid2(w, h), dim3(BX, BY), 0, st, __VA_ARGS__); \
+        return hipGetLastError();                                           \
+    } while (0)
+
+hipError_t launch_fill_zero(float* const* bufs, int nbuf, long n, hipStream_t st) {
+    FillArgs a{};
+    if (nbuf > 16 || nbuf < 0) return hipErrorInvalidValue;
+    for (int i = 0; i < nbuf; ++i) a.bufs[i] = bufs[i];
+    long blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, st, a, nbuf, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gray_rgb(const uint8_t* rgb, float* out, int w, int h, int s, hipStream_t st) {
+    RF_L2(k_gray_rgb, rgb, out, w, h, s);
+}
+hipError_t launch_gray_u8(const uint8_t* in, float* out, int w, int h, int s, hipStream_t st) {
+    RF_L2(k_gray_u8, in, out, w, h, s);
+}
+hipError_t launch_gray_u16(const uint16_t* in, float* out, int w, int h, int s, hipStream_t st) {
+    RF_L2(k_gray_u16, in, out, w, h, s);
+}
+hipError_t launch_gray_f32(const float* in, float* out, int w, int h, int s, hipStream_t st) {
+    RF_L2(k_gray_f32, in, out, w, h, s);
+}
+hipError_t launch_downscale(const float* src, int W, int H, int S, int w, int h, int s, bool scaled, float scale,
+                            float* out, hipStream_t st) {
+    RF_L2(k_downscale, src, W, H, S, w, h, s, scaled ? 1 : 0, scale, out);
+}
+hipError_t launch_upscale(const float* src, int cw, int ch, int cs, int w, int h, int s, float scale, float* out,
+                          hipStream_t st) {
+    RF_L2(k_upscale, src, cw, ch, cs, w, h, s, scale, out);
+}
+hipError_t launch_warp(const float* src, int w, int h, int s, const float* u, const float* v, float* out,
+                       hipStream_t st) {
+    RF_L2(k_warp, src, w, h, s, u, v, out);
+}
+hipError_t launch_derivatives(const float* I0, const float* I1, int w, int h, int s, float* Ix, float* Iy,
+                              float* Iz, hipStream_t st) {
+    RF_L2(k_derivatives, I0, I1, w, h, s, Ix, Iy, Iz);
+}
+hipError_t launch_data_l1(const float* du, const float* dv, const float* pu1, const float* pu2, const float* pv1,
+                          const float* pv2, const float* Ix, const float* Iy, const float* Iz, int w, int h, int s,
+                          float lambda, float theta, float* du1, float* dv1, hipStream_t st) {
+    RF_L2(k_data_l1, du, dv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta, du1, dv1);
+}
+hipError_t launch_dual_tv(const float* du, const float* dv, const float* pu1, const float* pu2, const float* pv1,
+                          const float* pv2, int w, int h, int s, float tau, float theta, float* pu1s, float* pu2s,
+                          float* pv1s, float* pv2s, hipStream_t st) {
+    RF_L2(k_dual_tv, du, dv, pu1, pu2, pv1, pv2, w, h, s, tau, theta, pu1s, pu2s, pv1s, pv2s);
+}
+hipError_t launch_median5(const float* u, const float* v, int w, int h, int s, float* ou, float* ov,
+                          hipStream_t st) {
+    RF_L2(k_median5, u, v, w, h, s, ou, ov);
+}
+hipError_t launch_median3d(const float* X, const float* Y, const float* Z, int w, int h, int s, float* X1,
+                           float* Y1, float* Z1, hipStream_t st) {
+    RF_L2(k_median3d, X, Y, Z, w, h, s, X1, Y1, Z1);
+}
+hipError_t launch_add(const float* a, const float* b, long count, float* out, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, a, b, count, out);
+    return hipGetLastError();
+}
+hipError_t launch_recon_data_l1(const float* u, const float* v, const float* du, const float* dv, const float* up,
+                                const float* vp, const float* sku, const float* skv, const float* pu1,
+                                const float* pu2, const float* pv1, const float* pv2, const float* Ix,
+                                const float* Iy, const float* Iz, int w, int h, int s, float lambda, float aTv,
+                                float aP, const float* ufn, const float* vfn, float aFn, const float* mask,
+                                float* du1, float* dv1, hipStream_t st) {
+    if (ufn && vfn) {
+        RF_L2(k_recon_data_l1<true>, u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s,
+              lambda, aTv, aP, ufn, vfn, aFn, mask, du1, dv1);
+    }
+    RF_L2(k_recon_data_l1<false>, u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda,
+          aTv, aP, ufn, vfn, aFn, mask, du1, dv1);
+}
+hipError_t launch_uvproj(const float* u, const float* v, const float* du, const float* dv, const float* uc,
+                         const float* vc, const float* sku, const float* skv, int w, int h, int s, float lf,
+                         float aP, float* up, float* vp, hipStream_t st) {
+    RF_L2(k_uvproj, u, v, du, dv, uc, vc, sku, skv, w, h, s, lf, aP, up, vp);
+}
+hipError_t launch_solve3d(const float* up, const float* vp, const float* X0, const float* Y0, const float* Z0,
+                          const PQ& pq, const PQ* dpq, float lf, float lms, int w, int h, int s, float* X, float* Y,
+                          float* Z, hipStream_t st) {
+    RF_L2(k_solve3d<float>, up, vp, X0, Y0, Z0, pq, dpq, lf, lms, w, h, s, X, Y, Z);
+}
+hipError_t launch_solve3d_f64(const float* up, const float* vp, const float* X0, const float* Y0, const float* Z0,
+                              const PQd& pq, const PQd* dpq, float lf, float lms, int w, int h, int s, float* X,
+                              float* Y, float* Z, hipStream_t st) {
+    RF_L2(k_solve3d<double>, up, vp, X0, Y0, Z0, pq, dpq, lf, lms, w, h, s, X, Y, Z);
+}
+hipError_t launch_cleanup3d(const float* X0, const float* Y0, const float* Z0, const float* Xm, const float* Ym,
+                            const float* Zm, int w, int h, int s, float* X1, float* Y1, float* Z1, hipStream_t st) {
+    RF_L2(k_cleanup3d, X0, Y0, Z0, Xm, Ym, Zm, w, h, s, X1, Y1, Z1);
+}
+hipError_t launch_solve_uc(const float* X, const float* Y, const float* Z, const UcParams& c, const UcParams* dc,
+                           int w, int h, int s, float* uc, float* vc, hipStream_t st) {
+    RF_L2(k_solve_uc, X, Y, Z, c, dc, w, h, s, uc, vc);
+}
+hipError_t launch_update_sk(const float* sku, const float* skv, const float* u, const float* v, const float* du,
+                            const float* dv, const float* up, const float* vp, int w, int h, int s, float* sku1,
+                            float* skv1, hipStream_t st) {
+    RF_L2(k_update_sk, sku, skv, u, v, du, dv, up, vp, w, h, s, sku1, skv1);
+}
+hipError_t launch_flow_to_hsv(const float* u, const float* v, int w, int h, int s, float* uvrgb, float flowscale,
+                              hipStream_t st) {
+    RF_L2(k_flow_to_hsv, u, v, w, h, s, uvrgb, flowscale);
+}
+hipError_t launch_remove_edge(const float* X0, const float* Y0, const float* Z0, int range, float maxarea, int w,
+                              int h, int s, float* X1, float* Y1, float* Z1, hipStream_t st) {
+    RF_L2(k_remove_edge, X0, Y0, Z0, range, maxarea, w, h, s, X1, Y1, Z1);
+}
+
+}  // namespace rf

@@ -1,0 +1,130 @@
+// Solver context behind the C ABI: device buffers of one image pair, the static solve
+// schedule (plain per-step kernels or fused kernels), and an optional hipGraph of it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rfhip.h"
+#include "rf_device.hpp"
+
+namespace rf {
+
+constexpr int kMaxLevels = 16;
+
+void set_error(const std::string& msg);
+
+// Per-level camera data that the solve kernels read from device memory, so a captured
+// schedule stays valid when the pose changes between solves.
+struct DevParams {
+    PQ pq[kMaxLevels];
+    PQd pqd[kMaxLevels];
+    UcParams uc[kMaxLevels];
+};
+
+// The pointer table the schedule rotates (the reference's d_* members, Cudaflow.h:195-352,
+// Reconflow.h:15-31). Each field buffer holds s0*h0 floats plus one zero guard row.
+struct Fields {
+    float* pI0[kMaxLevels];
+    float* pI1[kMaxLevels];
+    float *i1warp, *dumed, *dvmed, *dumeds, *dvmeds;
+    float *pu1, *pu2, *pv1, *pv2, *pu1s, *pu2s, *pv1s, *pv2s;
+    float *Ix, *Iy, *Iz, *u, *v, *us, *vs;
+    float *X, *Y, *Z, *Xs, *Ys, *Zs, *Xm, *Ym, *Zm;
+    float *uproj, *vproj, *uc, *vc, *sku, *skv, *skus, *skvs;
+    float *ufn, *vfn, *ufn_l, *vfn_l, *mask, *mask_l;
+    float* uvrgb;
+    // fused-path extras: ping-pong partners and FP16 shadows
+    float *uprojs, *vprojs, *ucs, *vcs;
+    void *h_I0[kMaxLevels], *h_I1[kMaxLevels];
+    void *h_Ix, *h_Iy, *h_Iz, *h_p[8];
+    void *in0, *in1;
+};
+
+// HIP-event timing of one kernel kind at one level (eager solves only).
+struct Profiler {
+    bool armed = false;
+    int kid = -1, level = -1;
+    std::vector<hipEvent_t> ev;  // start/stop pairs
+    int used = 0;                // events recorded
+    int launches = 0;
+    bool open = false;
+    void begin(hipStream_t st, int k, int l) {
+        if (!armed || k != kid || l != level || open) return;
+        if (used + 2 > (int)ev.size()) return;
+        hipEventRecord(ev[used], st);
+        open = true;
+    }
+    void end(hipStream_t st, int k, int l, int n) {
+        if (!armed || k != kid || l != level || !open) return;
+        hipEventRecord(ev[used + 1], st);
+        used += 2;
+        launches += n;
+        open = false;
+    }
+};
+
+struct Geometry {
+    int L = 0;
+    std::vector<int> W, H, S;
+    long n = 0;     // s0 * h0
+    long bufn = 0;  // floats per field buffer (n + guard row, rounded to 64)
+};
+
+class Solver {
+  public:
+    Solver(int bw, int bh, int align);
+    ~Solver();
+    int initialize(const rf_params& p);
+    int init_large_displacement();
+    int set_camera(const double* K0, const double* K1);
+    int copy_images(const void* i0, const void* i1);
+    int copy_images_device(const float* i0, const float* i1);
+    int copy_prior(const float* u, const float* v, const float* mask, bool sparse);
+    int solve_flow(float flow_scale);
+    int solve_recon(const double* R0, const double* t0, const double* R1, const double* t1, float flow_scale);
+    int copy_flow(float* u, float* v, float* uvrgb);
+    int copy_3d(float* X, float* Y, float* Z);
+    int remove_edge(int range, float maxarea);
+    int sync();
+    int profile_arm(int kid, int level);
+    int profile_read(double* total_ms, int* launches);
+    Profiler& prof() { return prof_; }
+
+    const Geometry& geom() const { return g_; }
+    const rf_params& params() const { return p_; }
+    hipStream_t stream() const { return st_; }
+    const Fields& result() const { return cur_; }
+
+  private:
+    int release();
+    int run_schedule(bool recon);
+    hipError_t enqueue_flow_plain(Fields& f);
+    hipError_t enqueue_recon_plain(Fields& f);
+    hipError_t enqueue_flow_fused(Fields& f);
+    hipError_t enqueue_recon_fused(Fields& f);
+    hipError_t enqueue_gray_pyramid(Fields& f);
+    int check(hipError_t e, const char* what);
+
+    int bw_, bh_, align_;
+    rf_params p_{};
+    bool inited_ = false, prior_ = false, sparse_ = false, camera_ = false;
+    Geometry g_;
+    hipStream_t st_ = nullptr;
+    void* slab_ = nullptr;
+    size_t slab_bytes_ = 0;
+    Fields init_{}, cur_{};
+    DevParams hparams_{};
+    DevParams* dparams_ = nullptr;
+    std::vector<std::vector<double>> K0_, K1_;
+    double R0_[9], t0_[3], R1_[9], t1_[3];
+    float flow_scale_ = 5.0f;
+    // graph cache: one per schedule kind (flow / recon)
+    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    Fields gfinal_[2];
+    Profiler prof_;
+};
+
+}  // namespace rf
