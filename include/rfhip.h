@@ -151,6 +151,18 @@ int rf_get_result_device(const rf_context* ctx, const float** u, const float** v
 /* ReconFlow::removeEdgeArtifacts3D(surfaceWidth, maxSurfaceArea) (Reconflow.cu:468-477). */
 int rf_remove_edge_artifacts_3d(rf_context* ctx, int surface_width, float max_surface_area);
 
+/* ---- host I/O formats ----
+ * Middlebury .flo reader / writer: the prior-flow file of the reference demo
+ * (cv::optflow::readOpticalFlow, test_reconflow/main.cpp:82-90). With u or v NULL, the reader
+ * only returns the size; otherwise it fills stride-padded planes (zero padding, main.cpp:96). */
+int rf_read_flo(const char* path, int* width, int* height, float* u, float* v, int stride);
+int rf_write_flo(const char* path, const float* u, const float* v, int width, int height, int stride);
+/* ASCII PLY of the reconstruction with the demo's mask (main.cpp:108-139): keeps points with
+ * zmin < Z <= zmax, X <= xymax, -xymax < Y <= xymax, written as (X, -Y, -Z); rgb (BGR, 3 bytes
+ * per pixel, stride-padded) may be NULL. */
+int rf_write_ply(const char* path, const float* X, const float* Y, const float* Z, const uint8_t* rgb, int width,
+                 int height, int stride, float zmin, float zmax, float xymax);
+
 /* ---- kernel timing (no counterpart in the reference, which has no timers) ----
  * Arms HIP-event timing of one kernel kind at one pyramid level for the next solves that run
  * without a graph: events are recorded on the context's stream around every launch of that

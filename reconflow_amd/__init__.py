@@ -245,6 +245,69 @@ class ReconFlow(CudaFlow):
         return 0
 
 
+class Batch:
+    """Many pairs of uneven sizes solved together (rf_batch_* of rfhip.h): every launch of the
+    shared schedule covers the tiles of all pairs. ``params`` is an RFParams whose width and
+    height are ignored (taken per pair); images are handed over as device float buffers."""
+
+    def __init__(self, params: RFParams, widths, heights):
+        self._lib = _native.lib()
+        self._b = C.c_void_p()
+        n = len(widths)
+        W = (C.c_int * n)(*widths)
+        H = (C.c_int * n)(*heights)
+        check(self._lib.rf_batch_create(C.byref(self._b), C.byref(params), n, W, H), "rf_batch_create")
+        self.n = n
+        self.widths, self.heights = list(widths), list(heights)
+
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b is not None and b.value:
+            self._lib.rf_batch_destroy(b)
+            self._b = None
+
+    def stride(self, i: int) -> int:
+        return self._lib.rf_batch_stride(self._b, i)
+
+    def set_images_device(self, i: int, d_i0: int, d_i1: int) -> None:
+        check(self._lib.rf_batch_set_images_device(self._b, i, d_i0, d_i1), "rf_batch_set_images_device")
+
+    def set_camera(self, i, K0, K1, R1, t1) -> None:
+        ps = [_dptr(a) for a in (K0, K1, R1, t1)]
+        check(self._lib.rf_batch_set_camera(self._b, i, *[p for p, _ in ps]), "rf_batch_set_camera")
+
+    def solve(self) -> None:
+        check(self._lib.rf_batch_solve(self._b), "rf_batch_solve")
+
+    def synchronize(self) -> None:
+        check(self._lib.rf_batch_synchronize(self._b), "rf_batch_synchronize")
+
+    def stream(self) -> int:
+        return self._lib.rf_batch_get_stream(self._b) or 0
+
+    def result_ptrs(self, i: int):
+        ptrs = [C.c_void_p() for _ in range(5)]
+        check(self._lib.rf_batch_get_result_device(self._b, i, *[C.byref(p) for p in ptrs]),
+              "rf_batch_get_result_device")
+        return [p.value or 0 for p in ptrs]
+
+
+def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50.0, theta=0.3, tau=0.25,
+                 alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1, lambdams=100.0, use_graph=True, tri_f64=0,
+                 storage=STORAGE_F32) -> RFParams:
+    """Parameters for a Batch: configuration-0 style TV-L1 (method 0) or configuration-1 style
+    joint solve (method 14), float gray input."""
+    lib = _native.lib()
+    p = RFParams()
+    lib.rf_params_default(C.byref(p), 1, 1)
+    p.input_type, p.n_levels, p.scale, p.method = INPUT_32F, n_levels, 2.0, method
+    p.lambda_, p.theta, p.tau = lam, theta, tau
+    p.n_warp_iters, p.n_solver_iters = n_warp, n_iter
+    p.lambdaf, p.lambdams, p.alpha_tv, p.alpha_proj = lambdaf, lambdams, alpha_tv, alpha_proj
+    p.path, p.use_graph, p.tri_f64, p.storage = PATH_FUSED, int(use_graph), tri_f64, storage
+    return p
+
+
 # ---- seeded synthetic pairs ---------------------------------------------------------
 def make_flow_pair(w: int, h: int, seed: int, stride: int | None = None):
     """Config-0 style pair: returns (i0, i1, gt_u, gt_v), each (h, stride) float32."""

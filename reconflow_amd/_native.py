@@ -54,6 +54,9 @@ _PROTOS = {
     "rf_copy_3d": (_I, [_P, _P, _P, _P]),
     "rf_get_result_device": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "rf_remove_edge_artifacts_3d": (_I, [_P, _I, _F]),
+    "rf_read_flo": (_I, [C.c_char_p, C.POINTER(_I), C.POINTER(_I), _P, _P, _I]),
+    "rf_write_flo": (_I, [C.c_char_p, _P, _P, _I, _I, _I]),
+    "rf_write_ply": (_I, [C.c_char_p, _P, _P, _P, _P, _I, _I, _I, _F, _F, _F]),
     "rf_profile_arm": (_I, [_P, _I, _I]),
     "rf_profile_read": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_I)]),
     "rf_batch_create": (_I, [C.POINTER(_P), C.POINTER(RFParams), _I, _P, _P]),
@@ -120,6 +123,13 @@ def lib() -> C.CDLL:
         path = lib_path()
         if not path.exists():
             raise RuntimeError(f"{path} is missing: run reconflow_amd.build.build_all() (HIP extension not built)")
+        # One HIP runtime per process: torch ships its own libamdhip64 (same soname). Loading it
+        # first lets the solver bind to it; loading the solver first would bring a second
+        # runtime, and device pointers of one are invalid in the other.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = _bind(C.CDLL(str(path)), _PROTOS)
     return _lib
 

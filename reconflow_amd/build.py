@@ -19,7 +19,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 
-HIP_SOURCES = ["rf_kernels.hip", "rf_fused.hip", "rf_solver.hip", "rf_capi_kernels.hip", "rf_batch.hip"]
+HIP_SOURCES = ["rf_kernels.hip", "rf_batch.hip", "rf_solver.hip", "rf_capi_kernels.hip"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -65,11 +65,25 @@ def build_hip(jobs: int = 8) -> Path:
         if len(procs) >= jobs:
             _drain(procs)
     _drain(procs)
+    io_obj = OUT / "rf_io.o"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", str(CSRC / "rf_io.cpp"), "-o", str(io_obj)])
+    objs.append(io_obj)
     lib = OUT / "librfhip.so"
     tmp = OUT / "librfhip.so.tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
+          "-Wl,-rpath,/opt/rocm/lib"])
     tmp.replace(lib)
     return lib
+
+
+def build_demo() -> Path:
+    """Link the C++ command-line driver over include/rfhip_cudaflow.hpp (needs both libraries)."""
+    exe = OUT / "rf_demo"
+    tmp = OUT / "rf_demo.tmp"
+    _run(["g++", "-O2", "-std=c++17", str(CSRC / "rf_demo.cpp"), "-I", str(ROOT / "include"), "-L", str(OUT),
+          "-lrfhip", "-lrfsynth", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)])
+    tmp.replace(exe)
+    return exe
 
 
 def _drain(procs: list) -> None:
@@ -93,6 +107,7 @@ def build_synth() -> Path:
 def build_all() -> None:
     build_synth()
     build_hip()
+    build_demo()
 
 
 if __name__ == "__main__":

@@ -1,67 +1,1099 @@
-// Batch API (rfhip.h rf_batch_*): placeholder entry points until the batched schedule lands.
+// Batched fused solve (rf_batch.hpp) and the rf_batch_* C ABI.
+//
+// Schedule (shared by every pair of a batch; TV-L1: Cudaflow.cu:403-621, joint:
+// Reconflow.cu:147-466), one launch per step over the tiles of all pairs:
+//   gray conversion, pyramid levels, zero state;
+//   per level: [prior-flow downscale], [X buffers made identical]; per warp: warp, derivatives,
+//   n fused iterations, median-of-uhat + add [+ median of the flow]; upscale to the next level.
+// Joint iteration n in one kernel: a workgroup owns a 64 x 8 tile plus a one-pixel ring;
+//   phase 1 (tile + ring): joint data step -> uhat_n, projection blend -> uproj_n (LDS);
+//   phase 2 (tile + ring): triangulation -> X'_n (LDS); (tile) dual step -> p_n;
+//   phase 3 (tile): 3x3 XYZ median of X'_n, cleanup -> X_n, reprojection -> uc_n, Sk update.
+// TV-L1 iteration n in one kernel: data step on a 64 x 16 tile plus its right column and
+// bottom row (LDS), dual step on the tile.
+// All iterated fields are double-buffered (state n-1 in, n out), so ring pixels another
+// workgroup reads are never overwritten during a launch; the parities live on the host, so a
+// captured graph replays the same pointer pattern. The per-pixel arithmetic is the one of the
+// per-step kernels (rf_device.hpp): results equal the per-step path bit for bit.
+//
+// The triangulation reads X_{n-1} at column w and row h of the flat buffer (the reference's
+// border guards never fire). The reference reads those cells from one physical buffer that is
+// written only at valid cells; here both X buffers are made identical at the start of every
+// level, after which no launch writes outside the valid region, so the same values are read.
+//
+// Tile lists map the dispatch index onto (pair, tile) so that each XCD (workgroups go
+// round-robin over the 8 XCDs) receives a contiguous run of tiles, keeping ring re-reads in one
+// L2.
 
-#include "../../include/rfhip.h"
-#include "rf_solver.hpp"
+#include <hip/hip_fp16.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "rf_batch.hpp"
+#include "rf_kernels.hpp"
+
+namespace rf {
+namespace {
+
+constexpr int NXCD = 8;
+constexpr int TXW = 64;
+constexpr int TRW = 4;
+constexpr int RPT_JOINT = 2;  // 64 x 8 joint tiles
+constexpr int RPT_FLOW = 4;   // 64 x 16 TV-L1 tiles
+constexpr int FLAT = kMaxLevels;
+
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    int q = n / NXCD, r = n % NXCD;
+    int k = b % NXCD, j = b / NXCD;
+    return k * q + min(k, r) + j;
+}
+
+struct Launch {
+    const PairFields* P;
+    const int2* map;
+    int ntiles;
+};
+
+// Pointwise kernels: 64 x 4 tiles, one pixel per thread.
+#define BT_PIX(L, LEVEL)                                                       \
+    const int2 pt = L.map[xcd_tile(blockIdx.x, L.ntiles)];                    \
+    const PairFields& F = L.P[pt.x];                                           \
+    const int w = F.W[LEVEL], h = F.H[LEVEL], s = F.S[LEVEL];                  \
+    const int tiles_x = (w + TXW - 1) / TXW;                                   \
+    const int ix = (pt.y % tiles_x) * TXW + threadIdx.x;                       \
+    const int iy = (pt.y / tiles_x) * TRW + threadIdx.y;                       \
+    if (ix >= w || iy >= h) return;                                            \
+    const long pos = ix + (long)iy * s;
+
+// Storage of images, derivatives and dual fields: float, or __half with every update in FP32
+// (values rounded to nearest on store, widened exactly on load).
+template <typename TS>
+__device__ __forceinline__ TS st(float v) {
+    return v;
+}
+template <>
+__device__ __forceinline__ __half st<__half>(float v) {
+    return __float2half(v);
+}
+template <typename TS>
+__device__ __forceinline__ TS* as(float* p) {
+    return reinterpret_cast<TS*>(p);
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) kb_gray(Launch L, int type) {
+    BT_PIX(L, 0)
+    float g0, g1;
+    if (type == RF_INPUT_8UC3) {
+        const uint8_t *a = (const uint8_t*)F.in0, *b = (const uint8_t*)F.in1;
+        g0 = ((float)a[3 * pos] + (float)a[3 * pos + 1] + (float)a[3 * pos + 2]) / 3.0f;
+        g1 = ((float)b[3 * pos] + (float)b[3 * pos + 1] + (float)b[3 * pos + 2]) / 3.0f;
+        g0 = g0 / 256.0f;
+        g1 = g1 / 256.0f;
+    } else if (type == RF_INPUT_8U) {
+        g0 = (float)((const uint8_t*)F.in0)[pos] / 256.0f;
+        g1 = (float)((const uint8_t*)F.in1)[pos] / 256.0f;
+    } else if (type == RF_INPUT_16U) {
+        g0 = (float)((const uint16_t*)F.in0)[pos] / 65536.0f;
+        g1 = (float)((const uint16_t*)F.in1)[pos] / 65536.0f;
+    } else {
+        g0 = ((const float*)F.in0)[pos];
+        g1 = ((const float*)F.in1)[pos];
+    }
+    as<TS>(F.I0[0])[pos] = st<TS>(g0);
+    as<TS>(F.I1[0])[pos] = st<TS>(g1);
+}
+
+template <typename TI>
+__device__ __forceinline__ float down_px(const TI* src, int W, int H, int S, int ix, int iy, int w, int h) {
+    int dxn = 4 * w, dyn = 4 * h;
+    int xc = (4 * ix + 2) * W - 2 * w;
+    int yc = (4 * iy + 2) * H - 2 * h;
+    float a = sample_ratio(src, W, H, S, xc - W, dxn, yc, dyn);
+    float b = sample_ratio(src, W, H, S, xc + W, dxn, yc, dyn);
+    float c = sample_ratio(src, W, H, S, xc, dxn, yc - H, dyn);
+    float d = sample_ratio(src, W, H, S, xc, dxn, yc + H, dyn);
+    return ((a + b) + c) + d;
+}
+
+// DownscaleKernel for both images of level l from level l-1 (downscale.cu:6-26).
+template <typename TS>
+__global__ void __launch_bounds__(256) kb_pyramid(Launch L, int l) {
+    BT_PIX(L, l)
+    const int W = F.W[l - 1], H = F.H[l - 1], S = F.S[l - 1];
+    as<TS>(F.I0[l])[pos] = st<TS>(0.25f * down_px(as<TS>(F.I0[l - 1]), W, H, S, ix, iy, w, h));
+    as<TS>(F.I1[l])[pos] = st<TS>(0.25f * down_px(as<TS>(F.I1[l - 1]), W, H, S, ix, iy, w, h));
+}
+
+// Prior flow at level l, straight from level 0 (Reconflow.cu:232-238; downscale.cu:28-48).
+__global__ void __launch_bounds__(256) kb_prior(Launch L, int l) {
+    BT_PIX(L, l)
+    const int W = F.W[0], H = F.H[0], S = F.S[0];
+    const float sc = (float)w / (float)W;
+    F.ufn_l[pos] = (sc * 0.25f) * down_px(F.ufn, W, H, S, ix, iy, w, h);
+    F.vfn_l[pos] = (sc * 0.25f) * down_px(F.vfn, W, H, S, ix, iy, w, h);
+    F.mask_l[pos] = 0.25f * down_px(F.mask, W, H, S, ix, iy, w, h);
+}
+
+// Zero the solve-start state over the whole flat level-0 buffer (Reconflow.cu:209-220).
+__global__ void __launch_bounds__(256) kb_zero(Launch L, int recon) {
+    const int2 pt = L.map[xcd_tile(blockIdx.x, L.ntiles)];
+    const PairFields& F = L.P[pt.x];
+    const int w = F.S[0], h = F.H[0];
+    const int tiles_x = (w + TXW - 1) / TXW;
+    const int ix = (pt.y % tiles_x) * TXW + threadIdx.x, iy = (pt.y / tiles_x) * TRW + threadIdx.y;
+    if (ix >= w || iy >= h) return;
+    const long pos = ix + (long)iy * w;
+    F.u[0][pos] = 0.0f;
+    F.v[0][pos] = 0.0f;
+    if (recon) {
+        F.up[0][pos] = 0.0f;
+        F.vp[0][pos] = 0.0f;
+        F.uc[0][pos] = 0.0f;
+        F.vc[0][pos] = 0.0f;
+        F.sku[0][pos] = 0.0f;
+        F.skv[0][pos] = 0.0f;
+        F.X[0][pos] = 0.0f;
+        F.Y[0][pos] = 0.0f;
+        F.Z[0][pos] = 0.0f;
+    }
+}
+
+// Make both X buffers identical over the flat level-0 buffer.
+__global__ void __launch_bounds__(256) kb_sync_x(Launch L, int px) {
+    const int2 pt = L.map[xcd_tile(blockIdx.x, L.ntiles)];
+    const PairFields& F = L.P[pt.x];
+    const int w = F.S[0], h = F.H[0];
+    const int tiles_x = (w + TXW - 1) / TXW;
+    const int ix = (pt.y % tiles_x) * TXW + threadIdx.x, iy = (pt.y / tiles_x) * TRW + threadIdx.y;
+    if (ix >= w || iy >= h) return;
+    const long pos = ix + (long)iy * w;
+    F.X[px ^ 1][pos] = F.X[px][pos];
+    F.Y[px ^ 1][pos] = F.Y[px][pos];
+    F.Z[px ^ 1][pos] = F.Z[px][pos];
+}
+
+// WarpingKernel (warpImage.cu:15-29).
+template <typename TS>
+__global__ void __launch_bounds__(256) kb_warp(Launch L, int l, int pu) {
+    BT_PIX(L, l)
+    as<TS>(F.i1warp)[pos] = st<TS>(warp_sample(as<TS>(F.I1[l]), w, h, s, ix, iy, F.u[pu][pos], F.v[pu][pos]));
+}
+
+// ComputeDerivativesKernel (computeDerivatives.cu:8-57).
+template <typename TS>
+__global__ void __launch_bounds__(256) kb_deriv(Launch L, int l) {
+    BT_PIX(L, l)
+    const TS *I0 = as<TS>(F.I0[l]), *I1 = as<TS>(F.i1warp);
+    const long r = (long)iy * s;
+    int xm2 = refl(ix - 2, w), xm1 = refl(ix - 1, w), xp1 = refl(ix + 1, w), xp2 = refl(ix + 2, w);
+    int ym2 = refl(iy - 2, h), ym1 = refl(iy - 1, h), yp1 = refl(iy + 1, h), yp2 = refl(iy + 2, h);
+    auto g = [](const TS* p, long i) { return (float)p[i]; };
+    float t0 = d5(g(I0, r + xm2), g(I0, r + xm1), g(I0, r + xp1), g(I0, r + xp2));
+    float t1 = d5(g(I1, r + xm2), g(I1, r + xm1), g(I1, r + xp1), g(I1, r + xp2));
+    as<TS>(F.Ix)[pos] = st<TS>((t0 + t1) * 0.5f);
+    as<TS>(F.Iz)[pos] = st<TS>(g(I1, pos) - g(I0, pos));
+    t0 = d5(g(I0, ix + (long)ym2 * s), g(I0, ix + (long)ym1 * s), g(I0, ix + (long)yp1 * s), g(I0, ix + (long)yp2 * s));
+    t1 = d5(g(I1, ix + (long)ym2 * s), g(I1, ix + (long)ym1 * s), g(I1, ix + (long)yp1 * s), g(I1, ix + (long)yp2 * s));
+    as<TS>(F.Iy)[pos] = st<TS>((t0 + t1) * 0.5f);
+}
+
+// Median of uhat added to the flow (medianFilter.cu:3-75, addKernel.cu:3-11), in place.
+__global__ void __launch_bounds__(256) kb_med5_add(Launch L, int l, int pd, int pu) {
+    BT_PIX(L, l)
+    const float *du = F.du[pd], *dv = F.dv[pd];
+    float mu[25], mv[25];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            long q = median5_tap(ix + i - 2, iy + j - 2, w, h, s);
+            mu[j * 5 + i] = q >= 0 ? du[q] : 0.0f;
+            mv[j * 5 + i] = q >= 0 ? dv[q] : 0.0f;
+        }
+    F.u[pu][pos] = F.u[pu][pos] + exchange_select<25, 13>(mu);
+    F.v[pu][pos] = F.v[pu][pos] + exchange_select<25, 13>(mv);
+}
+
+// Median of the flow (Reconflow.cu:418-420): u[pu] -> u[pu^1].
+__global__ void __launch_bounds__(256) kb_median5(Launch L, int l, int pu) {
+    BT_PIX(L, l)
+    const float *u = F.u[pu], *v = F.v[pu];
+    float mu[25], mv[25];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            long q = median5_tap(ix + i - 2, iy + j - 2, w, h, s);
+            mu[j * 5 + i] = q >= 0 ? u[q] : 0.0f;
+            mv[j * 5 + i] = q >= 0 ? v[q] : 0.0f;
+        }
+    F.u[pu ^ 1][pos] = exchange_select<25, 13>(mu);
+    F.v[pu ^ 1][pos] = exchange_select<25, 13>(mv);
+}
+
+// Upscale from coarse level l+1 to level l (upscale.cu:14-27): the flow, and in joint mode
+// uc, uproj and Sk (Reconflow.cu:424-458), into the partner buffers.
+__global__ void __launch_bounds__(256) kb_upscale(Launch L, int l, int pu, int pf, int recon, float scale) {
+    BT_PIX(L, l)
+    const int cw = F.W[l + 1], ch = F.H[l + 1], cs = F.S[l + 1];
+    const int nx = (2 * ix + 1) * cw - w, dx = 2 * w, ny = (2 * iy + 1) * ch - h, dy = 2 * h;
+    const int xq = floor_div(nx, dx), yq = floor_div(ny, dy);
+    const float fx = (float)(nx - xq * dx) / (float)dx, fy = (float)(ny - yq * dy) / (float)dy;
+    const long a = refl(xq, cw) + (long)refl(yq, ch) * cs, b = refl(xq + 1, cw) + (long)refl(yq, ch) * cs;
+    const long c = refl(xq, cw) + (long)refl(yq + 1, ch) * cs, d = refl(xq + 1, cw) + (long)refl(yq + 1, ch) * cs;
+    auto up = [&](const float* src, float* dst) { dst[pos] = blend(src[a], src[b], src[c], src[d], fx, fy) * scale; };
+    up(F.u[pu], F.u[pu ^ 1]);
+    up(F.v[pu], F.v[pu ^ 1]);
+    if (recon) {
+        up(F.uc[pf], F.uc[pf ^ 1]);
+        up(F.vc[pf], F.vc[pf ^ 1]);
+        up(F.up[pf], F.up[pf ^ 1]);
+        up(F.vp[pf], F.vp[pf ^ 1]);
+        up(F.sku[pf], F.sku[pf ^ 1]);
+        up(F.skv[pf], F.skv[pf ^ 1]);
+    }
+}
+
+__global__ void __launch_bounds__(256) kb_hsv(Launch L, int pu, float flowscale);
+
+struct IterArgs {
+    Launch L;
+    int level, pd, pf, px, pu, first;
+    float lambda, theta, tau, aTv, aP, aFn, lf, lms;
+    int sparse;
+};
+
+// One joint iteration (see the file comment). TY = 8.
+template <int RPT, bool F64TRI, bool PRIOR, typename TS>
+__global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
+    constexpr int TY = TRW * RPT, EX = TXW + 2, EY = TY + 2, NE = EX * EY, NHALO = 2 * EX + 2 * TY;
+    static_assert(NHALO <= 256, "ring larger than the workgroup");
+    __shared__ float s_du[NE], s_dv[NE], s_up[NE], s_vp[NE], s_X[NE], s_Y[NE], s_Z[NE];
+    const int2 pt = a.L.map[xcd_tile(blockIdx.x, a.L.ntiles)];
+    const PairFields& F = a.L.P[pt.x];
+    const int l = a.level;
+    const int w = F.W[l], h = F.H[l], s = F.S[l];
+    const int tiles_x = (w + TXW - 1) / TXW;
+    const int x0 = (pt.y % tiles_x) * TXW, y0 = (pt.y / tiles_x) * TY;
+    const int tid = threadIdx.y * TXW + threadIdx.x;
+    const int pd = a.pd, pf = a.pf, px = a.px;
+    const float *du0 = F.du[pd], *dv0 = F.dv[pd];
+    const TS *p1u0 = as<TS>(F.p1u[pd]), *p2u0 = as<TS>(F.p2u[pd]), *p1v0 = as<TS>(F.p1v[pd]),
+             *p2v0 = as<TS>(F.p2v[pd]);
+    const float *up0 = F.up[pf], *vp0 = F.vp[pf], *sku0 = F.sku[pf], *skv0 = F.skv[pf], *uc0 = F.uc[pf],
+                *vc0 = F.vc[pf];
+    const float *X0 = F.X[px], *Y0 = F.Y[px], *Z0 = F.Z[px];
+    const float *U = F.u[a.pu], *V = F.v[a.pu];
+    const TS *Ix = as<TS>(F.Ix), *Iy = as<TS>(F.Iy), *Iz = as<TS>(F.Iz);
+    const float *ufn = F.ufn_l, *vfn = F.vfn_l, *mask = a.sparse ? F.mask_l : nullptr;
+
+    int hex = -1, hey = -1;
+    if (tid < NHALO) {
+        if (tid < EX) hex = tid, hey = 0;
+        else if (tid < 2 * EX) hex = tid - EX, hey = EY - 1;
+        else if (tid < 2 * EX + TY) hex = 0, hey = tid - 2 * EX + 1;
+        else hex = EX - 1, hey = tid - 2 * EX - TY + 1;
+    }
+
+    // phase 1: data step + projection blend (tile and ring)
+    float rp[RPT][4];
+    auto phase1 = [&](int ex, int ey, float* keep) {
+        const int x = x0 - 1 + ex, y = y0 - 1 + ey;
+        if (x < 0 || y < 0 || x >= w || y >= h) return;
+        const long q = x + (long)y * s;
+        const bool hl = x > 0, hu = y > 0;
+        float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
+              p1vl = 0.0f, p2vd = 0.0f;
+        if (!a.first) {
+            duh = du0[q];
+            dvh = dv0[q];
+            p1u = p1u0[q];
+            p2u = p2u0[q];
+            p1v = p1v0[q];
+            p2v = p2v0[q];
+            if (hl) p1ul = p1u0[q - 1], p1vl = p1v0[q - 1];
+            if (hu) p2ud = p2u0[q - s], p2vd = p2v0[q - s];
+        }
+        const float uu = U[q], vv = V[q], sku = sku0[q], skv = skv0[q];
+        float AA, bu, bv;
+        if (PRIOR) {
+            const float aFn = mask ? a.aFn * mask[q] : a.aFn;
+            AA = (a.aTv + a.aP) + aFn;
+            bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, duh, up0[q], uu, sku, ufn[q]);
+            bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, dvh, vp0[q], vv, skv, vfn[q]);
+        } else {
+            AA = a.aTv + a.aP;
+            bu = recon_blend(a.aTv, a.aP, AA, duh, up0[q], uu, sku);
+            bv = recon_blend(a.aTv, a.aP, AA, dvh, vp0[q], vv, skv);
+        }
+        float du, dv;
+        l1_threshold_div(Ix[q], Iy[q], Iz[q], bu, bv, a.lambda, AA, du, dv);
+        const float nduh = du + div_p(p1u, p1ul, p2u, p2ud, hl, hu) / AA;
+        const float ndvh = dv + div_p(p1v, p1vl, p2v, p2vd, hl, hu) / AA;
+        const int e = ex + ey * EX;
+        s_du[e] = nduh;
+        s_dv[e] = ndvh;
+        s_up[e] = uvproj_px(a.lf, a.aP, uc0[q], uu, nduh, sku);
+        s_vp[e] = uvproj_px(a.lf, a.aP, vc0[q], vv, ndvh, skv);
+        if (keep) {
+            keep[0] = p1u;
+            keep[1] = p2u;
+            keep[2] = p1v;
+            keep[3] = p2v;
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) phase1(threadIdx.x + 1, threadIdx.y + TRW * r + 1, rp[r]);
+    if (hex >= 0) phase1(hex, hey, nullptr);
+    __syncthreads();
+
+    // phase 2: triangulation (tile and ring), dual step (tile)
+    auto tri = [&](int ex, int ey) {
+        const int x = x0 - 1 + ex, y = y0 - 1 + ey;
+        if (x < 0 || y < 0 || x >= w || y >= h) return;
+        const long q = x + (long)y * s;
+        const bool hl = x > 0, hu = y > 0;
+        const long ql = hl ? q - 1 : q, qu = hu ? q - s : q;
+        const int e = ex + ey * EX;
+        float oX, oY, oZ;
+        if (F64TRI)
+            solve3d_px<double>(F.pqd[l], (double)a.lf, (double)a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q],
+                               X0[q + 1], Y0[q + 1], Z0[q + 1], X0[q + s], Y0[q + s], Z0[q + s], X0[ql], Y0[ql],
+                               Z0[ql], X0[qu], Y0[qu], Z0[qu], hl, hu, oX, oY, oZ);
+        else
+            solve3d_px<float>(F.pq[l], a.lf, a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q], X0[q + 1],
+                              Y0[q + 1], Z0[q + 1], X0[q + s], Y0[q + s], Z0[q + s], X0[ql], Y0[ql], Z0[ql], X0[qu],
+                              Y0[qu], Z0[qu], hl, hu, oX, oY, oZ);
+        s_X[e] = oX;
+        s_Y[e] = oY;
+        s_Z[e] = oZ;
+    };
+    const float fac = a.tau / a.theta;
+    TS *p1u1 = as<TS>(F.p1u[pd ^ 1]), *p2u1 = as<TS>(F.p2u[pd ^ 1]), *p1v1 = as<TS>(F.p1v[pd ^ 1]),
+       *p2v1 = as<TS>(F.p2v[pd ^ 1]);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int ex = threadIdx.x + 1, ey = threadIdx.y + TRW * r + 1;
+        tri(ex, ey);
+        const int x = x0 + threadIdx.x, y = y0 + threadIdx.y + TRW * r;
+        if (x < w && y < h) {
+            const long q = x + (long)y * s;
+            const int e = ex + ey * EX;
+            float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
+            if (x + 1 < w) {
+                dux = s_du[e + 1] - s_du[e];
+                dvx = s_dv[e + 1] - s_dv[e];
+            }
+            if (y + 1 < h) {
+                duy = s_du[e + EX] - s_du[e];
+                dvy = s_dv[e + EX] - s_dv[e];
+            }
+            float o1, o2;
+            dual_px(dux, duy, rp[r][0], rp[r][1], fac, o1, o2);
+            p1u1[q] = st<TS>(o1);
+            p2u1[q] = st<TS>(o2);
+            dual_px(dvx, dvy, rp[r][2], rp[r][3], fac, o1, o2);
+            p1v1[q] = st<TS>(o1);
+            p2v1[q] = st<TS>(o2);
+        }
+    }
+    if (hex >= 0) tri(hex, hey);
+    __syncthreads();
+
+    // phase 3: XYZ median, cleanup, reprojection, Sk update; write the tile's state
+    const UcParams& ucp = F.ucp[l];
+    float *du1 = F.du[pd ^ 1], *dv1 = F.dv[pd ^ 1];
+    float *up1 = F.up[pf ^ 1], *vp1 = F.vp[pf ^ 1], *sku1 = F.sku[pf ^ 1], *skv1 = F.skv[pf ^ 1];
+    float *uc1 = F.uc[pf ^ 1], *vc1 = F.vc[pf ^ 1], *X1 = F.X[px ^ 1], *Y1 = F.Y[px ^ 1], *Z1 = F.Z[px ^ 1];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int x = x0 + threadIdx.x, y = y0 + threadIdx.y + TRW * r;
+        if (x >= w || y >= h) continue;
+        const long q = x + (long)y * s;
+        const int e = (threadIdx.x + 1) + (threadIdx.y + TRW * r + 1) * EX;
+        float mX[9], mY[9], mZ[9];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                // medianFilter.cu:193-214 tap rule, resolved to a cell of the extended tile
+                const int col = x + i - 1, row = y + j - 1;
+                int cc = -1, rr = -1;
+                if (col >= 0 && col < w && row >= 0 && row < h) cc = col, rr = row;
+                else if (col < 0 && row >= 0 && row < h) cc = 0, rr = row;
+                else if (col > w && row >= 0 && row < h) cc = w - 1, rr = row;
+                else if (col >= 0 && col < w && row < 0) cc = col, rr = 0;
+                else if (col >= 0 && col < w && row > h) cc = col, rr = h - 1;
+                const int k = j * 3 + i;
+                if (cc >= 0) {
+                    const int ee = (cc - x0 + 1) + (rr - y0 + 1) * EX;
+                    mX[k] = s_X[ee];
+                    mY[k] = s_Y[ee];
+                    mZ[k] = s_Z[ee];
+                } else {
+                    mX[k] = 0.0f;
+                    mY[k] = 0.0f;
+                    mZ[k] = 0.0f;
+                }
+            }
+        const float xm = exchange_select<9, 5>(mX), ym = exchange_select<9, 5>(mY), zm = exchange_select<9, 5>(mZ);
+        float Xn, Yn, Zn;
+        cleanup_px(s_X[e], s_Y[e], s_Z[e], xm, ym, zm, Xn, Yn, Zn);
+        float ucn, vcn;
+        uc_px(ucp, Xn, Yn, Zn, x, y, ucn, vcn);
+        const float duh = s_du[e], dvh = s_dv[e], upn = s_up[e], vpn = s_vp[e];
+        X1[q] = Xn;
+        Y1[q] = Yn;
+        Z1[q] = Zn;
+        uc1[q] = ucn;
+        vc1[q] = vcn;
+        du1[q] = duh;
+        dv1[q] = dvh;
+        up1[q] = upn;
+        vp1[q] = vpn;
+        sku1[q] = sk_px(sku0[q], U[q], duh, upn);
+        skv1[q] = sk_px(skv0[q], V[q], dvh, vpn);
+    }
+}
+
+// One TV-L1 iteration: data step (tile + right column + bottom row), dual step (tile).
+template <int RPT, typename TS>
+__global__ void __launch_bounds__(256) kb_flow_iter(IterArgs a) {
+    constexpr int TY = TRW * RPT, EX = TXW + 1, EY = TY + 1, NE = EX * EY, NHALO = EX + TY;
+    static_assert(NHALO <= 256, "ring larger than the workgroup");
+    __shared__ float s_du[NE], s_dv[NE];
+    const int2 pt = a.L.map[xcd_tile(blockIdx.x, a.L.ntiles)];
+    const PairFields& F = a.L.P[pt.x];
+    const int l = a.level;
+    const int w = F.W[l], h = F.H[l], s = F.S[l];
+    const int tiles_x = (w + TXW - 1) / TXW;
+    const int x0 = (pt.y % tiles_x) * TXW, y0 = (pt.y / tiles_x) * TY;
+    const int tid = threadIdx.y * TXW + threadIdx.x;
+    const int pd = a.pd;
+    const float *du0 = F.du[pd], *dv0 = F.dv[pd];
+    const TS *p1u0 = as<TS>(F.p1u[pd]), *p2u0 = as<TS>(F.p2u[pd]), *p1v0 = as<TS>(F.p1v[pd]),
+             *p2v0 = as<TS>(F.p2v[pd]);
+    const TS *Ix = as<TS>(F.Ix), *Iy = as<TS>(F.Iy), *Iz = as<TS>(F.Iz);
+    int hex = -1, hey = -1;
+    if (tid < NHALO) {
+        if (tid < EX) hex = tid, hey = TY;
+        else hex = TXW, hey = tid - EX;
+    }
+    float rp[RPT][4];
+    auto data = [&](int ex, int ey, float* keep) {
+        const int x = x0 + ex, y = y0 + ey;
+        if (x >= w || y >= h) return;
+        const long q = x + (long)y * s;
+        const bool hl = x > 0, hu = y > 0;
+        float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
+              p1vl = 0.0f, p2vd = 0.0f;
+        if (!a.first) {
+            duh = du0[q];
+            dvh = dv0[q];
+            p1u = p1u0[q];
+            p2u = p2u0[q];
+            p1v = p1v0[q];
+            p2v = p2v0[q];
+            if (hl) p1ul = p1u0[q - 1], p1vl = p1v0[q - 1];
+            if (hu) p2ud = p2u0[q - s], p2vd = p2v0[q - s];
+        }
+        float du, dv;
+        l1_threshold(Ix[q], Iy[q], Iz[q], duh, dvh, a.lambda, a.theta, du, dv);
+        const int e = ex + ey * EX;
+        s_du[e] = du + a.theta * div_p(p1u, p1ul, p2u, p2ud, hl, hu);
+        s_dv[e] = dv + a.theta * div_p(p1v, p1vl, p2v, p2vd, hl, hu);
+        if (keep) {
+            keep[0] = p1u;
+            keep[1] = p2u;
+            keep[2] = p1v;
+            keep[3] = p2v;
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) data(threadIdx.x, threadIdx.y + TRW * r, rp[r]);
+    if (hex >= 0) data(hex, hey, nullptr);
+    __syncthreads();
+    const float fac = a.tau / a.theta;
+    TS *p1u1 = as<TS>(F.p1u[pd ^ 1]), *p2u1 = as<TS>(F.p2u[pd ^ 1]), *p1v1 = as<TS>(F.p1v[pd ^ 1]),
+       *p2v1 = as<TS>(F.p2v[pd ^ 1]);
+    float *du1 = F.du[pd ^ 1], *dv1 = F.dv[pd ^ 1];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int ex = threadIdx.x, ey = threadIdx.y + TRW * r;
+        const int x = x0 + ex, y = y0 + ey;
+        if (x >= w || y >= h) continue;
+        const long q = x + (long)y * s;
+        const int e = ex + ey * EX;
+        float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
+        if (x + 1 < w) {
+            dux = s_du[e + 1] - s_du[e];
+            dvx = s_dv[e + 1] - s_dv[e];
+        }
+        if (y + 1 < h) {
+            duy = s_du[e + EX] - s_du[e];
+            dvy = s_dv[e + EX] - s_dv[e];
+        }
+        float o1, o2;
+        dual_px(dux, duy, rp[r][0], rp[r][1], fac, o1, o2);
+        p1u1[q] = st<TS>(o1);
+        p2u1[q] = st<TS>(o2);
+        dual_px(dvx, dvy, rp[r][2], rp[r][3], fac, o1, o2);
+        p1v1[q] = st<TS>(o1);
+        p2v1[q] = st<TS>(o2);
+        du1[q] = s_du[e];
+        dv1[q] = s_dv[e];
+    }
+}
+
+// Zero uhat (a warp with no solver iterations leaves the zeroed uhat of the reference).
+__global__ void __launch_bounds__(256) kb_zero_uhat(Launch L, int l, int pd) {
+    BT_PIX(L, l)
+    F.du[pd][pos] = 0.0f;
+    F.dv[pd][pos] = 0.0f;
+}
+
+}  // namespace
+
+// ===================================================================================
+// Batch
+
+Batch::~Batch() { release(); }
+
+void Batch::release() {
+    for (auto& e : prof_.ev) hipEventDestroy(e);
+    prof_.ev.clear();
+    for (auto& g : gexec_)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    if (maps_) hipFree(maps_);
+    if (dp_) hipFree(dp_);
+    if (slab_) hipFree(slab_);
+    if (st_) hipStreamDestroy(st_);
+    maps_ = nullptr;
+    dp_ = nullptr;
+    slab_ = nullptr;
+    st_ = nullptr;
+}
+
+int Batch::check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return RF_OK;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return RF_ERR_HIP;
+}
+
+static int align_up_b(int n, int m) {
+    int mod = n % m;
+    return mod ? n + m - mod : n;
+}
+
+int Batch::create(const rf_params& p, int n, const int* widths, const int* heights) {
+    if (n <= 0 || !widths || !heights || p.n_levels <= 0 || p.n_levels > kMaxLevels || !(p.scale > 1.0f)) {
+        set_error("rf_batch_create: invalid count, sizes or pyramid");
+        return RF_ERR_ARG;
+    }
+    if (p.input_type < RF_INPUT_32F || p.input_type > RF_INPUT_16U) {
+        set_error("rf_batch_create: unknown input type");
+        return RF_ERR_ARG;
+    }
+    const int align = p.stride_alignment > 0 ? p.stride_alignment : 32;
+    std::vector<Geo> geo(n);
+    for (int i = 0; i < n; ++i) {
+        int nh = heights[i], nw = widths[i];
+        if (nw <= 0 || nh <= 0) {
+            set_error("rf_batch_create: pair " + std::to_string(i) + " has no pixels");
+            return RF_ERR_ARG;
+        }
+        for (int l = 0; l < p.n_levels; ++l) {
+            if (nw < 1 || nh < 1) {
+                set_error("rf_batch_create: pair " + std::to_string(i) + " runs out of pixels at level " +
+                          std::to_string(l));
+                return RF_ERR_ARG;
+            }
+            geo[i].W.push_back(nw);
+            geo[i].H.push_back(nh);
+            geo[i].S.push_back(align_up_b(nw, align));
+            nh = (int)((float)nh / p.scale);
+            nw = (int)((float)nw / p.scale);
+        }
+        geo[i].n0 = (long)geo[i].S[0] * geo[i].H[0];
+        geo[i].bufn = ((geo[i].n0 + geo[i].S[0]) + 63) / 64 * 64;
+    }
+    release();
+    p_ = p;
+    geo_ = geo;
+    if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return RF_ERR_HIP;
+    const int L = p.n_levels;
+    const long per_fields = 2L * L + 52;  // pyramids + state/double buffers + prior + uvrgb(3) + scratch + inputs(2)
+    size_t total = 0;
+    for (auto& g : geo_) total += (size_t)per_fields * g.bufn * sizeof(float);
+    if (hipMalloc(&slab_, total) != hipSuccess) {
+        slab_ = nullptr;
+        set_error("rf_batch_create: device allocation of " + std::to_string(total) + " bytes failed");
+        release();
+        return RF_ERR_ALLOC;
+    }
+    if (check(hipMemset(slab_, 0, total), "hipMemset")) return RF_ERR_HIP;
+    hp_.assign(n, PairFields{});
+    float* cur = (float*)slab_;
+    for (int i = 0; i < n; ++i) {
+        PairFields& F = hp_[i];
+        const Geo& g = geo_[i];
+        auto take = [&](long k) {
+            float* r = cur;
+            cur += k * g.bufn;
+            return r;
+        };
+        for (int l = 0; l < L; ++l) {
+            F.W[l] = g.W[l];
+            F.H[l] = g.H[l];
+            F.S[l] = g.S[l];
+            F.I0[l] = take(1);
+            F.I1[l] = take(1);
+        }
+        F.in0 = take(1);
+        F.in1 = take(1);
+        F.i1warp = take(1);
+        F.Ix = take(1);
+        F.Iy = take(1);
+        F.Iz = take(1);
+        for (int k = 0; k < 2; ++k) {
+            F.du[k] = take(1);
+            F.dv[k] = take(1);
+            F.p1u[k] = take(1);
+            F.p2u[k] = take(1);
+            F.p1v[k] = take(1);
+            F.p2v[k] = take(1);
+            F.up[k] = take(1);
+            F.vp[k] = take(1);
+            F.sku[k] = take(1);
+            F.skv[k] = take(1);
+            F.uc[k] = take(1);
+            F.vc[k] = take(1);
+            F.X[k] = take(1);
+            F.Y[k] = take(1);
+            F.Z[k] = take(1);
+            F.u[k] = take(1);
+            F.v[k] = take(1);
+        }
+        F.ufn = take(1);
+        F.vfn = take(1);
+        F.ufn_l = take(1);
+        F.vfn_l = take(1);
+        F.mask = take(1);
+        F.mask_l = take(1);
+        F.uvrgb = take(3);
+        F.scratch = take(1);
+    }
+    if (hipMalloc((void**)&dp_, sizeof(PairFields) * n) != hipSuccess) {
+        dp_ = nullptr;
+        set_error("rf_batch_create: descriptor allocation failed");
+        release();
+        return RF_ERR_ALLOC;
+    }
+    // tile lists: levels 0..L-1 and the flat level-0 buffer; tile heights 4, 8, 16
+    const int tys[3] = {4, 4 * RPT_JOINT, 4 * RPT_FLOW};
+    sets_.assign((kMaxLevels + 1) * 3, TileSet{});
+    std::vector<int2> all;
+    std::vector<long> offs(sets_.size(), -1);
+    for (int lv = 0; lv <= kMaxLevels; ++lv) {
+        if (lv >= L && lv != FLAT) continue;
+        for (int c = 0; c < 3; ++c) {
+            if (lv == FLAT && c != 0) continue;
+            const int ty = tys[c];
+            offs[lv * 3 + c] = (long)all.size();
+            for (int i = 0; i < n; ++i) {
+                const int w = lv == FLAT ? geo_[i].S[0] : geo_[i].W[lv];
+                const int h = lv == FLAT ? geo_[i].H[0] : geo_[i].H[lv];
+                const int nt = ((w + TXW - 1) / TXW) * ((h + ty - 1) / ty);
+                for (int t = 0; t < nt; ++t) all.push_back(make_int2(i, t));
+            }
+            sets_[lv * 3 + c].ntiles = (int)(all.size() - offs[lv * 3 + c]);
+            sets_[lv * 3 + c].ty = ty;
+        }
+    }
+    if (hipMalloc((void**)&maps_, sizeof(int2) * std::max<size_t>(all.size(), 1)) != hipSuccess) {
+        maps_ = nullptr;
+        set_error("rf_batch_create: tile list allocation failed");
+        release();
+        return RF_ERR_ALLOC;
+    }
+    if (check(hipMemcpy(maps_, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice), "upload tiles"))
+        return RF_ERR_HIP;
+    for (size_t k = 0; k < sets_.size(); ++k)
+        if (offs[k] >= 0) sets_[k].map = maps_ + offs[k];
+    prior_ = false;
+    dirty_ = true;
+    return upload_pairs();
+}
+
+TileSet& Batch::tiles(int level, int ty) {
+    const int c = ty == 4 ? 0 : (ty == 4 * RPT_JOINT ? 1 : 2);
+    return sets_[level * 3 + c];
+}
+
+int Batch::upload_pairs() {
+    if (check(hipMemcpyAsync(dp_, hp_.data(), sizeof(PairFields) * hp_.size(), hipMemcpyHostToDevice, st_),
+              "upload pair descriptors"))
+        return RF_ERR_HIP;
+    if (check(hipStreamSynchronize(st_), "upload pair descriptors")) return RF_ERR_HIP;
+    dirty_ = false;
+    return RF_OK;
+}
+
+static size_t in_bytes(int t) {
+    switch (t) {
+        case RF_INPUT_8UC3: return 3;
+        case RF_INPUT_8U: return 1;
+        case RF_INPUT_16U: return 2;
+        default: return 4;
+    }
+}
+
+int Batch::set_images_host(int i, const void* i0, const void* i1) {
+    if (i < 0 || i >= n() || !i0 || !i1) {
+        set_error("rf_batch: bad pair index or null image");
+        return RF_ERR_ARG;
+    }
+    size_t bytes = (size_t)geo_[i].n0 * in_bytes(p_.input_type);
+    if (check(hipMemcpyAsync((void*)hp_[i].in0, i0, bytes, hipMemcpyHostToDevice, st_), "copy image 0") ||
+        check(hipMemcpyAsync((void*)hp_[i].in1, i1, bytes, hipMemcpyHostToDevice, st_), "copy image 1"))
+        return RF_ERR_HIP;
+    return check(hipStreamSynchronize(st_), "copy images");
+}
+
+int Batch::set_images_device(int i, const float* i0, const float* i1) {
+    if (i < 0 || i >= n() || !i0 || !i1) {
+        set_error("rf_batch_set_images_device: bad pair index or null image");
+        return RF_ERR_ARG;
+    }
+    if (p_.input_type != RF_INPUT_32F) {
+        set_error("rf_batch_set_images_device: the batch expects a non-float input type");
+        return RF_ERR_ARG;
+    }
+    size_t bytes = (size_t)geo_[i].n0 * 4;
+    if (check(hipMemcpyAsync((void*)hp_[i].in0, i0, bytes, hipMemcpyDeviceToDevice, st_), "copy image 0")) return RF_ERR_HIP;
+    return check(hipMemcpyAsync((void*)hp_[i].in1, i1, bytes, hipMemcpyDeviceToDevice, st_), "copy image 1");
+}
+
+// setCameraMatrices + the per-level P, Q (Reconflow.cu:78-116, 225-230).
+int Batch::set_camera(int i, const double* K0, const double* K1, const double* R0, const double* t0,
+                      const double* R1, const double* t1) {
+    if (i < 0 || i >= n() || !K0 || !K1 || !R1 || !t1) {
+        set_error("rf_batch_set_camera: bad pair index or null matrix");
+        return RF_ERR_ARG;
+    }
+    static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Z3[3] = {0, 0, 0};
+    if (!R0) R0 = I3;
+    if (!t0) t0 = Z3;
+    double k0[9], k1[9];
+    memcpy(k0, K0, sizeof k0);
+    memcpy(k1, K1, sizeof k1);
+    for (int l = 0; l < p_.n_levels; ++l) {
+        if (l > 0) {
+            for (double* K : {k0, k1}) {
+                double nk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                nk[0] = K[0] / (double)p_.scale;
+                nk[2] = K[2] / (double)p_.scale;
+                nk[4] = K[4] / (double)p_.scale;
+                nk[5] = K[5] / (double)p_.scale;
+                if (p_.fix_coarse_k) nk[8] = 1.0;
+                memcpy(K, nk, sizeof nk);
+            }
+        }
+        double P[12], Q[12];
+        for (int pass = 0; pass < 2; ++pass) {
+            const double* K = pass ? k1 : k0;
+            const double* R = pass ? R1 : R0;
+            const double* t = pass ? t1 : t0;
+            double* out = pass ? Q : P;
+            for (int a = 0; a < 3; ++a) {
+                for (int b = 0; b < 3; ++b) {
+                    double acc = 0;
+                    for (int k = 0; k < 3; ++k) acc += K[a * 3 + k] * R[k * 3 + b];
+                    out[a * 4 + b] = acc;
+                }
+                double acc = 0;
+                for (int k = 0; k < 3; ++k) acc += K[a * 3 + k] * t[k];
+                out[a * 4 + 3] = acc;
+            }
+        }
+        hp_[i].pq[l] = make_pq(P, Q);
+        hp_[i].pqd[l] = make_pqd(P, Q);
+        hp_[i].ucp[l] = make_uc(R1, t1, k1);
+    }
+    dirty_ = true;
+    return RF_OK;
+}
+
+int Batch::set_prior(int i, const float* u, const float* v, const float* mask, bool sparse) {
+    if (i < 0 || i >= n() || !u || !v) {
+        set_error("rf_batch: bad pair index or null prior");
+        return RF_ERR_ARG;
+    }
+    size_t bytes = (size_t)geo_[i].n0 * 4;
+    if (check(hipMemcpyAsync(hp_[i].ufn, u, bytes, hipMemcpyHostToDevice, st_), "copy prior u") ||
+        check(hipMemcpyAsync(hp_[i].vfn, v, bytes, hipMemcpyHostToDevice, st_), "copy prior v"))
+        return RF_ERR_HIP;
+    std::vector<float> ones;
+    if (!mask) {
+        ones.assign(geo_[i].n0, 1.0f);
+        mask = ones.data();
+    }
+    if (check(hipMemcpyAsync(hp_[i].mask, mask, bytes, hipMemcpyHostToDevice, st_), "copy mask")) return RF_ERR_HIP;
+    sparse_ = sparse;
+    prior_ = true;
+    return check(hipStreamSynchronize(st_), "copy prior");
+}
+
+#define RF_TRY(x)                          \
+    do {                                   \
+        hipError_t e__ = (x);              \
+        if (e__ != hipSuccess) return e__; \
+    } while (0)
+
+#define RF_LAUNCH(K, TS, ...)                                                          \
+    do {                                                                               \
+        const TileSet& ts__ = (TS);                                                    \
+        if (ts__.ntiles > 0) {                                                         \
+            Launch L__{dp_, ts__.map, ts__.ntiles};                                    \
+            hipLaunchKernelGGL(K, dim3(ts__.ntiles), dim3(TXW, TRW), 0, st_, L__, ##__VA_ARGS__); \
+            RF_TRY(hipGetLastError());                                                 \
+        }                                                                              \
+    } while (0)
+
+hipError_t Batch::enqueue(bool recon) {
+    return p_.storage == RF_STORAGE_F16 ? enqueue_t<__half>(recon) : enqueue_t<float>(recon);
+}
+
+template <typename TS>
+hipError_t Batch::enqueue_t(bool recon) {
+    const int L = p_.n_levels;
+    Parity pr{};
+    RF_LAUNCH(kb_gray<TS>, tiles(0, 4), p_.input_type);
+    for (int l = 1; l < L; ++l) RF_LAUNCH(kb_pyramid<TS>, tiles(l, 4), l);
+    RF_LAUNCH(kb_zero, tiles(FLAT, 4), recon ? 1 : 0);
+    const bool fn = recon && prior_ && (p_.method == RF_METHODR_TVL1_MS_FN || p_.method == RF_METHODR_TVL1_MS_FNSPARSE);
+    const float theta = recon ? 1.0f / p_.alpha_tv : p_.theta;  // Reconflow.cu:32
+    for (int level = L - 1; level >= 0; --level) {
+        if (recon) {
+            if (fn) RF_LAUNCH(kb_prior, tiles(level, 4), level);
+            RF_LAUNCH(kb_sync_x, tiles(FLAT, 4), pr.x);
+        }
+        for (int wi = 0; wi < p_.n_warp_iters; ++wi) {
+            RF_LAUNCH(kb_warp<TS>, tiles(level, 4), level, pr.u);
+            RF_LAUNCH(kb_deriv<TS>, tiles(level, 4), level);
+            if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, 4), level, pr.d);
+            prof_.begin(st_, RF_KID_FUSED_ITER, level);
+            for (int it = 0; it < p_.n_solver_iters; ++it) {
+                const TileSet& ts = tiles(level, recon ? 4 * RPT_JOINT : 4 * RPT_FLOW);
+                IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
+                           p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
+                           (fn && sparse_) ? 1 : 0};
+                const dim3 grid(ts.ntiles), blk(TXW, TRW);
+                if (ts.ntiles > 0) {
+                    if (!recon) hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), grid, blk, 0, st_, a);
+                    else if (fn && p_.tri_f64)
+                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, true, true, TS>), grid, blk, 0, st_, a);
+                    else if (fn)
+                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, false, true, TS>), grid, blk, 0, st_, a);
+                    else if (p_.tri_f64)
+                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, true, false, TS>), grid, blk, 0, st_, a);
+                    else
+                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, false, false, TS>), grid, blk, 0, st_, a);
+                    RF_TRY(hipGetLastError());
+                }
+                pr.d ^= 1;
+                if (recon) {
+                    pr.f ^= 1;
+                    pr.x ^= 1;
+                }
+            }
+            prof_.end(st_, RF_KID_FUSED_ITER, level, p_.n_solver_iters);
+            RF_LAUNCH(kb_med5_add, tiles(level, 4), level, pr.d, pr.u);
+            if (recon) {
+                RF_LAUNCH(kb_median5, tiles(level, 4), level, pr.u);
+                pr.u ^= 1;
+            }
+        }
+        if (level > 0) {
+            RF_LAUNCH(kb_upscale, tiles(level - 1, 4), level - 1, pr.u, pr.f, recon ? 1 : 0, p_.scale);
+            pr.u ^= 1;
+            if (recon) pr.f ^= 1;
+        }
+    }
+    if (p_.with_visualization) RF_LAUNCH(kb_hsv, tiles(0, 4), pr.u, flow_scale_);
+    final_ = pr;
+    return hipSuccess;
+}
+
+int Batch::solve(bool recon, float flow_scale) {
+    if (!st_) {
+        set_error("rf_batch_solve: batch not created");
+        return RF_ERR_STATE;
+    }
+    if (dirty_ && upload_pairs()) return RF_ERR_HIP;
+    const int kind = recon ? 1 : 0;
+    if (p_.with_visualization && flow_scale != flow_scale_ && gexec_[kind]) {
+        hipGraphExecDestroy(gexec_[kind]);
+        gexec_[kind] = nullptr;
+    }
+    flow_scale_ = flow_scale;
+    if (!p_.use_graph || prof_.armed) {
+        if (check(enqueue(recon), "batch solve")) return RF_ERR_HIP;
+        return RF_OK;
+    }
+    if (!gexec_[kind]) {
+        hipGraph_t graph = nullptr;
+        if (check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+            return RF_ERR_HIP;
+        hipError_t e = enqueue(recon);
+        hipError_t e2 = hipStreamEndCapture(st_, &graph);
+        if (check(e, "batch solve (capture)") || check(e2, "hipStreamEndCapture")) {
+            if (graph) hipGraphDestroy(graph);
+            return RF_ERR_HIP;
+        }
+        hipError_t e3 = hipGraphInstantiate(&gexec_[kind], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (check(e3, "hipGraphInstantiate")) {
+            gexec_[kind] = nullptr;
+            return RF_ERR_HIP;
+        }
+        gfinal_[kind] = final_;
+    }
+    final_ = gfinal_[kind];
+    return check(hipGraphLaunch(gexec_[kind], st_), "hipGraphLaunch");
+}
+
+int Batch::sync() { return check(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
+
+void Batch::result(int i, const float** u, const float** v, const float** X, const float** Y,
+                   const float** Z) const {
+    const PairFields& F = hp_[i];
+    if (u) *u = F.u[final_.u];
+    if (v) *v = F.v[final_.u];
+    if (X) *X = F.X[final_.x];
+    if (Y) *Y = F.Y[final_.x];
+    if (Z) *Z = F.Z[final_.x];
+}
+
+// removeEdgeArtifacts3D on the result (Reconflow.cu:468-477): filtered into the partner X
+// buffers, which then become the result.
+int Batch::remove_edge(int i, int range, float maxarea) {
+    const PairFields& F = hp_[i];
+    const int x = final_.x;
+    hipError_t e = launch_remove_edge(F.X[x], F.Y[x], F.Z[x], range, maxarea, geo_[i].W[0], geo_[i].H[0], geo_[i].S[0],
+                                      F.X[x ^ 1], F.Y[x ^ 1], F.Z[x ^ 1], st_);
+    if (check(e, "remove edge artifacts")) return RF_ERR_HIP;
+    final_.x ^= 1;
+    return RF_OK;
+}
+
+namespace {
+// Middlebury colour wheel (flowToHSV.cu:194-302), batched form of the per-step kernel.
+__constant__ float cb_wheel[3][55] = {
+    {255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 213, 170, 128,
+     85,  43,  0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   19,
+     39,  58,  78,  98,  117, 137, 156, 176, 196, 215, 235, 255, 255, 255, 255, 255, 255},
+    {0,   17,  34,  51,  68,  85,  102, 119, 136, 153, 170, 187, 204, 221, 238, 255, 255, 255, 255,
+     255, 255, 255, 255, 255, 255, 255, 232, 209, 186, 163, 140, 116, 93,  70,  47,  24,  0,   0,
+     0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0},
+    {0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,
+     0,   0,   0,   63,  127, 191, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255,
+     255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 213, 170, 128, 85,  43}};
+__global__ void __launch_bounds__(256) kb_hsv(Launch L, int pu, float flowscale) {
+    BT_PIX(L, 0)
+    float du = F.u[pu][pos] / flowscale, dv = F.v[pu][pos] / flowscale;
+    const int ncols = 55;
+    float rad = sqrtf(du * du + dv * dv);
+    float a = atan2f(-dv, -du) / 3.14159f;
+    float fk = (a + 1.0f) / 2.0f * ((float)ncols - 1.0f) + 1.0f;
+    int k0 = (int)floorf(fk);
+    int k1 = k0 + 1;
+    if (k1 == ncols + 1) k1 = 1;
+    float f = fk - (float)k0;
+    k0 = ((k0 % ncols) + ncols) % ncols;
+    k1 = ((k1 % ncols) + ncols) % ncols;
+    float col[3];
+    for (int c = 0; c < 3; ++c) {
+        float cc = (1.0f - f) * (cb_wheel[c][k0] / 255.0f) + f * (cb_wheel[c][k1] / 255.0f);
+        col[c] = rad <= 1.0f ? 1.0f - rad * (1.0f - cc) : cc * 0.75f;
+    }
+    F.uvrgb[3 * pos + 0] = col[2];
+    F.uvrgb[3 * pos + 1] = col[1];
+    F.uvrgb[3 * pos + 2] = col[0];
+}
+}  // namespace
+
+}  // namespace rf
+
+// ===================================================================================
+// C ABI
+struct rf_batch {
+    rf::Batch b;
+};
 
 extern "C" {
 
 int rf_batch_create(rf_batch** out, const rf_params* params, int n, const int* widths, const int* heights) {
-    (void)params;
-    (void)n;
-    (void)widths;
-    (void)heights;
-    if (out) *out = nullptr;
-    rf::set_error("rf_batch_create: not implemented");
-    return RF_ERR_UNSUPPORTED;
-}
-int rf_batch_destroy(rf_batch* b) {
-    (void)b;
+    if (!out || !params) {
+        rf::set_error("rf_batch_create: null argument");
+        return RF_ERR_ARG;
+    }
+    *out = nullptr;
+    auto* b = new rf_batch;
+    int rc = b->b.create(*params, n, widths, heights);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
     return RF_OK;
 }
+
+int rf_batch_destroy(rf_batch* b) {
+    delete b;
+    return RF_OK;
+}
+
 int rf_batch_stride(const rf_batch* b, int i) {
-    (void)b;
-    (void)i;
-    return RF_ERR_UNSUPPORTED;
+    if (!b || i < 0 || i >= b->b.n()) return RF_ERR_ARG;
+    return b->b.stride(i);
 }
+
 int rf_batch_set_images_device(rf_batch* b, int i, const float* d_i0, const float* d_i1) {
-    (void)b;
-    (void)i;
-    (void)d_i0;
-    (void)d_i1;
-    return RF_ERR_UNSUPPORTED;
+    if (!b) return RF_ERR_ARG;
+    return b->b.set_images_device(i, d_i0, d_i1);
 }
+
 int rf_batch_set_camera(rf_batch* b, int i, const double K0[9], const double K1[9], const double R1[9],
                         const double t1[3]) {
-    (void)b;
-    (void)i;
-    (void)K0;
-    (void)K1;
-    (void)R1;
-    (void)t1;
-    return RF_ERR_UNSUPPORTED;
+    if (!b) return RF_ERR_ARG;
+    return b->b.set_camera(i, K0, K1, nullptr, nullptr, R1, t1);
 }
+
 int rf_batch_solve(rf_batch* b) {
-    (void)b;
-    return RF_ERR_UNSUPPORTED;
+    if (!b) return RF_ERR_ARG;
+    const int m = b->b.params().method;
+    return b->b.solve(m >= RF_METHODR_TVL1_MS, 5.0f);
 }
+
 int rf_batch_synchronize(rf_batch* b) {
-    (void)b;
-    return RF_ERR_UNSUPPORTED;
+    if (!b) return RF_ERR_ARG;
+    return b->b.sync();
 }
+
 int rf_batch_get_result_device(const rf_batch* b, int i, const float** u, const float** v, const float** X,
                                const float** Y, const float** Z) {
-    (void)b;
-    (void)i;
-    (void)u;
-    (void)v;
-    (void)X;
-    (void)Y;
-    (void)Z;
-    return RF_ERR_UNSUPPORTED;
+    if (!b || i < 0 || i >= b->b.n()) return RF_ERR_ARG;
+    b->b.result(i, u, v, X, Y, Z);
+    return RF_OK;
 }
-void* rf_batch_get_stream(const rf_batch* b) {
-    (void)b;
-    return nullptr;
-}
+
+void* rf_batch_get_stream(const rf_batch* b) { return b ? (void*)b->b.stream() : nullptr; }
 
 }  // extern "C"

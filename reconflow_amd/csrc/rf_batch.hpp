@@ -1,0 +1,104 @@
+// Batched fused solve: many image pairs of uneven sizes advance through one static schedule,
+// every launch covering the tiles of all pairs at that step. A single pair is a batch of one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/rfhip.h"
+#include "rf_device.hpp"
+#include "rf_solver.hpp"
+
+namespace rf {
+
+// Device-resident descriptor of one pair. Double-buffered fields are indexed by a parity
+// the host tracks (the same for every pair of a batch, since the schedule is shared).
+struct PairFields {
+    int W[kMaxLevels], H[kMaxLevels], S[kMaxLevels];
+    float* I0[kMaxLevels];
+    float* I1[kMaxLevels];
+    const void *in0, *in1;
+    float *i1warp, *Ix, *Iy, *Iz;
+    float *du[2], *dv[2], *p1u[2], *p2u[2], *p1v[2], *p2v[2];
+    float *up[2], *vp[2], *sku[2], *skv[2], *uc[2], *vc[2];
+    float *X[2], *Y[2], *Z[2];
+    float *u[2], *v[2];
+    float *ufn, *vfn, *ufn_l, *vfn_l, *mask, *mask_l;
+    float* uvrgb;
+    float* scratch;  // spare field (edge-artifact filter output)
+    PQ pq[kMaxLevels];
+    PQd pqd[kMaxLevels];
+    UcParams ucp[kMaxLevels];
+};
+
+// One launch's tile list: (pair, tile index within the pair) for tiles of 64 x ty pixels.
+struct TileSet {
+    int2* map = nullptr;  // device
+    int ntiles = 0;
+    int ty = 4;
+};
+
+// Buffer parities of the shared schedule.
+struct Parity {
+    int d = 0;  // uhat, p
+    int f = 0;  // uproj, sk, uc
+    int x = 0;  // X, Y, Z
+    int u = 0;  // u, v
+};
+
+class Batch {
+  public:
+    Batch() = default;
+    ~Batch();
+    int create(const rf_params& p, int n, const int* widths, const int* heights);
+    int n() const { return (int)geo_.size(); }
+    int stride(int i) const { return geo_[i].S[0]; }
+    int set_images_host(int i, const void* i0, const void* i1);
+    int set_images_device(int i, const float* i0, const float* i1);
+    int set_camera(int i, const double* K0, const double* K1, const double* R0, const double* t0, const double* R1,
+                   const double* t1);
+    int set_prior(int i, const float* u, const float* v, const float* mask, bool sparse);
+    void enable_prior() { prior_ = true; }
+    int solve(bool recon, float flow_scale);
+    int sync();
+    void result(int i, const float** u, const float** v, const float** X, const float** Y, const float** Z) const;
+    float* result_uvrgb(int i) const { return hp_[i].uvrgb; }
+    int remove_edge(int i, int range, float maxarea);
+    hipStream_t stream() const { return st_; }
+    Profiler& prof() { return prof_; }
+    const rf_params& params() const { return p_; }
+
+  private:
+    struct Geo {
+        std::vector<int> W, H, S;
+        long n0 = 0, bufn = 0;
+    };
+    hipError_t enqueue(bool recon);
+    template <typename TS>
+    hipError_t enqueue_t(bool recon);
+    int check(hipError_t e, const char* what);
+    int upload_pairs();
+    TileSet& tiles(int level, int ty);
+    void release();
+
+    rf_params p_{};
+    std::vector<Geo> geo_;
+    std::vector<PairFields> hp_;
+    PairFields* dp_ = nullptr;
+    void* slab_ = nullptr;
+    hipStream_t st_ = nullptr;
+    bool prior_ = false, sparse_ = false;
+    float flow_scale_ = 5.0f;
+    // tile sets: [level][ty class]; ty classes 4 (pointwise), 8 (joint), 16 (TV-L1); level
+    // kMaxLevels is the flat level-0 buffer (stride x height)
+    std::vector<TileSet> sets_;
+    int2* maps_ = nullptr;
+    Parity final_{};
+    bool dirty_ = true;
+    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    Parity gfinal_[2];
+    Profiler prof_;
+};
+
+}  // namespace rf

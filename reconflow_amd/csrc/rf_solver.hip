@@ -14,7 +14,7 @@
 #include <algorithm>
 #include <string>
 
-#include "rf_fused.hpp"
+#include "rf_batch.hpp"
 #include "rf_kernels.hpp"
 #include "rf_solver.hpp"
 
@@ -39,6 +39,7 @@ int Solver::check(hipError_t e, const char* what) {
 }
 
 int Solver::release() {
+    batch_.reset();
     for (auto& e : prof_.ev) hipEventDestroy(e);
     prof_.ev.clear();
     prof_.armed = false;
@@ -100,6 +101,20 @@ int Solver::initialize(const rf_params& p) {
     g_ = g;
     p_ = p;
     align_ = align;
+    if (p.path == RF_PATH_FUSED) {
+        batch_.reset(new Batch());
+        int rc = batch_->create(p, 1, &p.width, &p.height);
+        if (rc) {
+            batch_.reset();
+            g_ = Geometry();
+            return rc;
+        }
+        g_.n = (long)g_.S[0] * g_.H[0];
+        inited_ = true;
+        prior_ = false;
+        camera_ = false;
+        return RF_OK;
+    }
     const long s0 = g_.S[0], h0 = g_.H[0];
     g_.n = s0 * h0;
     g_.bufn = ((g_.n + s0) + 63) / 64 * 64;
@@ -179,6 +194,7 @@ int Solver::init_large_displacement() {
         return RF_ERR_STATE;
     }
     prior_ = true;
+    if (batch_) batch_->enable_prior();
     return RF_OK;
 }
 
@@ -189,6 +205,8 @@ int Solver::set_camera(const double* K0, const double* K1) {
         set_error("rf_set_camera: not initialized or null matrix");
         return inited_ ? RF_ERR_ARG : RF_ERR_STATE;
     }
+    memcpy(K0raw_, K0, sizeof K0raw_);
+    memcpy(K1raw_, K1, sizeof K1raw_);
     K0_.assign(g_.L, std::vector<double>(9, 0.0));
     K1_.assign(g_.L, std::vector<double>(9, 0.0));
     for (int i = 0; i < 9; ++i) {
@@ -222,6 +240,7 @@ int Solver::copy_images(const void* i0, const void* i1) {
         set_error("rf_copy_images: not initialized or null image");
         return inited_ ? RF_ERR_ARG : RF_ERR_STATE;
     }
+    if (batch_) return batch_->set_images_host(0, i0, i1);
     size_t bytes = (size_t)g_.n * input_bytes_per_px(p_.input_type);
     if (check(hipMemcpyAsync(init_.in0, i0, bytes, hipMemcpyHostToDevice, st_), "copy image 0")) return RF_ERR_HIP;
     if (check(hipMemcpyAsync(init_.in1, i1, bytes, hipMemcpyHostToDevice, st_), "copy image 1")) return RF_ERR_HIP;
@@ -237,6 +256,7 @@ int Solver::copy_images_device(const float* i0, const float* i1) {
         set_error("rf_copy_images_device: the context expects a non-float input type");
         return RF_ERR_ARG;
     }
+    if (batch_) return batch_->set_images_device(0, i0, i1);
     size_t bytes = (size_t)g_.n * 4;
     if (check(hipMemcpyAsync(init_.in0, i0, bytes, hipMemcpyDeviceToDevice, st_), "copy image 0")) return RF_ERR_HIP;
     return check(hipMemcpyAsync(init_.in1, i1, bytes, hipMemcpyDeviceToDevice, st_), "copy image 1");
@@ -247,6 +267,10 @@ int Solver::copy_prior(const float* u, const float* v, const float* mask, bool s
     if (!inited_ || !prior_ || !u || !v) {
         set_error("rf_copy_*_flow: call rf_initialize_large_displacement first and pass u and v");
         return RF_ERR_STATE;
+    }
+    if (batch_) {
+        sparse_ = sparse;
+        return batch_->set_prior(0, u, v, mask, sparse);
     }
     size_t bytes = (size_t)g_.n * 4;
     if (check(hipMemcpyAsync(init_.ufn, u, bytes, hipMemcpyHostToDevice, st_), "copy prior u")) return RF_ERR_HIP;
@@ -445,57 +469,60 @@ hipError_t Solver::enqueue_recon_plain(Fields& f) {
     return hipSuccess;
 }
 
-hipError_t Solver::enqueue_flow_fused(Fields& f) {
-    FusedCtx c{g_, p_, st_, dparams_, prior_, sparse_, flow_scale_, &prof_};
-    return fused_flow(c, f);
-}
-
-hipError_t Solver::enqueue_recon_fused(Fields& f) {
-    FusedCtx c{g_, p_, st_, dparams_, prior_, sparse_, flow_scale_, &prof_};
-    return fused_recon(c, f);
-}
-
 int Solver::profile_arm(int kid, int level) {
     if (!inited_) {
         set_error("rf_profile_arm: not initialized");
         return RF_ERR_STATE;
     }
-    if (prof_.ev.empty()) {
-        prof_.ev.resize(8192);
-        for (auto& e : prof_.ev)
+    Profiler& pr = batch_ ? batch_->prof() : prof_;
+    if (pr.ev.empty()) {
+        pr.ev.resize(8192);
+        for (auto& e : pr.ev)
             if (check(hipEventCreate(&e), "hipEventCreate")) return RF_ERR_HIP;
     }
-    prof_.armed = true;
-    prof_.kid = kid;
-    prof_.level = level;
-    prof_.used = 0;
-    prof_.launches = 0;
-    prof_.open = false;
+    pr.armed = true;
+    pr.kid = kid;
+    pr.level = level;
+    pr.used = 0;
+    pr.launches = 0;
+    pr.open = false;
     return RF_OK;
 }
 
 int Solver::profile_read(double* total_ms, int* launches) {
-    if (check(hipStreamSynchronize(st_), "rf_profile_read")) return RF_ERR_HIP;
+    Profiler& pr = batch_ ? batch_->prof() : prof_;
+    if (check(hipStreamSynchronize(stream()), "rf_profile_read")) return RF_ERR_HIP;
     double tot = 0;
-    for (int i = 0; i + 1 < prof_.used; i += 2) {
+    for (int i = 0; i + 1 < pr.used; i += 2) {
         float ms = 0;
-        if (check(hipEventElapsedTime(&ms, prof_.ev[i], prof_.ev[i + 1]), "hipEventElapsedTime")) return RF_ERR_HIP;
+        if (check(hipEventElapsedTime(&ms, pr.ev[i], pr.ev[i + 1]), "hipEventElapsedTime")) return RF_ERR_HIP;
         tot += ms;
     }
     if (total_ms) *total_ms = tot;
-    if (launches) *launches = prof_.launches;
-    prof_.armed = false;
+    if (launches) *launches = pr.launches;
+    pr.armed = false;
     return RF_OK;
+}
+
+hipStream_t Solver::stream() const { return batch_ ? batch_->stream() : st_; }
+
+void Solver::result(const float** u, const float** v, const float** X, const float** Y, const float** Z) const {
+    if (batch_) {
+        batch_->result(0, u, v, X, Y, Z);
+        return;
+    }
+    if (u) *u = cur_.u;
+    if (v) *v = cur_.v;
+    if (X) *X = cur_.X;
+    if (Y) *Y = cur_.Y;
+    if (Z) *Z = cur_.Z;
 }
 
 // Runs one solve. The pointer table restarts from its initial assignment every solve, so the
 // schedule (and a graph captured from it) is identical from solve to solve.
 int Solver::run_schedule(bool recon) {
     const int kind = recon ? 1 : 0;
-    auto enqueue = [&](Fields& f) -> hipError_t {
-        if (p_.path == RF_PATH_FUSED) return recon ? enqueue_recon_fused(f) : enqueue_flow_fused(f);
-        return recon ? enqueue_recon_plain(f) : enqueue_flow_plain(f);
-    };
+    auto enqueue = [&](Fields& f) -> hipError_t { return recon ? enqueue_recon_plain(f) : enqueue_flow_plain(f); };
     if (!p_.use_graph || prof_.armed) {
         Fields f = init_;
         if (check(enqueue(f), "solve")) return RF_ERR_HIP;
@@ -535,6 +562,7 @@ int Solver::solve_flow(float flow_scale) {
         set_error("rf_solve_flow: only METHOD_TVL1 is implemented");
         return RF_ERR_UNSUPPORTED;
     }
+    if (batch_) return batch_->solve(false, flow_scale);
     if (p_.use_graph && p_.with_visualization && flow_scale != flow_scale_ && gexec_[0]) {
         hipGraphExecDestroy(gexec_[0]);
         gexec_[0] = nullptr;
@@ -570,6 +598,10 @@ int Solver::solve_recon(const double* R0, const double* t0, const double* R1, co
     memcpy(t0_, t0 ? t0 : Z3, sizeof t0_);
     memcpy(R1_, R1, sizeof R1_);
     memcpy(t1_, t1, sizeof t1_);
+    if (batch_) {
+        int rc = batch_->set_camera(0, K0raw_, K1raw_, R0_, t0_, R1_, t1_);
+        return rc ? rc : batch_->solve(true, flow_scale);
+    }
     // P = K0[l]*[R0|t0], Q = K1[l]*[R1|t1] in double (Reconflow.cu:225-226), cast to float
     for (int l = 0; l < g_.L; ++l) {
         double P[12], Q[12];
@@ -605,7 +637,7 @@ int Solver::solve_recon(const double* R0, const double* t0, const double* R1, co
     return run_schedule(true);
 }
 
-int Solver::sync() { return check(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
+int Solver::sync() { return check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
 
 // CudaFlow::copyOpticalFlowToHost (Cudaflow.cu:313-324).
 int Solver::copy_flow(float* u, float* v, float* uvrgb) {
@@ -614,11 +646,15 @@ int Solver::copy_flow(float* u, float* v, float* uvrgb) {
         return inited_ ? RF_ERR_ARG : RF_ERR_STATE;
     }
     size_t bytes = (size_t)g_.n * 4;
-    if (check(hipMemcpyAsync(u, cur_.u, bytes, hipMemcpyDeviceToHost, st_), "copy u")) return RF_ERR_HIP;
-    if (check(hipMemcpyAsync(v, cur_.v, bytes, hipMemcpyDeviceToHost, st_), "copy v")) return RF_ERR_HIP;
-    if (uvrgb && check(hipMemcpyAsync(uvrgb, cur_.uvrgb, bytes * 3, hipMemcpyDeviceToHost, st_), "copy uvrgb"))
+    const float *du, *dv;
+    result(&du, &dv, nullptr, nullptr, nullptr);
+    const float* drgb = batch_ ? batch_->result_uvrgb(0) : cur_.uvrgb;
+    hipStream_t st = stream();
+    if (check(hipMemcpyAsync(u, du, bytes, hipMemcpyDeviceToHost, st), "copy u")) return RF_ERR_HIP;
+    if (check(hipMemcpyAsync(v, dv, bytes, hipMemcpyDeviceToHost, st), "copy v")) return RF_ERR_HIP;
+    if (uvrgb && check(hipMemcpyAsync(uvrgb, drgb, bytes * 3, hipMemcpyDeviceToHost, st), "copy uvrgb"))
         return RF_ERR_HIP;
-    return check(hipStreamSynchronize(st_), "copy flow");
+    return check(hipStreamSynchronize(st), "copy flow");
 }
 
 // ReconFlow::copy3dToHost (Reconflow.cu:65-70).
@@ -628,10 +664,13 @@ int Solver::copy_3d(float* X, float* Y, float* Z) {
         return inited_ ? RF_ERR_ARG : RF_ERR_STATE;
     }
     size_t bytes = (size_t)g_.n * 4;
-    if (check(hipMemcpyAsync(X, cur_.X, bytes, hipMemcpyDeviceToHost, st_), "copy X")) return RF_ERR_HIP;
-    if (check(hipMemcpyAsync(Y, cur_.Y, bytes, hipMemcpyDeviceToHost, st_), "copy Y")) return RF_ERR_HIP;
-    if (check(hipMemcpyAsync(Z, cur_.Z, bytes, hipMemcpyDeviceToHost, st_), "copy Z")) return RF_ERR_HIP;
-    return check(hipStreamSynchronize(st_), "copy 3d");
+    const float *dX, *dY, *dZ;
+    result(nullptr, nullptr, &dX, &dY, &dZ);
+    hipStream_t st = stream();
+    if (check(hipMemcpyAsync(X, dX, bytes, hipMemcpyDeviceToHost, st), "copy X")) return RF_ERR_HIP;
+    if (check(hipMemcpyAsync(Y, dY, bytes, hipMemcpyDeviceToHost, st), "copy Y")) return RF_ERR_HIP;
+    if (check(hipMemcpyAsync(Z, dZ, bytes, hipMemcpyDeviceToHost, st), "copy Z")) return RF_ERR_HIP;
+    return check(hipStreamSynchronize(st), "copy 3d");
 }
 
 // ReconFlow::removeEdgeArtifacts3D (Reconflow.cu:468-477): X -> Xs, then swap.
@@ -644,6 +683,7 @@ int Solver::remove_edge(int range, float maxarea) {
         set_error("rf_remove_edge_artifacts_3d: surface width must lie in [1, min(width, height))");
         return RF_ERR_ARG;
     }
+    if (batch_) return batch_->remove_edge(0, range, maxarea);
     Fields& f = cur_;
     if (check(launch_remove_edge(f.X, f.Y, f.Z, range, maxarea, g_.W[0], g_.H[0], g_.S[0], f.Xs, f.Ys, f.Zs, st_),
               "remove edge artifacts"))
@@ -823,12 +863,7 @@ int rf_copy_3d(rf_context* ctx, float* X, float* Y, float* Z) {
 int rf_get_result_device(const rf_context* ctx, const float** u, const float** v, const float** X, const float** Y,
                          const float** Z) {
     RF_CTX(ctx);
-    const auto& r = ctx->s->result();
-    if (u) *u = r.u;
-    if (v) *v = r.v;
-    if (X) *X = r.X;
-    if (Y) *Y = r.Y;
-    if (Z) *Z = r.Z;
+    ctx->s->result(u, v, X, Y, Z);
     return RF_OK;
 }
 

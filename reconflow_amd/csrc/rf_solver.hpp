@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,9 @@ struct Geometry {
     long bufn = 0;  // floats per field buffer (n + guard row, rounded to 64)
 };
 
+class Batch;
+
+// One image pair. The per-step path runs here; the fused path runs as a batch of one pair.
 class Solver {
   public:
     Solver(int bw, int bh, int align);
@@ -95,16 +99,14 @@ class Solver {
 
     const Geometry& geom() const { return g_; }
     const rf_params& params() const { return p_; }
-    hipStream_t stream() const { return st_; }
-    const Fields& result() const { return cur_; }
+    hipStream_t stream() const;
+    void result(const float** u, const float** v, const float** X, const float** Y, const float** Z) const;
 
   private:
     int release();
     int run_schedule(bool recon);
     hipError_t enqueue_flow_plain(Fields& f);
     hipError_t enqueue_recon_plain(Fields& f);
-    hipError_t enqueue_flow_fused(Fields& f);
-    hipError_t enqueue_recon_fused(Fields& f);
     hipError_t enqueue_gray_pyramid(Fields& f);
     int check(hipError_t e, const char* what);
 
@@ -125,6 +127,8 @@ class Solver {
     hipGraphExec_t gexec_[2] = {nullptr, nullptr};
     Fields gfinal_[2];
     Profiler prof_;
+    std::unique_ptr<Batch> batch_;  // the fused path
+    double K0raw_[9], K1raw_[9];
 };
 
 }  // namespace rf

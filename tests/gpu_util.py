@@ -31,6 +31,28 @@ def ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+_hip = None
+
+
+def fetch(dptr_value, n):
+    """n floats at a raw device address -> host numpy (hipMemcpy device-to-host, after a device
+    synchronisation)."""
+    global _hip
+    torch = torch_cuda()
+    if _hip is None:
+        from reconflow_amd import _native
+        _native.lib()  # binds the process's single HIP runtime first
+        _hip = C.CDLL("libamdhip64.so.7")  # by soname: the runtime already loaded
+        _hip.hipDeviceSynchronize.restype = C.c_int
+        _hip.hipMemcpy.restype = C.c_int
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    torch.cuda.synchronize()
+    assert _hip.hipDeviceSynchronize() == 0
+    out = np.empty(n, np.float32)
+    assert _hip.hipMemcpy(out.ctypes.data, dptr_value, n * 4, 2) == 0  # 2 = device to host
+    return out
+
+
 def dptr(a):
     a = np.ascontiguousarray(np.asarray(a, np.float64).ravel())
     return a.ctypes.data_as(C.POINTER(C.c_double)), a

@@ -278,7 +278,7 @@ def test_gray_conversions(lib, orc):
     res = {}
     for kind, dt in (("f32", np.float32), ("f64", np.float64)):
         o = [np.zeros((h + 1) * s, dt)]
-        orc.call("gray_rgb", kind, rgb, w, h, s, *o)
+        orc.call("gray_rgb", kind, rgb, o[0], w, h, s)
         res[kind] = o
     g = [dzeros((h + 1) * s)]
     assert call(lib, "rgb_to_gray", dev(rgb), g[0], w, h, s) == 0
@@ -289,7 +289,7 @@ def test_gray_conversions(lib, orc):
         res = {}
         for kind, dt in (("f32", np.float32), ("f64", np.float64)):
             o = [np.zeros((h + 1) * s, dt)]
-            orc.call(name, kind, arr, w, h, s, *o)
+            orc.call(name, kind, arr, o[0], w, h, s)
             res[kind] = o
         g = [dzeros((h + 1) * s)]
         assert call(lib, cname, dev(arr), g[0], w, h, s) == 0

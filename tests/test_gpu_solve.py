@@ -63,10 +63,22 @@ def test_flow_config0_paths_equal_plain(cfg0, path, graph):
         assert np.array_equal(u[:, :320], u32[:, :320]) and np.array_equal(v[:, :320], v32[:, :320]), rep
 
 
+def test_flow_config0_fp16_storage_epe(cfg0):
+    # FP16 images / derivatives / duals, FP32 updates: accuracy stated against the FP64 oracle
+    print("case: TV-L1 320x240 fused fp16 storage", flush=True)
+    f = _flow(path=rf.PATH_FUSED, graph=True, storage=rf.STORAGE_F16)
+    f.copyImagesToDevice(cfg0["i0"], cfg0["i1"])
+    f.solveOpticalFlow(5.0)
+    u, v = f.copyOpticalFlowToHost()
+    e = _epe(u, v, *cfg0["o64"], 320)
+    print(f"fp16 storage TV-L1: mean EPE vs FP64 {e.mean():.3e}, p99 {np.percentile(e, 99):.3e}", flush=True)
+    assert e.mean() <= 0.05
+
+
 def _recon(w, h, L, path=rf.PATH_PLAIN, graph=False, tri=0, fix=0, nw=5, ni=50, cvtype=rf.INPUT_32F, method=14,
-           prior=False):
+           prior=False, storage=rf.STORAGE_F32):
     r = rf.ReconFlow()
-    kw = dict(cvType=cvtype, path=path, use_graph=graph)
+    kw = dict(cvType=cvtype, path=path, use_graph=graph, storage=storage)
     if prior:
         r.initializeR(w, h, 1, L, 2.0, method, 50.0, 0.0, 0.1, 100.0, 33.3, 60.0, 100.0, 0.125, nw, ni, **kw)
     else:
@@ -118,6 +130,19 @@ def test_recon_config1_fp64_triangulation_epe(orc, cfg1):
     u64, v64, X64, Y64, Z64 = _oracle_recon(orc, cfg1, 640, 480, 5, "f64")
     e = _epe(u, v, u64, v64, 640)
     assert e.mean() <= 0.05, e.mean()
+
+
+@pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16], ids=["f32", "f16"])
+def test_recon_config1_fused_epe(orc, cfg1, storage):
+    print(f"case: joint 640x480 fused graph tri_f64 storage={storage}", flush=True)
+    r = _recon(640, 480, 5, path=rf.PATH_FUSED, graph=True, tri=1, storage=storage)
+    u, v, X, Y, Z = _run_recon(r, cfg1)
+    u64, v64, X64, Y64, Z64 = _oracle_recon(orc, cfg1, 640, 480, 5, "f64")
+    e = _epe(u, v, u64, v64, 640)
+    zr = np.abs(Z[:, :640] - Z64[:, :640]) / np.abs(Z64[:, :640]).clip(1e-6)
+    print(f"joint storage={storage}: mean EPE vs FP64 {e.mean():.3e}, depth rel err median {np.median(zr):.3e}",
+          flush=True)
+    assert e.mean() <= 0.05
 
 
 @pytest.mark.parametrize("path,graph", [(rf.PATH_PLAIN, True), (rf.PATH_FUSED, False), (rf.PATH_FUSED, True)],
