@@ -626,6 +626,271 @@ void update_sk(const T* sku0, const T* skv0, const T* u0, const T* v0, const T* 
 }
 
 // ---------------------------------------------------------------------------------
+// f2 / f3 / f4: display, post-filter and alternate data terms.
+
+// ComputeColorKernel (flowToHSV.cu:194-302): Middlebury colour wheel, BGR output. The
+// reference indexes its 55-entry tables with k0, k1 in [1, 55]; index 55 lies past the table
+// and is taken modulo 55 here (parity unpinned for that entry).
+static const double kWheel[3][55] = {
+    {255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 213, 170, 128,
+     85,  43,  0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   19,
+     39,  58,  78,  98,  117, 137, 156, 176, 196, 215, 235, 255, 255, 255, 255, 255, 255},
+    {0,   17,  34,  51,  68,  85,  102, 119, 136, 153, 170, 187, 204, 221, 238, 255, 255, 255, 255,
+     255, 255, 255, 255, 255, 255, 255, 232, 209, 186, 163, 140, 116, 93,  70,  47,  24,  0,   0,
+     0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0},
+    {0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,   0,
+     0,   0,   0,   63,  127, 191, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255,
+     255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 255, 213, 170, 128, 85,  43}};
+
+template <typename T>
+void flow_to_hsv(const T* u, const T* v, int w, int h, int s, T* rgb, T flowscale) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T du = u[p] / flowscale, dv = v[p] / flowscale;
+            const int ncols = 55;
+            T rad = std::sqrt(du * du + dv * dv);
+            T a = std::atan2(-dv, -du) / T(3.14159f);
+            T fk = (a + T(1)) / T(2) * (T(ncols) - T(1)) + T(1);
+            int k0 = (int)std::floor(fk);
+            int k1 = k0 + 1;
+            if (k1 == ncols + 1) k1 = 1;
+            T f = fk - (T)k0;
+            k0 = ((k0 % ncols) + ncols) % ncols;
+            k1 = ((k1 % ncols) + ncols) % ncols;
+            T col[3];
+            for (int c = 0; c < 3; ++c) {
+                T cc = (T(1) - f) * (T(kWheel[c][k0]) / T(255)) + f * (T(kWheel[c][k1]) / T(255));
+                col[c] = rad <= T(1) ? T(1) - rad * (T(1) - cc) : cc * T(0.75);
+            }
+            rgb[3 * p + 0] = col[2];
+            rgb[3 * p + 1] = col[1];
+            rgb[3 * p + 2] = col[0];
+        }
+}
+
+// RemoveEdgeArtifacts3DKernel (solve3dMinimalArea.cu:320-378): zero points whose surface-area
+// proxy over a `range`-pixel neighbourhood exceeds maxarea. Reads column w / row h at the border.
+template <typename T>
+void remove_edge(const T* X0, const T* Y0, const T* Z0, int range, T maxarea, int w, int h, int s, T* X1, T* Y1,
+                 T* Z1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T X = X0[p], Y = Y0[p], Z = Z0[p];
+            T Xx, Yx, Zx, Xy, Yy, Zy;
+            if (ix + range > w) {
+                long q = p - range;
+                Xx = -X0[q] + X;
+                Yx = -Y0[q] + Y;
+                Zx = -Z0[q] + Z;
+            } else {
+                long q = p + range;
+                Xx = X0[q] - X;
+                Yx = Y0[q] - Y;
+                Zx = Z0[q] - Z;
+            }
+            if (iy + range > h) {
+                long q = p - (long)range * s;
+                Xy = -X0[q] + X;
+                Yy = -Y0[q] + Y;
+                Zy = -Z0[q] + Z;
+            } else {
+                long q = p + (long)range * s;
+                Xy = X0[q] - X;
+                Yy = Y0[q] - Y;
+                Zy = Z0[q] - Z;
+            }
+            T c1 = Yx * Zy - Zx * Yy, c2 = Zx * Xy - Xx * Zy, c3 = Xx * Yy - Yx * Xy;
+            T area = std::sqrt((c1 * c1 + c2 * c2) + c3 * c3);
+            bool drop = area > maxarea;
+            X1[p] = drop ? T(0) : X;
+            Y1[p] = drop ? T(0) : Y;
+            Z1[p] = drop ? T(0) : Z;
+        }
+}
+
+// ComputeDerivKernel (computeDeriv.cu:6-40): single-image 5-point gradient.
+template <typename T>
+void deriv_single(const T* I, int w, int h, int s, T* Ix, T* Iy) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            auto at = [&](long x, long y) { return I[refl(x, w) + refl(y, h) * s]; };
+            T t0 = at(ix - 2, iy);
+            t0 -= at(ix - 1, iy) * T(8);
+            t0 += at(ix + 1, iy) * T(8);
+            t0 -= at(ix + 2, iy);
+            t0 /= T(12);
+            Ix[p] = t0;
+            t0 = at(ix, iy - 2);
+            t0 -= at(ix, iy - 1) * T(8);
+            t0 += at(ix, iy + 1) * T(8);
+            t0 -= at(ix, iy + 2);
+            t0 /= T(12);
+            Iy[p] = t0;
+        }
+}
+
+// ComputeDerivativesPatchKernel (computeDerivativesPatch.cu:8-67): the derivatives of
+// ComputeDerivatives summed over the 5x5 patch; each tap's own stencil is mirror-indexed.
+template <typename T>
+void derivatives_patch(const T* I0, const T* I1, int w, int h, int s, T* Ix, T* Iy, T* Iz) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            auto at = [&](const T* I, long x, long y) { return I[refl(x, w) + refl(y, h) * s]; };
+            T sx = 0, sy = 0, sz = 0;
+            for (int j = 0; j < 5; ++j)
+                for (int i = 0; i < 5; ++i) {
+                    const long col = ix + i - 2, row = iy + j - 2;
+                    T t0, t1;
+                    t0 = at(I0, col - 2, row);
+                    t0 -= at(I0, col - 1, row) * T(8);
+                    t0 += at(I0, col + 1, row) * T(8);
+                    t0 -= at(I0, col + 2, row);
+                    t0 /= T(12);
+                    t1 = at(I1, col - 2, row);
+                    t1 -= at(I1, col - 1, row) * T(8);
+                    t1 += at(I1, col + 1, row) * T(8);
+                    t1 -= at(I1, col + 2, row);
+                    t1 /= T(12);
+                    sx += (t0 + t1) * T(0.5);
+                    sz += at(I1, col, row) - at(I0, col, row);
+                    t0 = at(I0, col, row - 2);
+                    t0 -= at(I0, col, row - 1) * T(8);
+                    t0 += at(I0, col, row + 1) * T(8);
+                    t0 -= at(I0, col, row + 2);
+                    t0 /= T(12);
+                    t1 = at(I1, col, row - 2);
+                    t1 -= at(I1, col, row - 1) * T(8);
+                    t1 += at(I1, col, row + 1) * T(8);
+                    t1 -= at(I1, col, row + 2);
+                    t1 /= T(12);
+                    sy += (t0 + t1) * T(0.5);
+                }
+            long p = ix + (long)iy * s;
+            Ix[p] = sx;
+            Iy[p] = sy;
+            Iz[p] = sz;
+        }
+}
+
+// sqrtf(x*x + 0.001) of the Charbonnier kernels: the literal 0.001 is double there, so the sum
+// is formed in double and rounded to float before the float square root.
+static inline float charb_root(float x) { return std::sqrt((float)((double)(x * x) + 0.001)); }
+static inline double charb_root(double x) { return std::sqrt(x * x + 0.001); }
+
+// SolveDataCharbForTVKernel (solveDataCharbForTV.cu:3-85) and SolveDataCharbForTVGradKernel
+// (solveDataCharbForTVGrad.cu:3-103): IRLS 2x2 solves (1 and 2 passes), then the primal step.
+// With gradient terms null this is the plain Charbonnier form.
+template <typename T>
+void data_charb(const T* du0, const T* dv0, const T* duhat0, const T* dvhat0, const T* pu1, const T* pu2,
+                const T* pv1, const T* pv2, const T* Ix, const T* Iy, const T* It, const T* Ixx, const T* Ixy,
+                const T* Ixt, const T* Iyx, const T* Iyy, const T* Iyt, int w, int h, int s, T lambda, T lambdagrad,
+                T theta, T* du1, T* dv1, T* duhat1, T* dvhat1) {
+    const bool grad = Ixx != nullptr;
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T du = du0[p], dv = dv0[p];
+            const T dix = Ix[p], diy = Iy[p], dit = It[p], duhat = duhat0[p], dvhat = dvhat0[p];
+            const T it = T(1) / theta;
+            if (!grad) {
+                T innerSum = (dix * du + diy * dv) + dit;
+                T beta = lambda / charb_root(innerSum);
+                T a11 = (dix * dix) * beta + it;
+                T a12 = (dix * diy) * beta;
+                T a21 = a12;
+                T a22 = (diy * diy) * beta + it;
+                T detA = a11 * a22 - a12 * a21;
+                T b1 = it * duhat - (dix * dit) * beta;
+                T b2 = it * dvhat - (diy * dit) * beta;
+                du = (a22 * b1 - a12 * b2) / detA;
+                dv = ((-a21) * b1 + a11 * b2) / detA;
+            } else {
+                const T dixx = Ixx[p], dixy = Ixy[p], dixt = Ixt[p], diyx = Iyx[p], diyy = Iyy[p], diyt = Iyt[p];
+                for (int k = 0; k < 2; ++k) {
+                    T innerSum = (dix * du + diy * dv) + dit;
+                    T innerSumx = (dixx * du + dixy * dv) + dixt;
+                    T innerSumy = (diyx * du + diyy * dv) + diyt;
+                    T beta = lambda / charb_root(innerSum);
+                    T betax = lambdagrad / charb_root(innerSumx);
+                    T betay = lambdagrad / charb_root(innerSumy);
+                    T a11 = (((dix * dix) * beta + (dixx * dixx) * betax) + (diyx * diyx) * betay) + it;
+                    T a12 = ((dix * diy) * beta + (dixx * dixy) * betax) + (diyx * diyy) * betay;
+                    T a21 = a12;
+                    T a22 = (((diy * diy) * beta + (dixy * dixy) * betax) + (diyy * diyy) * betay) + it;
+                    T detA = a11 * a22 - a12 * a21;
+                    T b1 = ((it * duhat - (dix * dit) * beta) - (dixx * dixt) * betax) - (diyx * diyt) * betay;
+                    T b2 = ((it * dvhat - (diy * dit) * beta) - (dixy * dixt) * betax) - (diyy * diyt) * betay;
+                    du = (a22 * b1 - a12 * b2) / detA;
+                    dv = ((-a21) * b1 + a11 * b2) / detA;
+                }
+            }
+            du1[p] = du;
+            dv1[p] = dv;
+            duhat1[p] = du + theta * div_p(pu1, pu2, ix, iy, s);
+            dvhat1[p] = dv + theta * div_p(pv1, pv2, ix, iy, s);
+        }
+}
+
+// SolveDataL1InpaintKernel (solveDataL1Inpaint.cu:4-98): the L1 step, skipped where mask0 == 0.
+template <typename T>
+void data_l1_inpaint(const T* duhat0, const T* dvhat0, const T* mask0, const T* pu1, const T* pu2, const T* pv1,
+                     const T* pv2, const T* Ix, const T* Iy, const T* It, int w, int h, int s, T lambda, T theta,
+                     T* duhat1, T* dvhat1) {
+    data_l1(duhat0, dvhat0, pu1, pu2, pv1, pv2, Ix, Iy, It, w, h, s, lambda, theta, duhat1, dvhat1);
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            if (mask0[p] == T(0)) {
+                duhat1[p] = duhat0[p] + theta * div_p(pu1, pu2, ix, iy, s);
+                dvhat1[p] = dvhat0[p] + theta * div_p(pv1, pv2, ix, iy, s);
+            }
+        }
+}
+
+// SolveSceneFlowAndPointCloudKernel (solveSceneFlow.cu:49-91): back-projection of the depth at
+// the pixel and at the flow target; the host passes depth0 as both maps (Cudaflow.cu:843).
+template <typename T>
+void scene_flow(const T* u, const T* v, const T* depth0, const T* depth1, T fx, T fy, T cx, T cy, int w, int h,
+                int s, T* pc0, T* pc1, T* sf) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T d0 = depth0[p];
+            T uu = u[p], vv = v[p];
+            T d1;
+            if (!(std::fabs(uu) <= T(16777216)) || !(std::fabs(vv) <= T(16777216))) {
+                d1 = std::numeric_limits<T>::quiet_NaN();
+            } else {
+                T fu = std::floor(uu), fv = std::floor(vv);
+                d1 = bilin(depth1, w, h, s, ix + (long)fu, uu - fu, iy + (long)fv, vv - fv);
+            }
+            T p0z = d0, p0x = (((T)ix - cx) * p0z) / fx, p0y = (((T)iy - cy) * p0z) / fy;
+            T p1z = d1, p1x = (((T)ix - cx) * p1z) / fx, p1y = (((T)iy - cy) * p1z) / fy;
+            T sx = p0x - p1x, sy = p0y - p1y, sz = p0z - p1z;
+            if ((d0 < T(500)) || (d0 > T(2500))) sx = sy = sz = T(0);
+            pc0[3 * p] = p0x;
+            pc0[3 * p + 1] = p0y;
+            pc0[3 * p + 2] = p0z;
+            pc1[3 * p] = p1x;
+            pc1[3 * p + 1] = p1y;
+            pc1[3 * p + 2] = p1z;
+            sf[3 * p] = sx;
+            sf[3 * p + 1] = sy;
+            sf[3 * p + 2] = sz;
+        }
+}
+
+// ---------------------------------------------------------------------------------
 // Host orchestration.
 
 // Parameters of CudaFlow::initialize / ReconFlow::_initializeR (Cudaflow.cu:99-127;
@@ -716,42 +981,105 @@ static void gray_input(const Params& P, const void* in, T* out, int s0) {
 // A11: CudaFlow::_solveOpticalFlow for METHOD_TVL1 (Cudaflow.cu:403-621), with the
 // pointer swaps mirrored. Images are stride x height, outputs u, v likewise.
 template <typename T>
-int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, float* v_out) {
+// Method branches follow Cudaflow.cu:426-578: Charbonnier-gradient for methods 2 and 7 (with
+// gradient pyramids, :426-461, and three warps / derivative sets per warp, :489-504),
+// Charbonnier for method 1, patch derivatives for method 3, the L1 step otherwise; method 8 is
+// the masked L1 step of CudaFlow::_solveInpainting (Inpainting.cu:13-152).
+int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, float* v_out, const float* mask0_in = nullptr) {
     std::vector<int> pW, pH, pS;
     pyramid(P, pW, pH, pS);
     const int w0 = pW[0], h0 = pH[0], s0 = pS[0];
     const long n = (long)s0 * h0;
+    const int m = P.method;
+    const bool cg = (m == 2 || m == 7), inpaint = (m == 8);
     Pool<T> pool(n, s0);
-    std::vector<T*> pI0(P.n_levels), pI1(P.n_levels);
+    std::vector<T*> pI0(P.n_levels), pI1(P.n_levels), pIx0(P.n_levels), pIx1(P.n_levels), pIy0(P.n_levels),
+        pIy1(P.n_levels), pM0(P.n_levels);
     for (int l = 0; l < P.n_levels; ++l) {
         pI0[l] = pool.make();
         pI1[l] = pool.make();
+        if (cg) {
+            pIx0[l] = pool.make();
+            pIx1[l] = pool.make();
+            pIy0[l] = pool.make();
+            pIy1[l] = pool.make();
+        }
+        if (inpaint) pM0[l] = pool.make();
     }
     T *i1warp = pool.make(), *dumed = pool.make(), *dvmed = pool.make(), *dumeds = pool.make(),
       *dvmeds = pool.make();
+    T *du = pool.make(), *dv = pool.make(), *dus = pool.make(), *dvs = pool.make();
     T *pu1 = pool.make(), *pu2 = pool.make(), *pv1 = pool.make(), *pv2 = pool.make();
     T *pu1s = pool.make(), *pu2s = pool.make(), *pv1s = pool.make(), *pv2s = pool.make();
     T *Ix = pool.make(), *Iy = pool.make(), *Iz = pool.make();
+    T *ix1warp = nullptr, *iy1warp = nullptr, *Ixx = nullptr, *Ixy = nullptr, *Ixz = nullptr, *Iyx = nullptr,
+      *Iyy = nullptr, *Iyz = nullptr;
+    if (cg) {
+        ix1warp = pool.make();
+        iy1warp = pool.make();
+        Ixx = pool.make();
+        Ixy = pool.make();
+        Ixz = pool.make();
+        Iyx = pool.make();
+        Iyy = pool.make();
+        Iyz = pool.make();
+    }
     T *u = pool.make(), *v = pool.make(), *us = pool.make(), *vs = pool.make();
     (void)w0;
 
     gray_input(P, i0, pI0[0], s0);
     gray_input(P, i1, pI1[0], s0);
+    if (cg) {
+        deriv_single(pI0[0], pW[0], pH[0], s0, pIx0[0], pIy0[0]);
+        deriv_single(pI1[0], pW[0], pH[0], s0, pIx1[0], pIy1[0]);
+    }
+    if (inpaint) gray_f32(mask0_in, pM0[0], pW[0], pH[0], s0);
     for (int l = 1; l < P.n_levels; ++l) {
-        downscale(pI0[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI0[l]);
-        downscale(pI1[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI1[l]);
+        auto down = [&](std::vector<T*>& py) {
+            downscale(py[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, py[l]);
+        };
+        down(pI0);
+        down(pI1);
+        if (cg) {
+            down(pIx0);
+            down(pIx1);
+            down(pIy0);
+            down(pIy1);
+        }
+        if (inpaint) down(pM0);
     }
     zero(u, n);
     zero(v, n);
-    const T lambda = P.lambda, theta = P.theta, tau = P.tau;
+    const T lambda = P.lambda, lambdagrad = P.lambdagrad, theta = P.theta, tau = P.tau;
     for (int level = P.n_levels - 1; level >= 0; --level) {
         const int w = pW[level], h = pH[level], s = pS[level];
         for (int wi = 0; wi < P.n_warp_iters; ++wi) {
-            for (T* b : {dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
+            for (T* b : {du, dv, dus, dvs, dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
             warp(pI1[level], w, h, s, u, v, i1warp);
-            derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            if (cg) {
+                warp(pIx1[level], w, h, s, u, v, ix1warp);
+                warp(pIy1[level], w, h, s, u, v, iy1warp);
+            }
+            if (m == 3)
+                derivatives_patch(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            else
+                derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            if (cg) {
+                derivatives(pIx0[level], ix1warp, w, h, s, Ixx, Ixy, Ixz);
+                derivatives(pIy0[level], iy1warp, w, h, s, Iyx, Iyy, Iyz);
+            }
             for (int it = 0; it < P.n_solver_iters; ++it) {
-                data_l1(dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta, dumeds, dvmeds);
+                if (cg || m == 1) {
+                    data_charb(du, dv, dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, Ixx, Ixy, Ixz, Iyx, Iyy, Iyz, w,
+                               h, s, lambda, lambdagrad, theta, dus, dvs, dumeds, dvmeds);
+                    std::swap(du, dus);
+                    std::swap(dv, dvs);
+                } else if (inpaint) {
+                    data_l1_inpaint(dumed, dvmed, pM0[level], pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta,
+                                    dumeds, dvmeds);
+                } else {
+                    data_l1(dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta, dumeds, dvmeds);
+                }
                 std::swap(dumed, dumeds);
                 std::swap(dvmed, dvmeds);
                 dual_tv(dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, theta, pu1s, pu2s, pv1s, pv2s);
@@ -1051,6 +1379,46 @@ void orc_gray_u16_f32(const uint16_t* in, float* out, int w, int h, int s) { orc
 
 DEF2(double, f64)
 DEF2(float, f32)
+
+#define DEF3(T, SUF)                                                                                          \
+    void orc_flow_to_hsv_##SUF(const T* u, const T* v, int w, int h, int s, T* rgb, float fs) {              \
+        orc::flow_to_hsv<T>(u, v, w, h, s, rgb, (T)fs);                                                      \
+    }                                                                                                        \
+    void orc_remove_edge_##SUF(const T* X0, const T* Y0, const T* Z0, int range, float maxarea, int w, int h,  \
+                               int s, T* X1, T* Y1, T* Z1) {                                                 \
+        orc::remove_edge<T>(X0, Y0, Z0, range, (T)maxarea, w, h, s, X1, Y1, Z1);                              \
+    }                                                                                                        \
+    void orc_deriv_single_##SUF(const T* I, int w, int h, int s, T* Ix, T* Iy) {                             \
+        orc::deriv_single<T>(I, w, h, s, Ix, Iy);                                                            \
+    }                                                                                                        \
+    void orc_derivatives_patch_##SUF(const T* I0, const T* I1, int w, int h, int s, T* Ix, T* Iy, T* Iz) {   \
+        orc::derivatives_patch<T>(I0, I1, w, h, s, Ix, Iy, Iz);                                              \
+    }                                                                                                        \
+    void orc_data_charb_##SUF(const T* du0, const T* dv0, const T* duh, const T* dvh, const T* pu1,          \
+                              const T* pu2, const T* pv1, const T* pv2, const T* Ix, const T* Iy,            \
+                              const T* It, const T* Ixx, const T* Ixy, const T* Ixt, const T* Iyx,           \
+                              const T* Iyy, const T* Iyt, int w, int h, int s, float lambda, float lgrad,    \
+                              float theta, T* du1, T* dv1, T* duh1, T* dvh1) {                               \
+        orc::data_charb<T>(du0, dv0, duh, dvh, pu1, pu2, pv1, pv2, Ix, Iy, It, Ixx, Ixy, Ixt, Iyx, Iyy, Iyt, \
+                           w, h, s, (T)lambda, (T)lgrad, (T)theta, du1, dv1, duh1, dvh1);                    \
+    }                                                                                                        \
+    void orc_data_l1_inpaint_##SUF(const T* du, const T* dv, const T* m0, const T* pu1, const T* pu2,        \
+                                   const T* pv1, const T* pv2, const T* Ix, const T* Iy, const T* Iz, int w, \
+                                   int h, int s, float lambda, float theta, T* du1, T* dv1) {                \
+        orc::data_l1_inpaint<T>(du, dv, m0, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, (T)lambda, (T)theta,    \
+                                du1, dv1);                                                                   \
+    }                                                                                                        \
+    void orc_scene_flow_##SUF(const T* u, const T* v, const T* d0, const T* d1, float fx, float fy,          \
+                              float cx, float cy, int w, int h, int s, T* pc0, T* pc1, T* sf) {              \
+        orc::scene_flow<T>(u, v, d0, d1, (T)fx, (T)fy, (T)cx, (T)cy, w, h, s, pc0, pc1, sf);                 \
+    }                                                                                                        \
+    int orc_solve_flow_masked_##SUF(const orc::Params* P, const void* i0, const void* i1, const float* mask0, \
+                                    float* u, float* v) {                                                    \
+        return orc::solve_flow<T>(*P, i0, i1, u, v, mask0);                                                  \
+    }
+
+DEF3(double, f64)
+DEF3(float, f32)
 
 void orc_pyramid(const orc::Params* P, int* W, int* H, int* S) {
     std::vector<int> pW, pH, pS;
