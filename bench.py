@@ -40,11 +40,12 @@ KID = {"fused_iter": 0, "data": 1, "dual": 2, "uvproj": 3, "solve3d": 4, "median
        "solve_uc": 7, "update_sk": 8}
 
 
-def make_solver(rf, path, graph, tri):
+def make_solver(rf, path, graph, tri, joint_tile=0):
     r = rf.ReconFlow()
     r._initializeR(W, H, 1, LEVELS, 2.0, rf.ReconFlow.METHODR_TVL1_MS, 50.0, 0.0, 0.1, 100.0, 33.3, 60.0, 0.125,
                    WARPS, ITERS, cvType=rf.INPUT_32F, path=path, use_graph=graph)
     r.params.tri_f64 = int(tri)
+    r.params.joint_tile = int(joint_tile)
     r._init_params(r.params)
     return r
 
@@ -57,9 +58,14 @@ def main() -> int:
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
+    ap.add_argument("--joint-tile", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="fused joint tiling: 0 auto per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16 lanes")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain-compare", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (scales the FETCH_SIZE / WRITE_SIZE "
+                         "counters to bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this solver runs one image pair per device and the "
@@ -73,9 +79,17 @@ def main() -> int:
     import reconflow_amd as rf
 
     assert torch.cuda.is_available(), "bench needs a GPU"
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "launches": 5}))
+        return 0
     d = rf.make_recon_pair(W, H, args.seed)
     path = rf.PATH_FUSED if args.path == "fused" else rf.PATH_PLAIN
-    r = make_solver(rf, path, not args.no_graph, args.tri_f64)
+    r = make_solver(rf, path, not args.no_graph, args.tri_f64, args.joint_tile)
     r.setCameraMatrices(d["K"], d["K"])
     r.copyImagesToDevice(d["i0"], d["i1"])  # resident for every step
     for _ in range(args.warmup):
@@ -106,6 +120,13 @@ def main() -> int:
         per[name] = (tot.value, n.value)
     dom = max(per, key=lambda k: per[k][0])
     tot_ms, nl = per[dom]
+    per_level_us = []
+    for lv in range(LEVELS):  # the dominant kernel's average launch per pyramid level
+        assert lib.rf_profile_arm(r._ctx, KID[dom], lv) == 0
+        r.solveR(d["R"], d["t"], 50.0)
+        t_, n_ = C.c_double(), C.c_int()
+        assert lib.rf_profile_read(r._ctx, C.byref(t_), C.byref(n_)) == 0
+        per_level_us.append(round(t_.value * 1e3 / max(n_.value, 1), 3))
     avg_s = tot_ms / 1e3 / max(nl, 1)
     bytes_launch = BYTES_PER_PX[dom] * W * H
     achieved = bytes_launch / avg_s / 1e9
@@ -156,12 +177,13 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" + ("+f64-triangulation" if args.tri_f64 else ""),
-        "data": "synthetic (seeded planar scene, config-1 generator)",
+        "data": "synthetic (seeded piecewise-planar scene pair, rendered on the host)",
         "config": {"workload": "joint flow+depth 640x480, 5 levels, 5 warps, 50 iterations, one pair per step",
-                   "path": args.path, "graph": not args.no_graph, "seed": args.seed},
+                   "path": args.path, "graph": not args.no_graph, "seed": args.seed, "joint_tile": args.joint_tile},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "avg_launch_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": bytes_launch},
+                     "avg_launch_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": bytes_launch,
+                     "avg_launch_us_per_level": per_level_us},
         "cpu_baseline": cpu,
         "epe_vs_fp64_oracle": None if epe is None else round(epe, 5),
         "accuracy_ok": None if epe is None else bool(epe <= 0.05),

@@ -22,6 +22,7 @@ class RFParams(C.Structure):
         ("lambdaf", C.c_float), ("lambdams", C.c_float), ("alpha_tv", C.c_float), ("alpha_proj", C.c_float),
         ("alpha_fn", C.c_float), ("stride_alignment", C.c_int), ("path", C.c_int), ("storage", C.c_int),
         ("use_graph", C.c_int), ("fix_coarse_k", C.c_int), ("tri_f64", C.c_int),
+        ("joint_tile", C.c_int),
     ]
 
 

@@ -82,6 +82,17 @@ __device__ __forceinline__ TS* as(float* p) {
     return reinterpret_cast<TS*>(p);
 }
 
+// Element i of a field through a 32-bit byte offset (fields hold at most 2^30 elements), so that
+// the access compiles to a scalar base plus one vector offset.
+template <typename T>
+__device__ __forceinline__ T& el(T* p, unsigned i) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T& el(const T* p, unsigned i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T));
+}
+
 template <typename TS>
 __global__ void __launch_bounds__(256) kb_gray(Launch L, int type) {
     BT_PIX(L, 0)
@@ -459,6 +470,160 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
     }
 }
 
+// One joint iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
+// outer ring of lanes only feeds its (EXW-2) x (EXH-2) interior. Every global load of a lane is
+// issued up front (coordinates clamped into the image so that the loads are in bounds; lanes off
+// the image compute values nobody reads), then
+//   data step -> uhat_n, projection blend -> uproj_n (registers), triangulation -> X'_n;
+//   uhat_n and X'_n to LDS, one barrier;
+//   interior lanes: dual step -> p_n, 3x3 XYZ median of X'_n, cleanup -> X_n, reprojection ->
+//   uc_n, Sk update, stores.
+// Same per-pixel arithmetic as kb_joint_iter, so the results are identical. SINGLE: the launch
+// covers one pair whose tiles are the dispatch indices (no tile list read).
+template <int EXW, int EXH, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
+__global__ void __launch_bounds__(EXW* EXH) kb_joint_px(IterArgs a) {
+    constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
+    __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
+    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
+    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
+    const PairFields& F = a.L.P[pt.x];
+    const int l = a.level;
+    const int w = F.W[l], h = F.H[l], s = F.S[l];
+    const int tiles_x = (w + TW - 1) / TW;
+    const int x0 = (pt.y % tiles_x) * TW, y0 = (pt.y / tiles_x) * TH;
+    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
+    const int x = x0 - 1 + ex, y = y0 - 1 + ey;
+    const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+    // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
+    // offset
+    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
+    const bool hl = xc > 0, hu = yc > 0;
+    const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
+    const int pd = a.pd, pf = a.pf, px = a.px;
+
+    // ---- loads
+    float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
+          p1vl = 0.0f, p2vd = 0.0f;
+    if (!a.first) {
+        const TS *p1u0 = as<TS>(F.p1u[pd]), *p2u0 = as<TS>(F.p2u[pd]), *p1v0 = as<TS>(F.p1v[pd]),
+                 *p2v0 = as<TS>(F.p2v[pd]);
+        duh = el(F.du[pd], q);
+        dvh = el(F.dv[pd], q);
+        p1u = el(p1u0, q);
+        p2u = el(p2u0, q);
+        p1v = el(p1v0, q);
+        p2v = el(p2v0, q);
+        p1ul = el(p1u0, ql);
+        p1vl = el(p1v0, ql);
+        p2ud = el(p2u0, qu);
+        p2vd = el(p2v0, qu);
+    }
+    const float uu = el(F.u[a.pu], q), vv = el(F.v[a.pu], q);
+    const float sku = el(F.sku[pf], q), skv = el(F.skv[pf], q), up0 = el(F.up[pf], q), vp0 = el(F.vp[pf], q);
+    const float uc0 = el(F.uc[pf], q), vc0 = el(F.vc[pf], q);
+    const float ix_ = (float)el(as<TS>(F.Ix), q), iy_ = (float)el(as<TS>(F.Iy), q), iz_ = (float)el(as<TS>(F.Iz), q);
+    const float *X0 = F.X[px], *Y0 = F.Y[px], *Z0 = F.Z[px];
+    const unsigned qr = q + 1u, qd = q + (unsigned)s;
+    const float Xc = el(X0, q), Yc = el(Y0, q), Zc = el(Z0, q), Xr = el(X0, qr), Yr = el(Y0, qr), Zr = el(Z0, qr);
+    const float Xd = el(X0, qd), Yd = el(Y0, qd), Zd = el(Z0, qd), Xl = el(X0, ql), Yl = el(Y0, ql), Zl = el(Z0, ql);
+    const float Xu = el(X0, qu), Yu = el(Y0, qu), Zu = el(Z0, qu);
+    float ufn = 0.0f, vfn = 0.0f, msk = 1.0f;
+    if (PRIOR) {
+        ufn = el(F.ufn_l, q);
+        vfn = el(F.vfn_l, q);
+        if (a.sparse) msk = el(F.mask_l, q);
+    }
+
+    // ---- data step + projection blend
+    float AA, bu, bv;
+    if (PRIOR) {
+        const float aFn = a.sparse ? a.aFn * msk : a.aFn;
+        AA = (a.aTv + a.aP) + aFn;
+        bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, duh, up0, uu, sku, ufn);
+        bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, dvh, vp0, vv, skv, vfn);
+    } else {
+        AA = a.aTv + a.aP;
+        bu = recon_blend(a.aTv, a.aP, AA, duh, up0, uu, sku);
+        bv = recon_blend(a.aTv, a.aP, AA, dvh, vp0, vv, skv);
+    }
+    float du, dv;
+    l1_threshold_div(ix_, iy_, iz_, bu, bv, a.lambda, AA, du, dv);
+    const float nduh = du + div_p(p1u, p1ul, p2u, p2ud, hl, hu) / AA;
+    const float ndvh = dv + div_p(p1v, p1vl, p2v, p2vd, hl, hu) / AA;
+    const float upn = uvproj_px(a.lf, a.aP, uc0, uu, nduh, sku);
+    const float vpn = uvproj_px(a.lf, a.aP, vc0, vv, ndvh, skv);
+
+    // ---- triangulation
+    float oX, oY, oZ;
+    if (F64TRI)
+        solve3d_px<double>(F.pqd[l], (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd,
+                           Xl, Yl, Zl, Xu, Yu, Zu, hl, hu, oX, oY, oZ);
+    else
+        solve3d_px<float>(F.pq[l], a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
+                          Yu, Zu, hl, hu, oX, oY, oZ);
+    s_du[e] = nduh;
+    s_dv[e] = ndvh;
+    s_X[e] = oX;
+    s_Y[e] = oY;
+    s_Z[e] = oZ;
+    __syncthreads();
+    if (ex == 0 || ey == 0 || ex == EXW - 1 || ey == EXH - 1 || x >= w || y >= h) return;
+
+    // ---- dual step (interior)
+    const float fac = a.tau / a.theta;
+    float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
+    if (x + 1 < w) {
+        dux = s_du[e + 1] - nduh;
+        dvx = s_dv[e + 1] - ndvh;
+    }
+    if (y + 1 < h) {
+        duy = s_du[e + EXW] - nduh;
+        dvy = s_dv[e + EXW] - ndvh;
+    }
+    float o1, o2;
+    dual_px(dux, duy, p1u, p2u, fac, o1, o2);
+    el(as<TS>(F.p1u[pd ^ 1]), q) = st<TS>(o1);
+    el(as<TS>(F.p2u[pd ^ 1]), q) = st<TS>(o2);
+    dual_px(dvx, dvy, p1v, p2v, fac, o1, o2);
+    el(as<TS>(F.p1v[pd ^ 1]), q) = st<TS>(o1);
+    el(as<TS>(F.p2v[pd ^ 1]), q) = st<TS>(o2);
+
+    // ---- XYZ median, cleanup, reprojection, Sk update. Tap rule of medianFilter.cu:193-214 for
+    // an interior pixel (taps lie within one pixel of it): a tap in column w or row h, or left
+    // of column 0 and above row 0 at once, keeps 0; a tap left of column 0 (above row 0) reads
+    // column 0 (row 0); the rest read themselves. X' is finite (the triangulation falls back to
+    // the finite X_{n-1}), so the min/max median applies.
+    float mX[9], mY[9], mZ[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int col = x + i - 1, row = y + j - 1;
+            const bool zero = col == w || row == h || (col < 0 && row < 0);
+            const int ee = (max(col, 0) - x0 + 1) + (max(row, 0) - y0 + 1) * EXW;
+            const int k = j * 3 + i;
+            mX[k] = zero ? 0.0f : s_X[ee];
+            mY[k] = zero ? 0.0f : s_Y[ee];
+            mZ[k] = zero ? 0.0f : s_Z[ee];
+        }
+    const float xm = median9_finite(mX), ym = median9_finite(mY), zm = median9_finite(mZ);
+    float Xn, Yn, Zn;
+    cleanup_px(oX, oY, oZ, xm, ym, zm, Xn, Yn, Zn);
+    float ucn, vcn;
+    uc_px(F.ucp[l], Xn, Yn, Zn, x, y, ucn, vcn);
+    el(F.X[px ^ 1], q) = Xn;
+    el(F.Y[px ^ 1], q) = Yn;
+    el(F.Z[px ^ 1], q) = Zn;
+    el(F.uc[pf ^ 1], q) = ucn;
+    el(F.vc[pf ^ 1], q) = vcn;
+    el(F.du[pd ^ 1], q) = nduh;
+    el(F.dv[pd ^ 1], q) = ndvh;
+    el(F.up[pf ^ 1], q) = upn;
+    el(F.vp[pf ^ 1], q) = vpn;
+    el(F.sku[pf ^ 1], q) = sk_px(sku, uu, nduh, upn);
+    el(F.skv[pf ^ 1], q) = sk_px(skv, vv, ndvh, vpn);
+}
+
 // One TV-L1 iteration: data step (tile + right column + bottom row), dual step (tile).
 template <int RPT, typename TS>
 __global__ void __launch_bounds__(256) kb_flow_iter(IterArgs a) {
@@ -600,6 +765,10 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         set_error("rf_batch_create: unknown input type");
         return RF_ERR_ARG;
     }
+    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16) {
+        set_error("rf_batch_create: unknown joint tiling");
+        return RF_ERR_ARG;
+    }
     const int align = p.stride_alignment > 0 ? p.stride_alignment : 32;
     std::vector<Geo> geo(n);
     for (int i = 0; i < n; ++i) {
@@ -622,6 +791,10 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         }
         geo[i].n0 = (long)geo[i].S[0] * geo[i].H[0];
         geo[i].bufn = ((geo[i].n0 + geo[i].S[0]) + 63) / 64 * 64;
+        if (geo[i].bufn > (1L << 30)) {  // the fused kernels address fields with 32-bit byte offsets
+            set_error("rf_batch_create: pair " + std::to_string(i) + " exceeds 2^30 elements per field");
+            return RF_ERR_ARG;
+        }
     }
     release();
     p_ = p;
@@ -695,25 +868,23 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         release();
         return RF_ERR_ALLOC;
     }
-    // tile lists: levels 0..L-1 and the flat level-0 buffer; tile heights 4, 8, 16
-    const int tys[3] = {4, 4 * RPT_JOINT, 4 * RPT_FLOW};
-    sets_.assign((kMaxLevels + 1) * 3, TileSet{});
+    // tile lists: levels 0..L-1 and the flat level-0 buffer (pointwise tiles only)
+    sets_.assign((kMaxLevels + 1) * TC_COUNT, TileSet{});
     std::vector<int2> all;
     std::vector<long> offs(sets_.size(), -1);
     for (int lv = 0; lv <= kMaxLevels; ++lv) {
         if (lv >= L && lv != FLAT) continue;
-        for (int c = 0; c < 3; ++c) {
-            if (lv == FLAT && c != 0) continue;
-            const int ty = tys[c];
-            offs[lv * 3 + c] = (long)all.size();
+        for (int c = 0; c < TC_COUNT; ++c) {
+            if (lv == FLAT && c != TC_PIX) continue;
+            const int k = lv * TC_COUNT + c;
+            offs[k] = (long)all.size();
             for (int i = 0; i < n; ++i) {
                 const int w = lv == FLAT ? geo_[i].S[0] : geo_[i].W[lv];
                 const int h = lv == FLAT ? geo_[i].H[0] : geo_[i].H[lv];
-                const int nt = ((w + TXW - 1) / TXW) * ((h + ty - 1) / ty);
+                const int nt = ((w + kTileW[c] - 1) / kTileW[c]) * ((h + kTileH[c] - 1) / kTileH[c]);
                 for (int t = 0; t < nt; ++t) all.push_back(make_int2(i, t));
             }
-            sets_[lv * 3 + c].ntiles = (int)(all.size() - offs[lv * 3 + c]);
-            sets_[lv * 3 + c].ty = ty;
+            sets_[k].ntiles = (int)(all.size() - offs[k]);
         }
     }
     if (hipMalloc((void**)&maps_, sizeof(int2) * std::max<size_t>(all.size(), 1)) != hipSuccess) {
@@ -731,9 +902,21 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
     return upload_pairs();
 }
 
-TileSet& Batch::tiles(int level, int ty) {
-    const int c = ty == 4 ? 0 : (ty == 4 * RPT_JOINT ? 1 : 2);
-    return sets_[level * 3 + c];
+TileSet& Batch::tiles(int level, int cls) { return sets_[level * TC_COUNT + cls]; }
+
+// Joint iteration tiling of a level: 30x14-pixel tiles where the level has at least two per CU
+// of a 256-CU device (throughput), 14x14 on smaller levels (latency), unless fixed.
+int Batch::joint_class(int level) const {
+    switch (p_.joint_tile) {
+        case RF_JOINT_TILE_64x8_RPT2: return TC_J64;
+        case RF_JOINT_TILE_32x16: return TC_J32;
+        case RF_JOINT_TILE_16x16: return TC_J16;
+        default: break;
+    }
+    long n32 = 0;
+    for (const Geo& g : geo_) n32 += (long)((g.W[level] + kTileW[TC_J32] - 1) / kTileW[TC_J32]) *
+                                     ((g.H[level] + kTileH[TC_J32] - 1) / kTileH[TC_J32]);
+    return n32 >= 512 ? TC_J32 : TC_J16;
 }
 
 int Batch::upload_pairs() {
@@ -850,6 +1033,37 @@ int Batch::set_prior(int i, const float* u, const float* v, const float* mask, b
     return check(hipStreamSynchronize(st_), "copy prior");
 }
 
+template <typename TS, bool SINGLE, bool F64, bool PRIOR>
+static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
+    const dim3 grid(a.L.ntiles);
+    switch (cls) {
+        case TC_J64:
+            hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, F64, PRIOR, TS>), grid, dim3(TXW, TRW), 0, st, a);
+            break;
+        case TC_J32:
+            hipLaunchKernelGGL((kb_joint_px<32, 16, SINGLE, F64, PRIOR, TS>), grid, dim3(32, 16), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((kb_joint_px<16, 16, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            break;
+    }
+}
+
+template <typename TS>
+static void launch_joint(int cls, bool single, bool f64, bool prior, hipStream_t st, const IterArgs& a) {
+    const int k = (single ? 4 : 0) | (f64 ? 2 : 0) | (prior ? 1 : 0);
+    switch (k) {
+        case 0: launch_joint_t<TS, false, false, false>(cls, st, a); break;
+        case 1: launch_joint_t<TS, false, false, true>(cls, st, a); break;
+        case 2: launch_joint_t<TS, false, true, false>(cls, st, a); break;
+        case 3: launch_joint_t<TS, false, true, true>(cls, st, a); break;
+        case 4: launch_joint_t<TS, true, false, false>(cls, st, a); break;
+        case 5: launch_joint_t<TS, true, false, true>(cls, st, a); break;
+        case 6: launch_joint_t<TS, true, true, false>(cls, st, a); break;
+        default: launch_joint_t<TS, true, true, true>(cls, st, a); break;
+    }
+}
+
 #define RF_TRY(x)                          \
     do {                                   \
         hipError_t e__ = (x);              \
@@ -874,37 +1088,32 @@ template <typename TS>
 hipError_t Batch::enqueue_t(bool recon) {
     const int L = p_.n_levels;
     Parity pr{};
-    RF_LAUNCH(kb_gray<TS>, tiles(0, 4), p_.input_type);
-    for (int l = 1; l < L; ++l) RF_LAUNCH(kb_pyramid<TS>, tiles(l, 4), l);
-    RF_LAUNCH(kb_zero, tiles(FLAT, 4), recon ? 1 : 0);
+    RF_LAUNCH(kb_gray<TS>, tiles(0, TC_PIX), p_.input_type);
+    for (int l = 1; l < L; ++l) RF_LAUNCH(kb_pyramid<TS>, tiles(l, TC_PIX), l);
+    RF_LAUNCH(kb_zero, tiles(FLAT, TC_PIX), recon ? 1 : 0);
     const bool fn = recon && prior_ && (p_.method == RF_METHODR_TVL1_MS_FN || p_.method == RF_METHODR_TVL1_MS_FNSPARSE);
     const float theta = recon ? 1.0f / p_.alpha_tv : p_.theta;  // Reconflow.cu:32
     for (int level = L - 1; level >= 0; --level) {
         if (recon) {
-            if (fn) RF_LAUNCH(kb_prior, tiles(level, 4), level);
-            RF_LAUNCH(kb_sync_x, tiles(FLAT, 4), pr.x);
+            if (fn) RF_LAUNCH(kb_prior, tiles(level, TC_PIX), level);
+            RF_LAUNCH(kb_sync_x, tiles(FLAT, TC_PIX), pr.x);
         }
         for (int wi = 0; wi < p_.n_warp_iters; ++wi) {
-            RF_LAUNCH(kb_warp<TS>, tiles(level, 4), level, pr.u);
-            RF_LAUNCH(kb_deriv<TS>, tiles(level, 4), level);
-            if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, 4), level, pr.d);
+            RF_LAUNCH(kb_warp<TS>, tiles(level, TC_PIX), level, pr.u);
+            RF_LAUNCH(kb_deriv<TS>, tiles(level, TC_PIX), level);
+            if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, TC_PIX), level, pr.d);
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
             for (int it = 0; it < p_.n_solver_iters; ++it) {
-                const TileSet& ts = tiles(level, recon ? 4 * RPT_JOINT : 4 * RPT_FLOW);
+                const int cls = recon ? joint_class(level) : TC_FLOW;
+                const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
                            (fn && sparse_) ? 1 : 0};
-                const dim3 grid(ts.ntiles), blk(TXW, TRW);
                 if (ts.ntiles > 0) {
-                    if (!recon) hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), grid, blk, 0, st_, a);
-                    else if (fn && p_.tri_f64)
-                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, true, true, TS>), grid, blk, 0, st_, a);
-                    else if (fn)
-                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, false, true, TS>), grid, blk, 0, st_, a);
-                    else if (p_.tri_f64)
-                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, true, false, TS>), grid, blk, 0, st_, a);
+                    if (!recon)
+                        hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), dim3(ts.ntiles), dim3(TXW, TRW), 0, st_, a);
                     else
-                        hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, false, false, TS>), grid, blk, 0, st_, a);
+                        launch_joint<TS>(cls, n() == 1, p_.tri_f64 != 0, fn, st_, a);
                     RF_TRY(hipGetLastError());
                 }
                 pr.d ^= 1;
@@ -914,19 +1123,19 @@ hipError_t Batch::enqueue_t(bool recon) {
                 }
             }
             prof_.end(st_, RF_KID_FUSED_ITER, level, p_.n_solver_iters);
-            RF_LAUNCH(kb_med5_add, tiles(level, 4), level, pr.d, pr.u);
+            RF_LAUNCH(kb_med5_add, tiles(level, TC_PIX), level, pr.d, pr.u);
             if (recon) {
-                RF_LAUNCH(kb_median5, tiles(level, 4), level, pr.u);
+                RF_LAUNCH(kb_median5, tiles(level, TC_PIX), level, pr.u);
                 pr.u ^= 1;
             }
         }
         if (level > 0) {
-            RF_LAUNCH(kb_upscale, tiles(level - 1, 4), level - 1, pr.u, pr.f, recon ? 1 : 0, p_.scale);
+            RF_LAUNCH(kb_upscale, tiles(level - 1, TC_PIX), level - 1, pr.u, pr.f, recon ? 1 : 0, p_.scale);
             pr.u ^= 1;
             if (recon) pr.f ^= 1;
         }
     }
-    if (p_.with_visualization) RF_LAUNCH(kb_hsv, tiles(0, 4), pr.u, flow_scale_);
+    if (p_.with_visualization) RF_LAUNCH(kb_hsv, tiles(0, TC_PIX), pr.u, flow_scale_);
     final_ = pr;
     return hipSuccess;
 }

@@ -32,11 +32,17 @@ struct PairFields {
     UcParams ucp[kMaxLevels];
 };
 
-// One launch's tile list: (pair, tile index within the pair) for tiles of 64 x ty pixels.
+// Tile shapes (pixels per workgroup) of the batched launches: pointwise 64x4; joint iteration
+// 64x8 (two rows per lane), 30x14 and 14x14 (one pixel per lane of a 32x16 / 16x16 extended
+// tile); TV-L1 iteration 64x16.
+enum TileClass { TC_PIX = 0, TC_J64 = 1, TC_FLOW = 2, TC_J32 = 3, TC_J16 = 4, TC_COUNT = 5 };
+constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14};
+constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14};
+
+// One launch's tile list: (pair, tile index within the pair).
 struct TileSet {
     int2* map = nullptr;  // device
     int ntiles = 0;
-    int ty = 4;
 };
 
 // Buffer parities of the shared schedule.
@@ -79,7 +85,8 @@ class Batch {
     hipError_t enqueue_t(bool recon);
     int check(hipError_t e, const char* what);
     int upload_pairs();
-    TileSet& tiles(int level, int ty);
+    TileSet& tiles(int level, int cls);
+    int joint_class(int level) const;
     void release();
 
     rf_params p_{};
@@ -90,8 +97,7 @@ class Batch {
     hipStream_t st_ = nullptr;
     bool prior_ = false, sparse_ = false;
     float flow_scale_ = 5.0f;
-    // tile sets: [level][ty class]; ty classes 4 (pointwise), 8 (joint), 16 (TV-L1); level
-    // kMaxLevels is the flat level-0 buffer (stride x height)
+    // tile sets: [level][TileClass]; level kMaxLevels is the flat level-0 buffer (stride x height)
     std::vector<TileSet> sets_;
     int2* maps_ = nullptr;
     Parity final_{};

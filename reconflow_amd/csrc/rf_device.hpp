@@ -121,29 +121,29 @@ __device__ __forceinline__ void l1_threshold(float dix, float diy, float dit, fl
     }
 }
 
+// lower = ((-lambda) * g2) / A is -upper bit for bit (negation commutes with correctly rounded
+// multiplication and division), and bu - t = bu + (-t) exactly, so the two outer cases share
+// one pair of divisions.
 __device__ __forceinline__ void l1_threshold_div(float dix, float diy, float dit, float bu, float bv, float lambda,
                                                  float A, float& du, float& dv) {
     float rho = (dix * bu + diy * bv) + dit;
     float g2 = dix * dix + diy * diy;
     float upper = (lambda * g2) / A;
-    float lower = ((-lambda) * g2) / A;
-    du = __builtin_nanf("");
-    dv = du;
+    float lower = -upper;
     if ((rho <= upper) && (rho >= lower)) {
-        float magi = dix * dix + diy * diy;
-        if (magi != 0.0f) {
-            du = bu - (rho * dix) / magi;
-            dv = bv - (rho * diy) / magi;
+        if (g2 != 0.0f) {
+            du = bu - (rho * dix) / g2;
+            dv = bv - (rho * diy) / g2;
         } else {
             du = bu;
             dv = bv;
         }
-    } else if (rho < lower) {
-        du = bu + (lambda * dix) / A;
-        dv = bv + (lambda * diy) / A;
-    } else if (rho > upper) {
-        du = bu - (lambda * dix) / A;
-        dv = bv - (lambda * diy) / A;
+    } else {
+        const float tx = (lambda * dix) / A, ty = (lambda * diy) / A;
+        const bool below = rho < lower;
+        du = below ? bu + tx : bu - tx;
+        dv = below ? bv + ty : bv - ty;
+        if (!(below || rho > upper)) du = dv = __builtin_nanf("");  // NaN rho
     }
 }
 
@@ -323,6 +323,28 @@ __device__ __forceinline__ float exchange_select(float (&m)[N]) {
         }
     }
     return m[NSEL - 1];
+}
+
+// The value exchange_select<9, 5> returns, for NaN-free taps, with a min/max network: sort the
+// three columns (min3 / med3 / max3), then med3(max of the lows, med3 of the middles, min of
+// the highs) is the 5th smallest (a min/max network that is right for every 0/1 input is
+// right for every totally ordered input). A nonzero 5th smallest has one bit pattern; a zero
+// one may be +0 or -0 depending on tap order, so a zero result is recomputed by the exchange
+// sort. Callers guarantee NaN-free taps.
+__device__ __forceinline__ float median9_finite(float (&m)[9]) {
+    float lo[3], mi[3], hi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float a = m[c], b = m[c + 3], d = m[c + 6];
+        lo[c] = fminf(fminf(a, b), d);
+        hi[c] = fmaxf(fmaxf(a, b), d);
+        mi[c] = __builtin_amdgcn_fmed3f(a, b, d);
+    }
+    const float r = __builtin_amdgcn_fmed3f(fmaxf(fmaxf(lo[0], lo[1]), lo[2]),
+                                            __builtin_amdgcn_fmed3f(mi[0], mi[1], mi[2]),
+                                            fminf(fminf(hi[0], hi[1]), hi[2]));
+    if (r != 0.0f) return r;
+    return exchange_select<9, 5>(m);
 }
 
 // Tap index of the 5x5 flow median (medianFilter.cu:31-50); -1 = tap keeps its 0.

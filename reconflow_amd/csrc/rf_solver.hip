@@ -79,6 +79,10 @@ int Solver::initialize(const rf_params& p) {
         set_error("rf_initialize: unknown path");
         return RF_ERR_ARG;
     }
+    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16) {
+        set_error("rf_initialize: unknown joint tiling");
+        return RF_ERR_ARG;
+    }
     release();
     g_ = Geometry();
     const int align = p.stride_alignment > 0 ? p.stride_alignment : align_;

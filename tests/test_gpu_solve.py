@@ -3,9 +3,9 @@ oracle, on seeded synthetic pairs.
 
 Bars:
   * the plain FP32 path equals the FP32 restatement bit for bit (whole solve);
-  * TV-L1 (configuration 0: 320x240, 4 levels, 3 warps, 30 iterations): mean EPE against
+  * TV-L1 (320x240, 4 levels, 3 warps, 30 iterations): mean EPE against
     the FP64 oracle <= 1e-3 px;
-  * joint flow/depth (configuration 1: 640x480, 5 levels, 5 warps, 50 iterations): with the
+  * joint flow/depth (640x480, 5 levels, 5 warps, 50 iterations): with the
     triangulation in FP64 the result equals the FP32+FP64-triangulation restatement bit for
     bit and its mean EPE against the FP64 oracle is <= 0.05 px. (With the reference's FP32
     triangulation the FP32-vs-FP64 gap is a property of the arithmetic, not of this code: the
@@ -31,60 +31,60 @@ def _epe(u, v, ru, rv, w):
 
 
 @pytest.fixture(scope="module")
-def cfg0(orc):
+def tvl1_320(orc):
     i0, i1, gu, gv = rf.make_flow_pair(320, 240, 0)
     p = orc.flow_params(320, 240, 4)
     return dict(i0=i0, i1=i1, o32=orc.solve_flow(p, i0, i1, "f32"), o64=orc.solve_flow(p, i0, i1, "f64"))
 
 
-def test_flow_config0_plain_bitexact_and_epe(cfg0):
+def test_flow_320x240_plain_bitexact_and_epe(tvl1_320):
     print("case: TV-L1 320x240 plain", flush=True)
     f = _flow()
-    f.copyImagesToDevice(cfg0["i0"], cfg0["i1"])
+    f.copyImagesToDevice(tvl1_320["i0"], tvl1_320["i1"])
     f.solveOpticalFlow(5.0)
     u, v = f.copyOpticalFlowToHost()
-    u32, v32 = cfg0["o32"]
+    u32, v32 = tvl1_320["o32"]
     assert np.array_equal(u[:, :320], u32[:, :320]) and np.array_equal(v[:, :320], v32[:, :320])
-    u64, v64 = cfg0["o64"]
+    u64, v64 = tvl1_320["o64"]
     e = _epe(u, v, u64, v64, 320)
     assert e.mean() <= 1e-3, e.mean()
 
 
 @pytest.mark.parametrize("path,graph", [(rf.PATH_PLAIN, True), (rf.PATH_FUSED, False), (rf.PATH_FUSED, True)],
                          ids=["plain-graph", "fused", "fused-graph"])
-def test_flow_config0_paths_equal_plain(cfg0, path, graph):
+def test_flow_320x240_paths_equal_plain(tvl1_320, path, graph):
     print(f"case: TV-L1 320x240 path={path} graph={graph}", flush=True)
     f = _flow(path=path, graph=graph)
-    u32, v32 = cfg0["o32"]
+    u32, v32 = tvl1_320["o32"]
     for rep in range(3):  # graph replays must all agree
-        f.copyImagesToDevice(cfg0["i0"], cfg0["i1"])
+        f.copyImagesToDevice(tvl1_320["i0"], tvl1_320["i1"])
         f.solveOpticalFlow(5.0)
         u, v = f.copyOpticalFlowToHost()
         assert np.array_equal(u[:, :320], u32[:, :320]) and np.array_equal(v[:, :320], v32[:, :320]), rep
 
 
-def test_flow_config0_fp16_storage_epe(cfg0):
+def test_flow_320x240_fp16_storage_epe(tvl1_320):
     # FP16 images / derivatives / duals, FP32 updates: accuracy stated against the FP64 oracle
     print("case: TV-L1 320x240 fused fp16 storage", flush=True)
     f = _flow(path=rf.PATH_FUSED, graph=True, storage=rf.STORAGE_F16)
-    f.copyImagesToDevice(cfg0["i0"], cfg0["i1"])
+    f.copyImagesToDevice(tvl1_320["i0"], tvl1_320["i1"])
     f.solveOpticalFlow(5.0)
     u, v = f.copyOpticalFlowToHost()
-    e = _epe(u, v, *cfg0["o64"], 320)
+    e = _epe(u, v, *tvl1_320["o64"], 320)
     print(f"fp16 storage TV-L1: mean EPE vs FP64 {e.mean():.3e}, p99 {np.percentile(e, 99):.3e}", flush=True)
     assert e.mean() <= 0.05
 
 
 def _recon(w, h, L, path=rf.PATH_PLAIN, graph=False, tri=0, fix=0, nw=5, ni=50, cvtype=rf.INPUT_32F, method=14,
-           prior=False, storage=rf.STORAGE_F32):
+           prior=False, storage=rf.STORAGE_F32, joint_tile=0):
     r = rf.ReconFlow()
     kw = dict(cvType=cvtype, path=path, use_graph=graph, storage=storage)
     if prior:
         r.initializeR(w, h, 1, L, 2.0, method, 50.0, 0.0, 0.1, 100.0, 33.3, 60.0, 100.0, 0.125, nw, ni, **kw)
     else:
         r._initializeR(w, h, 1, L, 2.0, method, 50.0, 0.0, 0.1, 100.0, 33.3, 60.0, 0.125, nw, ni, **kw)
-    if tri or fix:
-        r.params.tri_f64, r.params.fix_coarse_k = tri, fix
+    if tri or fix or joint_tile:
+        r.params.tri_f64, r.params.fix_coarse_k, r.params.joint_tile = tri, fix, joint_tile
         r._init_params(r.params)
         if prior:
             r._lib.rf_initialize_large_displacement(r._ctx)
@@ -92,7 +92,7 @@ def _recon(w, h, L, path=rf.PATH_PLAIN, graph=False, tri=0, fix=0, nw=5, ni=50, 
 
 
 @pytest.fixture(scope="module")
-def cfg1():
+def joint_640():
     return rf.make_recon_pair(640, 480, 2)
 
 
@@ -115,29 +115,29 @@ def _same(a, b, w):
 
 
 @pytest.mark.parametrize("tri", [0, 1])
-def test_recon_config1_bitexact(orc, cfg1, tri):
+def test_recon_640x480_bitexact(orc, joint_640, tri):
     print(f"case: joint 640x480 plain tri_f64={tri}", flush=True)
     r = _recon(640, 480, 5, tri=tri)
-    got = _run_recon(r, cfg1)
-    ref = _oracle_recon(orc, cfg1, 640, 480, 5, "f32", tri=tri)
+    got = _run_recon(r, joint_640)
+    ref = _oracle_recon(orc, joint_640, 640, 480, 5, "f32", tri=tri)
     assert _same(got, ref, 640)
 
 
-def test_recon_config1_fp64_triangulation_epe(orc, cfg1):
+def test_recon_640x480_fp64_triangulation_epe(orc, joint_640):
     print("case: joint 640x480 tri_f64 vs FP64 oracle", flush=True)
     r = _recon(640, 480, 5, tri=1)
-    u, v, X, Y, Z = _run_recon(r, cfg1)
-    u64, v64, X64, Y64, Z64 = _oracle_recon(orc, cfg1, 640, 480, 5, "f64")
+    u, v, X, Y, Z = _run_recon(r, joint_640)
+    u64, v64, X64, Y64, Z64 = _oracle_recon(orc, joint_640, 640, 480, 5, "f64")
     e = _epe(u, v, u64, v64, 640)
     assert e.mean() <= 0.05, e.mean()
 
 
 @pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16], ids=["f32", "f16"])
-def test_recon_config1_fused_epe(orc, cfg1, storage):
+def test_recon_640x480_fused_epe(orc, joint_640, storage):
     print(f"case: joint 640x480 fused graph tri_f64 storage={storage}", flush=True)
     r = _recon(640, 480, 5, path=rf.PATH_FUSED, graph=True, tri=1, storage=storage)
-    u, v, X, Y, Z = _run_recon(r, cfg1)
-    u64, v64, X64, Y64, Z64 = _oracle_recon(orc, cfg1, 640, 480, 5, "f64")
+    u, v, X, Y, Z = _run_recon(r, joint_640)
+    u64, v64, X64, Y64, Z64 = _oracle_recon(orc, joint_640, 640, 480, 5, "f64")
     e = _epe(u, v, u64, v64, 640)
     zr = np.abs(Z[:, :640] - Z64[:, :640]) / np.abs(Z64[:, :640]).clip(1e-6)
     print(f"joint storage={storage}: mean EPE vs FP64 {e.mean():.3e}, depth rel err median {np.median(zr):.3e}",
@@ -156,6 +156,19 @@ def test_recon_paths_equal_oracle(orc, path, graph, tri, fix):
     for rep in range(3):
         got = _run_recon(r, d)
         assert _same(got, ref, 176), rep
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3], ids=["64x8-rpt2", "32x16", "16x16"])
+@pytest.mark.parametrize("tri", [0, 1], ids=["tri32", "tri64"])
+def test_recon_joint_tilings_equal_oracle(orc, tile, tri):
+    # every tiling of the fused joint iteration gives the restatement's bits (uneven level
+    # sizes: 200x152 -> 100x76 -> 50x38, partial tiles on every level)
+    print(f"case: joint 200x152 fused graph tile={tile} tri={tri}", flush=True)
+    d = rf.make_recon_pair(200, 152, 11)
+    ref = _oracle_recon(orc, d, 200, 152, 3, "f32", tri=tri, nw=2, ni=6)
+    r = _recon(200, 152, 3, path=rf.PATH_FUSED, graph=True, tri=tri, nw=2, ni=6, joint_tile=tile)
+    for rep in range(2):
+        assert _same(_run_recon(r, d), ref, 200), rep
 
 
 def test_rgb_input_and_prior_flow(orc):
