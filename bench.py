@@ -58,8 +58,9 @@ def main() -> int:
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
-    ap.add_argument("--joint-tile", type=int, default=0, choices=[0, 1, 2, 3],
-                    help="fused joint tiling: 0 auto per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16 lanes")
+    ap.add_argument("--joint-tile", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="fused joint tiling: 0 autotuned per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16, "
+                         "4 16x16 capped at 96 VGPRs")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain-compare", action="store_true")

@@ -86,12 +86,14 @@ typedef struct rf_params {
     int joint_tile;        /* fused joint iteration tiling: RF_JOINT_TILE_* (results are identical) */
 } rf_params;
 
-/* Tilings of the fused joint iteration. AUTO picks per pyramid level: 32x16-lane tiles where
- * the level has enough of them to fill the device, 16x16 on small levels (shorter latency). */
+/* Tilings of the fused joint iteration (all give the same results). AUTO: the first joint solve
+ * times 16x16, 16x16_OCC5 and 32x16 on alternating warps of every pyramid level and keeps the
+ * fastest per level (needs >= 3 warps; otherwise 16x16). That first solve synchronises. */
 #define RF_JOINT_TILE_AUTO 0
-#define RF_JOINT_TILE_64x8_RPT2 1 /* 64x4 lanes, two rows per lane plus a ring pass */
-#define RF_JOINT_TILE_32x16 2     /* one pixel per lane, 30x14 pixels per workgroup */
-#define RF_JOINT_TILE_16x16 3     /* one pixel per lane, 14x14 pixels per workgroup */
+#define RF_JOINT_TILE_64x8_RPT2 1  /* 64x4 lanes, two rows per lane plus a ring pass */
+#define RF_JOINT_TILE_32x16 2      /* one pixel per lane, 30x14 pixels per workgroup */
+#define RF_JOINT_TILE_16x16 3      /* one pixel per lane, 14x14 pixels per workgroup */
+#define RF_JOINT_TILE_16x16_OCC5 4 /* as 16x16, registers capped for five waves per SIMD */
 
 /* Fills p with the reference defaults (Cudaflow.cu:99-101, Reconflow.cu:27-30) for
  * width x height: 1 warp, 100 iterations, lambda 100, theta 0.33, tau 0.25, alignment 32,

@@ -6,6 +6,7 @@ implementation.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 BUILD = Path(__file__).resolve().parent / "_build"
@@ -114,7 +115,9 @@ def _bind(lib: C.CDLL, protos: dict) -> C.CDLL:
 
 
 def lib_path() -> Path:
-    return BUILD / "librfhip.so"
+    """The in-tree build, or the library named by RFHIP_LIBRARY (another build of the same ABI)."""
+    alt = os.environ.get("RFHIP_LIBRARY")
+    return Path(alt) if alt else BUILD / "librfhip.so"
 
 
 def lib() -> C.CDLL:

@@ -51,25 +51,28 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
         raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
 
 
-def build_hip(jobs: int = 8) -> Path:
-    """Compile every HIP source for gfx950 and link librfhip.so."""
-    OUT.mkdir(exist_ok=True)
+def build_hip(jobs: int = 8, csrc: Path | None = None, out: Path | None = None) -> Path:
+    """Compile every HIP source for gfx950 and link librfhip.so (from ``csrc`` into ``out``,
+    by default the package's sources and ``_build``)."""
+    csrc = CSRC if csrc is None else Path(csrc)
+    out = OUT if out is None else Path(out)
+    out.mkdir(parents=True, exist_ok=True)
     hipcc = _hipcc()
     objs = []
     procs = []
     for src in HIP_SOURCES:
-        obj = OUT / (Path(src).stem + ".o")
+        obj = out / (Path(src).stem + ".o")
         objs.append(obj)
-        cmd = [hipcc, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [hipcc, *HIP_FLAGS, "-c", str(csrc / src), "-o", str(obj)]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
         if len(procs) >= jobs:
             _drain(procs)
     _drain(procs)
-    io_obj = OUT / "rf_io.o"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", str(CSRC / "rf_io.cpp"), "-o", str(io_obj)])
+    io_obj = out / "rf_io.o"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", str(csrc / "rf_io.cpp"), "-o", str(io_obj)])
     objs.append(io_obj)
-    lib = OUT / "librfhip.so"
-    tmp = OUT / "librfhip.so.tmp"
+    lib = out / "librfhip.so"
+    tmp = out / "librfhip.so.tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
           "-Wl,-rpath,/opt/rocm/lib"])
     tmp.replace(lib)

@@ -81,6 +81,10 @@ template <typename TS>
 __device__ __forceinline__ TS* as(float* p) {
     return reinterpret_cast<TS*>(p);
 }
+template <typename TS>
+__device__ __forceinline__ const TS* as(const float* p) {
+    return reinterpret_cast<const TS*>(p);
+}
 
 // Element i of a field through a 32-bit byte offset (fields hold at most 2^30 elements), so that
 // the access compiles to a scalar base plus one vector offset.
@@ -270,12 +274,56 @@ __global__ void __launch_bounds__(256) kb_upscale(Launch L, int l, int pu, int p
 
 __global__ void __launch_bounds__(256) kb_hsv(Launch L, int pu, float flowscale);
 
+// The fields one joint iteration reads (state n-1) and writes (state n) on one pyramid level,
+// resolved from the pair descriptor (the compiler loads only what is used).
+struct JointView {
+    const float *du0, *dv0, *p1u0, *p2u0, *p1v0, *p2v0, *U, *V, *sku0, *skv0, *up0, *vp0, *uc0, *vc0, *X0, *Y0, *Z0;
+    const float *Ix, *Iy, *Iz, *ufn, *vfn, *mask;
+    float *du1, *dv1, *p1u1, *p2u1, *p1v1, *p2v1, *sku1, *skv1, *up1, *vp1, *uc1, *vc1, *X1, *Y1, *Z1;
+    int w, h, s, tiles_x;
+    float inv_tiles_x;  // RN(1/tiles_x): tile row of a tile index without an integer division
+    PQ pq;
+    PQd pqd;
+    UcParams ucp;
+};
+
+__device__ __forceinline__ JointView joint_view(const PairFields& F, int l, int pd, int pf, int px, int pu, int tw) {
+    JointView v;
+    v.du0 = F.du[pd], v.dv0 = F.dv[pd], v.p1u0 = F.p1u[pd], v.p2u0 = F.p2u[pd], v.p1v0 = F.p1v[pd],
+    v.p2v0 = F.p2v[pd];
+    v.U = F.u[pu], v.V = F.v[pu];
+    v.sku0 = F.sku[pf], v.skv0 = F.skv[pf], v.up0 = F.up[pf], v.vp0 = F.vp[pf], v.uc0 = F.uc[pf], v.vc0 = F.vc[pf];
+    v.X0 = F.X[px], v.Y0 = F.Y[px], v.Z0 = F.Z[px];
+    v.Ix = F.Ix, v.Iy = F.Iy, v.Iz = F.Iz, v.ufn = F.ufn_l, v.vfn = F.vfn_l, v.mask = F.mask_l;
+    v.du1 = F.du[pd ^ 1], v.dv1 = F.dv[pd ^ 1], v.p1u1 = F.p1u[pd ^ 1], v.p2u1 = F.p2u[pd ^ 1],
+    v.p1v1 = F.p1v[pd ^ 1], v.p2v1 = F.p2v[pd ^ 1];
+    v.sku1 = F.sku[pf ^ 1], v.skv1 = F.skv[pf ^ 1], v.up1 = F.up[pf ^ 1], v.vp1 = F.vp[pf ^ 1], v.uc1 = F.uc[pf ^ 1],
+    v.vc1 = F.vc[pf ^ 1];
+    v.X1 = F.X[px ^ 1], v.Y1 = F.Y[px ^ 1], v.Z1 = F.Z[px ^ 1];
+    v.w = F.W[l], v.h = F.H[l], v.s = F.S[l];
+    v.tiles_x = (v.w + tw - 1) / tw;
+    v.inv_tiles_x = 1.0f / (float)v.tiles_x;
+    v.pq = F.pq[l];
+    v.pqd = F.pqd[l];
+    v.ucp = F.ucp[l];
+    return v;
+}
+
 struct IterArgs {
     Launch L;
     int level, pd, pf, px, pu, first;
     float lambda, theta, tau, aTv, aP, aFn, lf, lms;
     int sparse;
 };
+
+// Tile index -> (column, row) of tiles for small indices (< 2^24) with a float reciprocal of the
+// row length and one correction each way.
+__device__ __forceinline__ int2 tile_xy(int t, int tiles_x, float inv) {
+    int r = (int)((float)t * inv);
+    if (r * tiles_x > t) --r;
+    else if ((r + 1) * tiles_x <= t) ++r;
+    return make_int2(t - r * tiles_x, r);
+}
 
 // One joint iteration (see the file comment). TY = 8.
 template <int RPT, bool F64TRI, bool PRIOR, typename TS>
@@ -480,17 +528,19 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
 //   uc_n, Sk update, stores.
 // Same per-pixel arithmetic as kb_joint_iter, so the results are identical. SINGLE: the launch
 // covers one pair whose tiles are the dispatch indices (no tile list read).
-template <int EXW, int EXH, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
-__global__ void __launch_bounds__(EXW* EXH) kb_joint_px(IterArgs a) {
+// OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
+// schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
+// the FP64 triangulation, met without spilling, for occupancy on large levels).
+template <int EXW, int EXH, int OCC, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
+__global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
     constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
     __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
     const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
     const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const PairFields& F = a.L.P[pt.x];
-    const int l = a.level;
-    const int w = F.W[l], h = F.H[l], s = F.S[l];
-    const int tiles_x = (w + TW - 1) / TW;
-    const int x0 = (pt.y % tiles_x) * TW, y0 = (pt.y / tiles_x) * TH;
+    const JointView V = joint_view(a.L.P[pt.x], a.level, a.pd, a.pf, a.px, a.pu, TW);
+    const int w = V.w, h = V.h, s = V.s;
+    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int x0 = txy.x * TW, y0 = txy.y * TH;
     const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
     const int x = x0 - 1 + ex, y = y0 - 1 + ey;
     const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
@@ -499,16 +549,14 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_px(IterArgs a) {
     const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
-    const int pd = a.pd, pf = a.pf, px = a.px;
 
     // ---- loads
     float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
           p1vl = 0.0f, p2vd = 0.0f;
     if (!a.first) {
-        const TS *p1u0 = as<TS>(F.p1u[pd]), *p2u0 = as<TS>(F.p2u[pd]), *p1v0 = as<TS>(F.p1v[pd]),
-                 *p2v0 = as<TS>(F.p2v[pd]);
-        duh = el(F.du[pd], q);
-        dvh = el(F.dv[pd], q);
+        const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
+        duh = el(V.du0, q);
+        dvh = el(V.dv0, q);
         p1u = el(p1u0, q);
         p2u = el(p2u0, q);
         p1v = el(p1v0, q);
@@ -518,20 +566,21 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_px(IterArgs a) {
         p2ud = el(p2u0, qu);
         p2vd = el(p2v0, qu);
     }
-    const float uu = el(F.u[a.pu], q), vv = el(F.v[a.pu], q);
-    const float sku = el(F.sku[pf], q), skv = el(F.skv[pf], q), up0 = el(F.up[pf], q), vp0 = el(F.vp[pf], q);
-    const float uc0 = el(F.uc[pf], q), vc0 = el(F.vc[pf], q);
-    const float ix_ = (float)el(as<TS>(F.Ix), q), iy_ = (float)el(as<TS>(F.Iy), q), iz_ = (float)el(as<TS>(F.Iz), q);
-    const float *X0 = F.X[px], *Y0 = F.Y[px], *Z0 = F.Z[px];
+    const float uu = el(V.U, q), vv = el(V.V, q);
+    const float sku = el(V.sku0, q), skv = el(V.skv0, q), up0 = el(V.up0, q), vp0 = el(V.vp0, q);
+    const float uc0 = el(V.uc0, q), vc0 = el(V.vc0, q);
+    const float ix_ = (float)el(as<TS>(V.Ix), q), iy_ = (float)el(as<TS>(V.Iy), q), iz_ = (float)el(as<TS>(V.Iz), q);
     const unsigned qr = q + 1u, qd = q + (unsigned)s;
-    const float Xc = el(X0, q), Yc = el(Y0, q), Zc = el(Z0, q), Xr = el(X0, qr), Yr = el(Y0, qr), Zr = el(Z0, qr);
-    const float Xd = el(X0, qd), Yd = el(Y0, qd), Zd = el(Z0, qd), Xl = el(X0, ql), Yl = el(Y0, ql), Zl = el(Z0, ql);
-    const float Xu = el(X0, qu), Yu = el(Y0, qu), Zu = el(Z0, qu);
+    const float Xc = el(V.X0, q), Yc = el(V.Y0, q), Zc = el(V.Z0, q), Xr = el(V.X0, qr), Yr = el(V.Y0, qr),
+                Zr = el(V.Z0, qr);
+    const float Xd = el(V.X0, qd), Yd = el(V.Y0, qd), Zd = el(V.Z0, qd), Xl = el(V.X0, ql), Yl = el(V.Y0, ql),
+                Zl = el(V.Z0, ql);
+    const float Xu = el(V.X0, qu), Yu = el(V.Y0, qu), Zu = el(V.Z0, qu);
     float ufn = 0.0f, vfn = 0.0f, msk = 1.0f;
     if (PRIOR) {
-        ufn = el(F.ufn_l, q);
-        vfn = el(F.vfn_l, q);
-        if (a.sparse) msk = el(F.mask_l, q);
+        ufn = el(V.ufn, q);
+        vfn = el(V.vfn, q);
+        if (a.sparse) msk = el(V.mask, q);
     }
 
     // ---- data step + projection blend
@@ -556,10 +605,10 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_px(IterArgs a) {
     // ---- triangulation
     float oX, oY, oZ;
     if (F64TRI)
-        solve3d_px<double>(F.pqd[l], (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd,
+        solve3d_px<double>(V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd,
                            Xl, Yl, Zl, Xu, Yu, Zu, hl, hu, oX, oY, oZ);
     else
-        solve3d_px<float>(F.pq[l], a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
+        solve3d_px<float>(V.pq, a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
                           Yu, Zu, hl, hu, oX, oY, oZ);
     s_du[e] = nduh;
     s_dv[e] = ndvh;
@@ -582,11 +631,11 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_px(IterArgs a) {
     }
     float o1, o2;
     dual_px(dux, duy, p1u, p2u, fac, o1, o2);
-    el(as<TS>(F.p1u[pd ^ 1]), q) = st<TS>(o1);
-    el(as<TS>(F.p2u[pd ^ 1]), q) = st<TS>(o2);
+    el(as<TS>(V.p1u1), q) = st<TS>(o1);
+    el(as<TS>(V.p2u1), q) = st<TS>(o2);
     dual_px(dvx, dvy, p1v, p2v, fac, o1, o2);
-    el(as<TS>(F.p1v[pd ^ 1]), q) = st<TS>(o1);
-    el(as<TS>(F.p2v[pd ^ 1]), q) = st<TS>(o2);
+    el(as<TS>(V.p1v1), q) = st<TS>(o1);
+    el(as<TS>(V.p2v1), q) = st<TS>(o2);
 
     // ---- XYZ median, cleanup, reprojection, Sk update. Tap rule of medianFilter.cu:193-214 for
     // an interior pixel (taps lie within one pixel of it): a tap in column w or row h, or left
@@ -610,18 +659,18 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_px(IterArgs a) {
     float Xn, Yn, Zn;
     cleanup_px(oX, oY, oZ, xm, ym, zm, Xn, Yn, Zn);
     float ucn, vcn;
-    uc_px(F.ucp[l], Xn, Yn, Zn, x, y, ucn, vcn);
-    el(F.X[px ^ 1], q) = Xn;
-    el(F.Y[px ^ 1], q) = Yn;
-    el(F.Z[px ^ 1], q) = Zn;
-    el(F.uc[pf ^ 1], q) = ucn;
-    el(F.vc[pf ^ 1], q) = vcn;
-    el(F.du[pd ^ 1], q) = nduh;
-    el(F.dv[pd ^ 1], q) = ndvh;
-    el(F.up[pf ^ 1], q) = upn;
-    el(F.vp[pf ^ 1], q) = vpn;
-    el(F.sku[pf ^ 1], q) = sk_px(sku, uu, nduh, upn);
-    el(F.skv[pf ^ 1], q) = sk_px(skv, vv, ndvh, vpn);
+    uc_px(V.ucp, Xn, Yn, Zn, x, y, ucn, vcn);
+    el(V.X1, q) = Xn;
+    el(V.Y1, q) = Yn;
+    el(V.Z1, q) = Zn;
+    el(V.uc1, q) = ucn;
+    el(V.vc1, q) = vcn;
+    el(V.du1, q) = nduh;
+    el(V.dv1, q) = ndvh;
+    el(V.up1, q) = upn;
+    el(V.vp1, q) = vpn;
+    el(V.sku1, q) = sk_px(sku, uu, nduh, upn);
+    el(V.skv1, q) = sk_px(skv, vv, ndvh, vpn);
 }
 
 // One TV-L1 iteration: data step (tile + right column + bottom row), dual step (tile).
@@ -765,7 +814,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         set_error("rf_batch_create: unknown input type");
         return RF_ERR_ARG;
     }
-    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16) {
+    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_OCC5) {
         set_error("rf_batch_create: unknown joint tiling");
         return RF_ERR_ARG;
     }
@@ -899,24 +948,72 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         if (offs[k] >= 0) sets_[k].map = maps_ + offs[k];
     prior_ = false;
     dirty_ = true;
+    tuned_ = false;
+    jcls_.clear();
     return upload_pairs();
 }
 
 TileSet& Batch::tiles(int level, int cls) { return sets_[level * TC_COUNT + cls]; }
 
-// Joint iteration tiling of a level: 30x14-pixel tiles where the level has at least two per CU
-// of a 256-CU device (throughput), 14x14 on smaller levels (latency), unless fixed.
-int Batch::joint_class(int level) const {
+// Joint iteration tiling of a level for a warp: a fixed choice; during the tuning solve the
+// candidates in turn; afterwards the tuned choice (14x14 tiles until then).
+static const int kTuneCand[3] = {TC_J16, TC_J16O, TC_J32};
+constexpr int kNumCand = 3;
+
+int Batch::joint_class(int level, int warp) const {
     switch (p_.joint_tile) {
         case RF_JOINT_TILE_64x8_RPT2: return TC_J64;
         case RF_JOINT_TILE_32x16: return TC_J32;
         case RF_JOINT_TILE_16x16: return TC_J16;
+        case RF_JOINT_TILE_16x16_OCC5: return TC_J16O;
         default: break;
     }
-    long n32 = 0;
-    for (const Geo& g : geo_) n32 += (long)((g.W[level] + kTileW[TC_J32] - 1) / kTileW[TC_J32]) *
-                                     ((g.H[level] + kTileH[TC_J32] - 1) / kTileH[TC_J32]);
-    return n32 >= 512 ? TC_J32 : TC_J16;
+    if (tuning_) return kTuneCand[warp % kNumCand];
+    return tuned_ ? jcls_[level] : TC_J16;
+}
+
+// The tuning solve: an eager solve whose warps alternate between the candidate tilings, timed
+// per (level, warp) with events; each level keeps its faster candidate. Its results are those of
+// any other solve.
+int Batch::tune_tiling(bool recon) {
+    const int L = p_.n_levels, nw = p_.n_warp_iters;
+    tev_.assign(2 * L * nw, nullptr);
+    int rc = RF_OK;
+    for (auto& ev : tev_)
+        if ((rc = check(hipEventCreate(&ev), "hipEventCreate")) != RF_OK) break;
+    if (!rc) {
+        tuning_ = true;
+        hipError_t e = enqueue(recon);
+        tuning_ = false;
+        rc = check(e, "tuning solve");
+    }
+    if (!rc) rc = check(hipStreamSynchronize(st_), "tuning solve");
+    jcls_.assign(L, TC_J16);
+    for (int l = 0; l < L && !rc; ++l) {
+        double t[kNumCand] = {0, 0, 0};
+        int n[kNumCand] = {0, 0, 0};
+        for (int wi = 0; wi < nw; ++wi) {
+            float ms = 0;
+            if (check(hipEventElapsedTime(&ms, tev_[2 * (l * nw + wi)], tev_[2 * (l * nw + wi) + 1]),
+                      "hipEventElapsedTime")) {
+                rc = RF_ERR_HIP;
+                break;
+            }
+            t[wi % kNumCand] += ms;
+            ++n[wi % kNumCand];
+        }
+        double best = 0;
+        for (int c = 0; c < kNumCand && !rc; ++c)
+            if (n[c] && (c == 0 || t[c] / n[c] < best)) {
+                best = t[c] / n[c];
+                jcls_[l] = kTuneCand[c];
+            }
+    }
+    for (auto& ev : tev_)
+        if (ev) (void)hipEventDestroy(ev);
+    tev_.clear();
+    tuned_ = rc == RF_OK;
+    return rc;
 }
 
 int Batch::upload_pairs() {
@@ -1041,10 +1138,13 @@ static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
             hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, F64, PRIOR, TS>), grid, dim3(TXW, TRW), 0, st, a);
             break;
         case TC_J32:
-            hipLaunchKernelGGL((kb_joint_px<32, 16, SINGLE, F64, PRIOR, TS>), grid, dim3(32, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<32, 16, 5, SINGLE, F64, PRIOR, TS>), grid, dim3(32, 16), 0, st, a);
+            break;
+        case TC_J16O:
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
         default:
-            hipLaunchKernelGGL((kb_joint_px<16, 16, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
     }
 }
@@ -1103,8 +1203,9 @@ hipError_t Batch::enqueue_t(bool recon) {
             RF_LAUNCH(kb_deriv<TS>, tiles(level, TC_PIX), level);
             if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, TC_PIX), level, pr.d);
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
+            if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi)], st_));
             for (int it = 0; it < p_.n_solver_iters; ++it) {
-                const int cls = recon ? joint_class(level) : TC_FLOW;
+                const int cls = recon ? joint_class(level, wi) : TC_FLOW;
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
@@ -1122,6 +1223,7 @@ hipError_t Batch::enqueue_t(bool recon) {
                     pr.x ^= 1;
                 }
             }
+            if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi) + 1], st_));
             prof_.end(st_, RF_KID_FUSED_ITER, level, p_.n_solver_iters);
             RF_LAUNCH(kb_med5_add, tiles(level, TC_PIX), level, pr.d, pr.u);
             if (recon) {
@@ -1152,6 +1254,9 @@ int Batch::solve(bool recon, float flow_scale) {
         gexec_[kind] = nullptr;
     }
     flow_scale_ = flow_scale;
+    if (recon && p_.joint_tile == RF_JOINT_TILE_AUTO && !tuned_ && p_.n_warp_iters >= kNumCand && p_.n_solver_iters > 0 &&
+        !prof_.armed)
+        return tune_tiling(recon);
     if (!p_.use_graph || prof_.armed) {
         if (check(enqueue(recon), "batch solve")) return RF_ERR_HIP;
         return RF_OK;

@@ -35,9 +35,10 @@ struct PairFields {
 // Tile shapes (pixels per workgroup) of the batched launches: pointwise 64x4; joint iteration
 // 64x8 (two rows per lane), 30x14 and 14x14 (one pixel per lane of a 32x16 / 16x16 extended
 // tile); TV-L1 iteration 64x16.
-enum TileClass { TC_PIX = 0, TC_J64 = 1, TC_FLOW = 2, TC_J32 = 3, TC_J16 = 4, TC_COUNT = 5 };
-constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14};
-constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14};
+// (TC_J16O: the 14x14 tiling compiled for occupancy rather than latency.)
+enum TileClass { TC_PIX = 0, TC_J64 = 1, TC_FLOW = 2, TC_J32 = 3, TC_J16 = 4, TC_J16O = 5, TC_COUNT = 6 };
+constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14};
+constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14};
 
 // One launch's tile list: (pair, tile index within the pair).
 struct TileSet {
@@ -86,7 +87,8 @@ class Batch {
     int check(hipError_t e, const char* what);
     int upload_pairs();
     TileSet& tiles(int level, int cls);
-    int joint_class(int level) const;
+    int joint_class(int level, int warp) const;
+    int tune_tiling(bool recon);
     void release();
 
     rf_params p_{};
@@ -102,6 +104,11 @@ class Batch {
     int2* maps_ = nullptr;
     Parity final_{};
     bool dirty_ = true;
+    // joint tiling per level: fixed by the parameters, or chosen by timing the candidates on
+    // alternating warps of the first solve (every tiling gives the same results)
+    std::vector<int> jcls_;
+    bool tuned_ = false, tuning_ = false;
+    std::vector<hipEvent_t> tev_;
     hipGraphExec_t gexec_[2] = {nullptr, nullptr};
     Parity gfinal_[2];
     Profiler prof_;

@@ -158,7 +158,7 @@ def test_recon_paths_equal_oracle(orc, path, graph, tri, fix):
         assert _same(got, ref, 176), rep
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3], ids=["64x8-rpt2", "32x16", "16x16"])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4], ids=["64x8-rpt2", "32x16", "16x16", "16x16-occ5"])
 @pytest.mark.parametrize("tri", [0, 1], ids=["tri32", "tri64"])
 def test_recon_joint_tilings_equal_oracle(orc, tile, tri):
     # every tiling of the fused joint iteration gives the restatement's bits (uneven level
