@@ -220,16 +220,9 @@ __global__ void __launch_bounds__(256) kb_med5_add(Launch L, int l, int pd, int 
     BT_PIX(L, l)
     const float *du = F.du[pd], *dv = F.dv[pd];
     float mu[25], mv[25];
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            long q = median5_tap(ix + i - 2, iy + j - 2, w, h, s);
-            mu[j * 5 + i] = q >= 0 ? du[q] : 0.0f;
-            mv[j * 5 + i] = q >= 0 ? dv[q] : 0.0f;
-        }
-    F.u[pu][pos] = F.u[pu][pos] + exchange_select<25, 13>(mu);
-    F.v[pu][pos] = F.v[pu][pos] + exchange_select<25, 13>(mv);
+    gather5x5(du, dv, ix, iy, w, h, s, mu, mv);
+    F.u[pu][pos] = F.u[pu][pos] + median25(mu);
+    F.v[pu][pos] = F.v[pu][pos] + median25(mv);
 }
 
 // Median of the flow (Reconflow.cu:418-420): u[pu] -> u[pu^1].
@@ -237,16 +230,9 @@ __global__ void __launch_bounds__(256) kb_median5(Launch L, int l, int pu) {
     BT_PIX(L, l)
     const float *u = F.u[pu], *v = F.v[pu];
     float mu[25], mv[25];
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            long q = median5_tap(ix + i - 2, iy + j - 2, w, h, s);
-            mu[j * 5 + i] = q >= 0 ? u[q] : 0.0f;
-            mv[j * 5 + i] = q >= 0 ? v[q] : 0.0f;
-        }
-    F.u[pu ^ 1][pos] = exchange_select<25, 13>(mu);
-    F.v[pu ^ 1][pos] = exchange_select<25, 13>(mv);
+    gather5x5(u, v, ix, iy, w, h, s, mu, mv);
+    F.u[pu ^ 1][pos] = median25(mu);
+    F.v[pu ^ 1][pos] = median25(mv);
 }
 
 // Upscale from coarse level l+1 to level l (upscale.cu:14-27): the flow, and in joint mode

@@ -347,6 +347,44 @@ __device__ __forceinline__ float median9_finite(float (&m)[9]) {
     return exchange_select<9, 5>(m);
 }
 
+// The value exchange_select<9, 5> returns, for any taps: the network when no tap is NaN.
+__device__ __forceinline__ float median9(float (&m)[9]) {
+    bool nan = __builtin_isnan(m[8]);
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) nan |= __builtin_isunordered(m[k], m[k + 1]);
+    if (nan) return exchange_select<9, 5>(m);
+    return median9_finite(m);
+}
+
+#define RF_CE(a, b)                                \
+    do {                                           \
+        const float lo_ = fminf(m[a], m[b]);       \
+        m[b] = fmaxf(m[a], m[b]);                  \
+        m[a] = lo_;                                \
+    } while (0)
+#define RF_CE_MIN(a, b) m[a] = fminf(m[a], m[b])
+#define RF_CE_MAX(a, b) m[b] = fmaxf(m[a], m[b])
+#include "rf_median25.hpp"
+#undef RF_CE
+#undef RF_CE_MIN
+#undef RF_CE_MAX
+
+// The value exchange_select<25, 13> returns (the 5x5 flow median, medianFilter.cu:52-71), for
+// any taps: with no NaN tap the 13th smallest is the min/max network's output; a nonzero one has
+// a single bit pattern. NaN taps (the exchange sort's answer then depends on tap order) and a
+// zero result (+0 or -0 by tap order) take the exchange sort itself.
+__device__ __forceinline__ float median25(float (&in)[25]) {
+    bool nan = __builtin_isnan(in[24]);
+#pragma unroll
+    for (int k = 0; k < 24; k += 2) nan |= __builtin_isunordered(in[k], in[k + 1]);
+    float m[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) m[k] = in[k];
+    const float r = median25_network(m);
+    if (!nan && r != 0.0f) return r;
+    return exchange_select<25, 13>(in);
+}
+
 // Tap index of the 5x5 flow median (medianFilter.cu:31-50); -1 = tap keeps its 0.
 __device__ __forceinline__ long median5_tap(int col, int row, int w, int h, int s) {
     if (col >= 0 && col < w && row >= 0 && row < h) return col + (long)s * row;
@@ -355,6 +393,27 @@ __device__ __forceinline__ long median5_tap(int col, int row, int w, int h, int 
     if (col >= 0 && col < w && row < 0) return col;
     if (col >= 0 && col < w && row >= h) return col + (long)s * (h - 1);
     return -1;
+}
+
+// The 5x5 taps of two fields around (ix, iy) under median5_tap's rule, without branches: a tap
+// outside the image in one direction only reads the nearest edge pixel (the clamp), one
+// outside in both directions keeps 0.
+__device__ __forceinline__ void gather5x5(const float* a, const float* b, int ix, int iy, int w, int h, int s,
+                                          float (&ma)[25], float (&mb)[25]) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int row = iy + j - 2;
+        const bool rout = row < 0 || row >= h;
+        const unsigned rb = (unsigned)min(max(row, 0), h - 1) * (unsigned)s;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int col = ix + i - 2;
+            const bool zero = rout && (col < 0 || col >= w);
+            const unsigned q = rb + (unsigned)min(max(col, 0), w - 1);
+            ma[j * 5 + i] = zero ? 0.0f : a[q];
+            mb[j * 5 + i] = zero ? 0.0f : b[q];
+        }
+    }
 }
 
 // Tap index of the 3x3 XYZ median (medianFilter.cu:193-214): the right/bottom replacements

@@ -146,16 +146,9 @@ __global__ void __launch_bounds__(256) k_median5(const float* u, const float* v,
                                                   float* ov) {
     RF_PIX
     float mu[25], mv[25];
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            long q = median5_tap(ix + i - 2, iy + j - 2, w, h, s);
-            mu[j * 5 + i] = q >= 0 ? u[q] : 0.0f;
-            mv[j * 5 + i] = q >= 0 ? v[q] : 0.0f;
-        }
-    ou[pos] = exchange_select<25, 13>(mu);
-    ov[pos] = exchange_select<25, 13>(mv);
+    gather5x5(u, v, ix, iy, w, h, s, mu, mv);
+    ou[pos] = median25(mu);
+    ov[pos] = median25(mv);
 }
 
 // medianFilter.cu:166-243
@@ -172,9 +165,9 @@ __global__ void __launch_bounds__(256) k_median3d(const float* X, const float* Y
             mY[j * 3 + i] = q >= 0 ? Y[q] : 0.0f;
             mZ[j * 3 + i] = q >= 0 ? Z[q] : 0.0f;
         }
-    X1[pos] = exchange_select<9, 5>(mX);
-    Y1[pos] = exchange_select<9, 5>(mY);
-    Z1[pos] = exchange_select<9, 5>(mZ);
+    X1[pos] = median9(mX);
+    Y1[pos] = median9(mY);
+    Z1[pos] = median9(mZ);
 }
 
 // addKernel.cu:3-11
