@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the joint flow+depth solve on one MI355X.
 
-Workload: one 640x480 seeded synthetic pair (intrinsics f=500, c=(320,240); camera-1 pose
-|rotation| <= 3 deg, |t| = 0.1; 8 planes at depths 2-10), ReconFlow METHODR_TVL1_MS, 5 levels
-(scale divisor 2), 5 warps, 50 iterations, lambda 50, tau 0.125, alphaTv 33.3, alphaProj 60,
-lambdaf 0.1, lambdams 100. A step is one full solve (gray conversion, pyramid, all levels,
-warps and iterations) from images already resident on the GPU.
+Default workload: one 640x480 seeded synthetic pair (intrinsics f=500, c=(320,240); camera-1
+pose |rotation| <= 3 deg, |t| = 0.1; 8 planes at depths 2-10), ReconFlow METHODR_TVL1_MS,
+5 levels (scale divisor 2), 5 warps, 50 iterations, lambda 50, tau 0.125, alphaTv 33.3,
+alphaProj 60, lambdaf 0.1, lambdams 100. A step is one full solve (gray conversion, pyramid,
+all levels, warps and iterations) from images already resident on the GPU, replayed from a
+captured graph.
 
-Prints one JSON line: frames/s, the roofline of the dominant kernel (its algorithmic bytes
-per launch over its average launch time, timed with HIP events on the solver's stream), the
-FP64 CPU oracle's time on the same frame (the reported CPU baseline) and the mean EPE of the
-GPU flow against that FP64 result.
+Prints one JSON line: frames/s, the roofline of the dominant kernel (its algorithmic bytes per
+launch over its average launch time at level 0, timed with HIP events on the solver's stream),
+the FP64 CPU oracle's time on the same frame (the reported CPU baseline), the mean EPE of the
+GPU flow against that FP64 result, and the per-step (plain) GPU path's frames/s beside it.
+
+--workload batch64: 64 seeded pairs of uneven sizes (W in 160..1280, H in 120..720, multiples of
+16), even seeds TV-L1 (4 levels, 3 warps, 30 iterations; FP16 images, derivatives and duals),
+odd seeds joint (as above; FP16 derivatives, FP32 images); the two batches run on their own
+streams.
 """
 from __future__ import annotations
 
@@ -30,14 +36,16 @@ W, H, LEVELS, WARPS, ITERS = 640, 480, 5, 5, 50
 PEAK_HBM_GBS = 8000.0
 
 # Algorithmic bytes per pixel of one launch (each distinct element read once and written
-# once, FP32): see the kernel table of the design notes.
+# once, FP32).
 BYTES_PER_PX = {
     "fused_iter": 140.0,  # u,v,I(3) read; uhat(2), p(4), uproj(2), X(3), uc(2), sk(2) read + written
     "data": 68.0, "dual": 40.0, "uvproj": 40.0, "solve3d": 32.0, "median3d": 24.0, "cleanup": 36.0,
     "solve_uc": 20.0, "update_sk": 40.0,
 }
+FLOW_ITER_BYTES = 60.0  # one fused TV-L1 iteration: I(3) read; uhat(2), p(4) read + written
 KID = {"fused_iter": 0, "data": 1, "dual": 2, "uvproj": 3, "solve3d": 4, "median3d": 5, "cleanup": 6,
        "solve_uc": 7, "update_sk": 8}
+TILING = {0: "auto", 1: "64x8-rpt2", 2: "32x16", 3: "16x16", 4: "16x16-occ5"}
 
 
 def make_solver(rf, path, graph, tri, joint_tile=0):
@@ -50,50 +58,46 @@ def make_solver(rf, path, graph, tri, joint_tile=0):
     return r
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", choices=["fused", "plain"], default="fused")
-    ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
-    ap.add_argument("--joint-tile", type=int, default=0, choices=[0, 1, 2, 3, 4],
-                    help="fused joint tiling: 0 autotuned per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16, "
-                         "4 16x16 capped at 96 VGPRs")
-    ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain-compare", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a 256 MiB device-to-device copy 5 times (scales the FETCH_SIZE / WRITE_SIZE "
-                         "counters to bytes) and exit")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this solver runs one image pair per device and the "
-              "multi-GPU sharding (independent pairs per GPU) is not exercised here.", file=sys.stderr)
-        return 2
+def cpu_baseline(d, u, v):
+    """The FP64 restatement on the host: median of 5 timed frames after one warm-up at the
+    default OpenMP thread count, one frame on one thread, one FP32 frame; and the GPU flow's
+    mean EPE against the FP64 result."""
+    from oracle import oracle as orc
 
+    p = orc.recon_params(W, H, LEVELS)
+    args = (p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"])
+    threads = orc.get_threads()
+
+    def timed(kind):
+        t0 = time.perf_counter()
+        res = orc.solve_recon(*args, kind)
+        return time.perf_counter() - t0, res
+
+    _, ref = timed("f64")
+    runs = [timed("f64")[0] for _ in range(5)]
+    t_med = float(np.median(runs))
+    t32, _ = timed("f32")
+    orc.set_threads(1)
+    t1, _ = timed("f64")
+    orc.set_threads(threads)
+    epe = float(np.hypot(u[:, :W] - ref[0][:, :W], v[:, :W] - ref[1][:, :W]).mean())
+    cpu = {"value": round(1.0 / t_med, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+           "sample": f"one 640x480 joint frame (5 levels x 5 warps x 50 iterations) of the FP64 oracle, OpenMP "
+                     f"over rows on {threads} threads: median of 5 frames after a warm-up, {t_med:.2f} s/frame; "
+                     f"1 thread {t1:.2f} s/frame; FP32 restatement {t32:.2f} s/frame",
+           "one_thread_frames_per_s": round(1.0 / t1, 4), "fp32_frames_per_s": round(1.0 / t32, 4)}
+    return cpu, epe
+
+
+def run_joint(args, rf, torch):
     import ctypes as C
 
-    import torch
-
-    import reconflow_amd as rf
-
-    assert torch.cuda.is_available(), "bench needs a GPU"
-    if args.copy_calibration:
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "launches": 5}))
-        return 0
     d = rf.make_recon_pair(W, H, args.seed)
     path = rf.PATH_FUSED if args.path == "fused" else rf.PATH_PLAIN
     r = make_solver(rf, path, not args.no_graph, args.tri_f64, args.joint_tile)
     r.setCameraMatrices(d["K"], d["K"])
     r.copyImagesToDevice(d["i0"], d["i1"])  # resident for every step
-    for _ in range(args.warmup):
+    for _ in range(args.warmup):  # with an autotuned tiling the first solve is the tuning solve
         r.solveR(d["R"], d["t"], 50.0)
     r.synchronize()
     torch.cuda.synchronize()
@@ -106,10 +110,15 @@ def main() -> int:
     ms = dt / args.steps * 1e3
     fps = args.steps / dt
     u, v = r.copyOpticalFlowToHost()
-    X, Y, Z = r.copy3dToHost()
+
+    lib = r._lib
+    tiling = None
+    if path == rf.PATH_FUSED:
+        tl = (C.c_int * LEVELS)()
+        assert lib.rf_joint_tiling(r._ctx, tl, LEVELS) == 0
+        tiling = [TILING[t] for t in tl]
 
     # dominant kernel: time it live with HIP events on the solver's stream (eager solves)
-    lib = r._lib
     cand = ["fused_iter"] if path == rf.PATH_FUSED else ["data", "dual", "uvproj", "solve3d", "median3d",
                                                         "cleanup", "solve_uc", "update_sk"]
     per = {}
@@ -134,8 +143,8 @@ def main() -> int:
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists():
-        tj = json.loads(tfile.read_text())
-        traffic = tj.get(f"{dom}_{args.path}_{W}x{H}")
+        key = f"{dom}_{tiling[0] if tiling else args.path}_{W}x{H}"
+        traffic = json.loads(tfile.read_text()).get(key)
 
     plain_fps = None
     if path == rf.PATH_FUSED and not args.no_plain_compare:
@@ -152,21 +161,9 @@ def main() -> int:
         plain_fps = n_plain / (time.perf_counter() - t1)
         del rp
 
-    cpu = None
-    epe = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle as orc
-        p = orc.recon_params(W, H, LEVELS)
-        tc = time.perf_counter()
-        ref = orc.solve_recon(p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f64")
-        tcpu = time.perf_counter() - tc
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": round(1.0 / tcpu, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": "one full 640x480 joint frame (5 levels x 5 warps x 50 iterations) of the FP64 oracle, "
-                         f"OpenMP over rows, {tcpu:.2f} s"}
-        epe = float(np.hypot(u[:, :W] - ref[0][:, :W], v[:, :W] - ref[1][:, :W]).mean())
-
-    line = {
+    cpu, epe = (None, None) if args.no_cpu_baseline else cpu_baseline(d, u, v)
+    px_iter = sum((W >> l) * (H >> l) for l in range(LEVELS)) * WARPS * ITERS
+    return {
         "metric": "640x480 flow+depth frames/s at EPE<=0.05px vs FP64 oracle; % of HBM peak",
         "value": round(fps, 3),
         "unit": "frames/s",
@@ -180,17 +177,127 @@ def main() -> int:
         "dtype": "f32" + ("+f64-triangulation" if args.tri_f64 else ""),
         "data": "synthetic (seeded piecewise-planar scene pair, rendered on the host)",
         "config": {"workload": "joint flow+depth 640x480, 5 levels, 5 warps, 50 iterations, one pair per step",
-                   "path": args.path, "graph": not args.no_graph, "seed": args.seed, "joint_tile": args.joint_tile},
+                   "path": args.path, "graph": not args.no_graph, "seed": args.seed,
+                   "joint_tile": TILING[args.joint_tile], "joint_tiling_per_level": tiling},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "avg_launch_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": bytes_launch,
+                     "level": 0, "avg_launch_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": bytes_launch,
                      "avg_launch_us_per_level": per_level_us},
         "cpu_baseline": cpu,
         "epe_vs_fp64_oracle": None if epe is None else round(epe, 5),
         "accuracy_ok": None if epe is None else bool(epe <= 0.05),
         "plain_fp32_path_fps": None if plain_fps is None else round(plain_fps, 3),
-        "mpix_iter_per_s": round(fps * 409200 * WARPS * ITERS / 1e6, 1),
+        "speedup_vs_plain_path": None if plain_fps is None else round(fps / plain_fps, 3),
+        "mpix_iter_per_s": round(fps * px_iter / 1e6, 1),
     }
+
+
+def run_batch(args, rf, torch):
+    rng = np.random.default_rng(2024)
+    sizes = [(160 + 16 * int(rng.integers(0, 71)), 120 + 16 * int(rng.integers(0, 38))) for _ in range(64)]
+    groups = {}
+    for kind, method, L, nw, ni in (("flow", 0, 4, 3, 30), ("joint", 14, 5, 5, 50)):
+        idx = [k for k in range(64) if (k % 2 == 0) == (kind == "flow")]
+        ws, hs = [sizes[k][0] for k in idx], [sizes[k][1] for k in idx]
+        if kind == "flow":
+            p = rf.batch_params(0, L, nw, ni, lam=50.0, theta=0.3, tau=0.25, storage=rf.STORAGE_F16)
+        else:
+            p = rf.batch_params(14, L, nw, ni, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1,
+                                lambdams=100.0, tri_f64=args.tri_f64, storage=rf.STORAGE_F16_DERIV)
+        b = rf.Batch(p, ws, hs)
+        keep = []
+        for i, k in enumerate(idx):
+            s = b.stride(i)
+            if kind == "flow":
+                i0, i1, _, _ = rf.make_flow_pair(ws[i], hs[i], k, stride=s)
+            else:
+                dd = rf.make_recon_pair(ws[i], hs[i], k, stride=s)
+                i0, i1 = dd["i0"], dd["i1"]
+                b.set_camera(i, dd["K"], dd["K"], dd["R"], dd["t"])
+            t0, t1 = torch.from_numpy(i0).cuda(), torch.from_numpy(i1).cuda()
+            keep += [t0, t1]
+            b.set_images_device(i, t0.data_ptr(), t1.data_ptr())
+        px_iter = sum(sum((w_ >> l) * (h_ >> l) for l in range(L)) * nw * ni for w_, h_ in zip(ws, hs))
+        groups[kind] = dict(b=b, keep=keep, px_iter=px_iter, n=len(idx))
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        for g in groups.values():
+            g["b"].solve()
+        for g in groups.values():
+            g["b"].synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for g in groups.values():  # the two batches run concurrently on their own streams
+            g["b"].solve()
+        for g in groups.values():
+            g["b"].synchronize()
+    dt = time.perf_counter() - t0
+    pairs_s = 64 * args.steps / dt
+    alg = (groups["flow"]["px_iter"] * FLOW_ITER_BYTES + groups["joint"]["px_iter"] * BYTES_PER_PX["fused_iter"])
+    alg_gbs = alg * args.steps / dt / 1e9
+    return {
+        "metric": "batched pairs/s: 64 seeded pairs of uneven sizes, half TV-L1 half joint flow+depth",
+        "value": round(pairs_s, 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 updates; f16 storage: TV-L1 images, derivatives and duals, joint derivatives"
+                 + (", f64 triangulation" if args.tri_f64 else ""),
+        "data": "synthetic (seeded pairs, rendered on the host)",
+        "config": {"workload": "64 pairs, W 160..1280, H 120..720 (multiples of 16); TV-L1 4 levels x 3 warps x "
+                               "30 iterations, joint 5 levels x 5 warps x 50 iterations",
+                   "pixels": int(sum(w_ * h_ for w_, h_ in sizes))},
+        "mpix_iter_per_s": round((groups["flow"]["px_iter"] + groups["joint"]["px_iter"]) * args.steps / dt / 1e6, 1),
+        "algorithmic_gb_per_s_fp32_equivalent": round(alg_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(alg_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(alg_gbs / PEAK_HBM_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["joint640", "batch64"], default="joint640")
+    ap.add_argument("--path", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
+    ap.add_argument("--joint-tile", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="fused joint tiling: 0 autotuned per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16, "
+                         "4 16x16 capped at 96 VGPRs")
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain-compare", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (scales the FETCH_SIZE / WRITE_SIZE "
+                         "counters to bytes) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this solver runs one image pair per device and the "
+              "multi-GPU sharding (independent pairs per GPU) is not exercised here.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import reconflow_amd as rf
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "launches": 5}))
+        return 0
+    line = run_batch(args, rf, torch) if args.workload == "batch64" else run_joint(args, rf, torch)
     print(json.dumps(line))
     return 0
 

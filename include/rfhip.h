@@ -50,7 +50,8 @@ extern "C" {
 
 /* ---- storage of images, derivatives and dual fields ---- */
 #define RF_STORAGE_F32 0
-#define RF_STORAGE_F16 1 /* FP16 storage, every update computed in FP32 */
+#define RF_STORAGE_F16 1       /* FP16 images, derivatives (and TV-L1 duals); updates in FP32 */
+#define RF_STORAGE_F16_DERIV 2 /* FP32 images; FP16 derivatives (and TV-L1 duals); updates in FP32 */
 
 /* Status codes. */
 #define RF_OK 0
@@ -190,6 +191,11 @@ int rf_write_ply(const char* path, const float* X, const float* Y, const float* 
 #define RF_KID_UPDATE_SK 8
 int rf_profile_arm(rf_context* ctx, int kernel_id, int level);
 int rf_profile_read(rf_context* ctx, double* total_ms, int* launches);
+
+/* The fused joint tiling (RF_JOINT_TILE_*) each pyramid level uses, for levels 0..n-1: the
+ * fixed choice, or what the tuning solve picked (RF_JOINT_TILE_AUTO before it ran). Fused path
+ * only (no counterpart in the reference). */
+int rf_joint_tiling(rf_context* ctx, int* tiling, int n);
 
 /* ---- batch API: many pairs of uneven sizes per launch ---- */
 typedef struct rf_batch rf_batch;

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <limits>
 #include <vector>
+#include <omp.h>
 
 namespace orc {
 
@@ -1431,5 +1432,9 @@ void orc_pyramid(const orc::Params* P, int* W, int* H, int* S) {
 }
 
 int orc_params_size(void) { return (int)sizeof(orc::Params); }
+
+// Host threads of the OpenMP loops (for timing the restatement at a stated core count).
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -97,6 +97,15 @@ def call(name: str, kind: str, *args):
     return f(*cargs)
 
 
+def set_threads(n: int) -> None:
+    """OpenMP threads of the restatement's row loops."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
+
+
 def pyramid(p: Params):
     n = p.n_levels
     W, H, S = (C.c_int * n)(), (C.c_int * n)(), (C.c_int * n)()

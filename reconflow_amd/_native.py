@@ -61,6 +61,7 @@ _PROTOS = {
     "rf_write_ply": (_I, [C.c_char_p, _P, _P, _P, _P, _I, _I, _I, _F, _F, _F]),
     "rf_profile_arm": (_I, [_P, _I, _I]),
     "rf_profile_read": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_I)]),
+    "rf_joint_tiling": (_I, [_P, C.POINTER(_I), _I]),
     "rf_batch_create": (_I, [C.POINTER(_P), C.POINTER(RFParams), _I, _P, _P]),
     "rf_batch_destroy": (_I, [_P]),
     "rf_batch_stride": (_I, [_P, _I]),

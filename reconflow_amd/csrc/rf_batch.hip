@@ -67,8 +67,9 @@ struct Launch {
     if (ix >= w || iy >= h) return;                                            \
     const long pos = ix + (long)iy * s;
 
-// Storage of images, derivatives and dual fields: float, or __half with every update in FP32
-// (values rounded to nearest on store, widened exactly on load).
+// Storage types: TI for the images (pyramid, warped image), TS for the derivatives and the TV-L1
+// dual fields; float, or __half with every update in FP32 (values rounded to nearest on store,
+// widened exactly on load). The joint solver keeps its dual fields in FP32.
 template <typename TS>
 __device__ __forceinline__ TS st(float v) {
     return v;
@@ -198,14 +199,14 @@ __global__ void __launch_bounds__(256) kb_warp(Launch L, int l, int pu) {
 }
 
 // ComputeDerivativesKernel (computeDerivatives.cu:8-57).
-template <typename TS>
+template <typename TI, typename TS>
 __global__ void __launch_bounds__(256) kb_deriv(Launch L, int l) {
     BT_PIX(L, l)
-    const TS *I0 = as<TS>(F.I0[l]), *I1 = as<TS>(F.i1warp);
+    const TI *I0 = as<TI>(F.I0[l]), *I1 = as<TI>(F.i1warp);
     const long r = (long)iy * s;
     int xm2 = refl(ix - 2, w), xm1 = refl(ix - 1, w), xp1 = refl(ix + 1, w), xp2 = refl(ix + 2, w);
     int ym2 = refl(iy - 2, h), ym1 = refl(iy - 1, h), yp1 = refl(iy + 1, h), yp2 = refl(iy + 2, h);
-    auto g = [](const TS* p, long i) { return (float)p[i]; };
+    auto g = [](const TI* p, long i) { return (float)p[i]; };
     float t0 = d5(g(I0, r + xm2), g(I0, r + xm1), g(I0, r + xp1), g(I0, r + xp2));
     float t1 = d5(g(I1, r + xm2), g(I1, r + xm1), g(I1, r + xp1), g(I1, r + xp2));
     as<TS>(F.Ix)[pos] = st<TS>((t0 + t1) * 0.5f);
@@ -326,8 +327,7 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
     const int tid = threadIdx.y * TXW + threadIdx.x;
     const int pd = a.pd, pf = a.pf, px = a.px;
     const float *du0 = F.du[pd], *dv0 = F.dv[pd];
-    const TS *p1u0 = as<TS>(F.p1u[pd]), *p2u0 = as<TS>(F.p2u[pd]), *p1v0 = as<TS>(F.p1v[pd]),
-             *p2v0 = as<TS>(F.p2v[pd]);
+    const float *p1u0 = F.p1u[pd], *p2u0 = F.p2u[pd], *p1v0 = F.p1v[pd], *p2v0 = F.p2v[pd];
     const float *up0 = F.up[pf], *vp0 = F.vp[pf], *sku0 = F.sku[pf], *skv0 = F.skv[pf], *uc0 = F.uc[pf],
                 *vc0 = F.vc[pf];
     const float *X0 = F.X[px], *Y0 = F.Y[px], *Z0 = F.Z[px];
@@ -417,8 +417,7 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
         s_Z[e] = oZ;
     };
     const float fac = a.tau / a.theta;
-    TS *p1u1 = as<TS>(F.p1u[pd ^ 1]), *p2u1 = as<TS>(F.p2u[pd ^ 1]), *p1v1 = as<TS>(F.p1v[pd ^ 1]),
-       *p2v1 = as<TS>(F.p2v[pd ^ 1]);
+    float *p1u1 = F.p1u[pd ^ 1], *p2u1 = F.p2u[pd ^ 1], *p1v1 = F.p1v[pd ^ 1], *p2v1 = F.p2v[pd ^ 1];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int ex = threadIdx.x + 1, ey = threadIdx.y + TRW * r + 1;
@@ -438,11 +437,11 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
             }
             float o1, o2;
             dual_px(dux, duy, rp[r][0], rp[r][1], fac, o1, o2);
-            p1u1[q] = st<TS>(o1);
-            p2u1[q] = st<TS>(o2);
+            p1u1[q] = o1;
+            p2u1[q] = o2;
             dual_px(dvx, dvy, rp[r][2], rp[r][3], fac, o1, o2);
-            p1v1[q] = st<TS>(o1);
-            p2v1[q] = st<TS>(o2);
+            p1v1[q] = o1;
+            p2v1[q] = o2;
         }
     }
     if (hex >= 0) tri(hex, hey);
@@ -540,7 +539,7 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
           p1vl = 0.0f, p2vd = 0.0f;
     if (!a.first) {
-        const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
+        const float *p1u0 = V.p1u0, *p2u0 = V.p2u0, *p1v0 = V.p1v0, *p2v0 = V.p2v0;
         duh = el(V.du0, q);
         dvh = el(V.dv0, q);
         p1u = el(p1u0, q);
@@ -617,11 +616,11 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     }
     float o1, o2;
     dual_px(dux, duy, p1u, p2u, fac, o1, o2);
-    el(as<TS>(V.p1u1), q) = st<TS>(o1);
-    el(as<TS>(V.p2u1), q) = st<TS>(o2);
+    el(V.p1u1, q) = o1;
+    el(V.p2u1, q) = o2;
     dual_px(dvx, dvy, p1v, p2v, fac, o1, o2);
-    el(as<TS>(V.p1v1), q) = st<TS>(o1);
-    el(as<TS>(V.p2v1), q) = st<TS>(o2);
+    el(V.p1v1, q) = o1;
+    el(V.p2v1, q) = o2;
 
     // ---- XYZ median, cleanup, reprojection, Sk update. Tap rule of medianFilter.cu:193-214 for
     // an interior pixel (taps lie within one pixel of it): a tap in column w or row h, or left
@@ -804,6 +803,10 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         set_error("rf_batch_create: unknown joint tiling");
         return RF_ERR_ARG;
     }
+    if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV) {
+        set_error("rf_batch_create: unknown storage");
+        return RF_ERR_ARG;
+    }
     const int align = p.stride_alignment > 0 ? p.stride_alignment : 32;
     std::vector<Geo> geo(n);
     for (int i = 0; i < n; ++i) {
@@ -956,6 +959,16 @@ int Batch::joint_class(int level, int warp) const {
     }
     if (tuning_) return kTuneCand[warp % kNumCand];
     return tuned_ ? jcls_[level] : TC_J16;
+}
+
+int Batch::joint_tiling(int level) const {
+    if (p_.joint_tile != RF_JOINT_TILE_AUTO) return p_.joint_tile;
+    if (!tuned_ || level < 0 || level >= (int)jcls_.size()) return RF_JOINT_TILE_AUTO;
+    switch (jcls_[level]) {
+        case TC_J32: return RF_JOINT_TILE_32x16;
+        case TC_J16O: return RF_JOINT_TILE_16x16_OCC5;
+        default: return RF_JOINT_TILE_16x16;
+    }
 }
 
 // The tuning solve: an eager solve whose warps alternate between the candidate tilings, timed
@@ -1167,15 +1180,19 @@ static void launch_joint(int cls, bool single, bool f64, bool prior, hipStream_t
     } while (0)
 
 hipError_t Batch::enqueue(bool recon) {
-    return p_.storage == RF_STORAGE_F16 ? enqueue_t<__half>(recon) : enqueue_t<float>(recon);
+    switch (p_.storage) {
+        case RF_STORAGE_F16: return enqueue_t<__half, __half>(recon);
+        case RF_STORAGE_F16_DERIV: return enqueue_t<float, __half>(recon);
+        default: return enqueue_t<float, float>(recon);
+    }
 }
 
-template <typename TS>
+template <typename TI, typename TS>
 hipError_t Batch::enqueue_t(bool recon) {
     const int L = p_.n_levels;
     Parity pr{};
-    RF_LAUNCH(kb_gray<TS>, tiles(0, TC_PIX), p_.input_type);
-    for (int l = 1; l < L; ++l) RF_LAUNCH(kb_pyramid<TS>, tiles(l, TC_PIX), l);
+    RF_LAUNCH(kb_gray<TI>, tiles(0, TC_PIX), p_.input_type);
+    for (int l = 1; l < L; ++l) RF_LAUNCH(kb_pyramid<TI>, tiles(l, TC_PIX), l);
     RF_LAUNCH(kb_zero, tiles(FLAT, TC_PIX), recon ? 1 : 0);
     const bool fn = recon && prior_ && (p_.method == RF_METHODR_TVL1_MS_FN || p_.method == RF_METHODR_TVL1_MS_FNSPARSE);
     const float theta = recon ? 1.0f / p_.alpha_tv : p_.theta;  // Reconflow.cu:32
@@ -1185,8 +1202,8 @@ hipError_t Batch::enqueue_t(bool recon) {
             RF_LAUNCH(kb_sync_x, tiles(FLAT, TC_PIX), pr.x);
         }
         for (int wi = 0; wi < p_.n_warp_iters; ++wi) {
-            RF_LAUNCH(kb_warp<TS>, tiles(level, TC_PIX), level, pr.u);
-            RF_LAUNCH(kb_deriv<TS>, tiles(level, TC_PIX), level);
+            RF_LAUNCH(kb_warp<TI>, tiles(level, TC_PIX), level, pr.u);
+            RF_LAUNCH((kb_deriv<TI, TS>), tiles(level, TC_PIX), level);
             if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, TC_PIX), level, pr.d);
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi)], st_));

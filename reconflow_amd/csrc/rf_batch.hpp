@@ -75,6 +75,7 @@ class Batch {
     hipStream_t stream() const { return st_; }
     Profiler& prof() { return prof_; }
     const rf_params& params() const { return p_; }
+    int joint_tiling(int level) const;  // RF_JOINT_TILE_* in use at a level
 
   private:
     struct Geo {
@@ -82,7 +83,7 @@ class Batch {
         long n0 = 0, bufn = 0;
     };
     hipError_t enqueue(bool recon);
-    template <typename TS>
+    template <typename TI, typename TS>
     hipError_t enqueue_t(bool recon);
     int check(hipError_t e, const char* what);
     int upload_pairs();

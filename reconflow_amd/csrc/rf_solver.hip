@@ -83,6 +83,14 @@ int Solver::initialize(const rf_params& p) {
         set_error("rf_initialize: unknown joint tiling");
         return RF_ERR_ARG;
     }
+    if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV) {
+        set_error("rf_initialize: unknown storage");
+        return RF_ERR_ARG;
+    }
+    if (p.storage != RF_STORAGE_F32 && p.path != RF_PATH_FUSED) {
+        set_error("rf_initialize: FP16 storage is an option of the fused path");
+        return RF_ERR_UNSUPPORTED;
+    }
     release();
     g_ = Geometry();
     const int align = p.stride_alignment > 0 ? p.stride_alignment : align_;
@@ -493,6 +501,19 @@ int Solver::profile_arm(int kid, int level) {
     return RF_OK;
 }
 
+int Solver::joint_tiling(int* tiling, int n) {
+    if (!tiling || n < 0) {
+        set_error("rf_joint_tiling: null output or negative count");
+        return RF_ERR_ARG;
+    }
+    if (!batch_) {
+        set_error("rf_joint_tiling: the fused path is not in use");
+        return RF_ERR_STATE;
+    }
+    for (int l = 0; l < n; ++l) tiling[l] = batch_->joint_tiling(l);
+    return RF_OK;
+}
+
 int Solver::profile_read(double* total_ms, int* launches) {
     Profiler& pr = batch_ ? batch_->prof() : prof_;
     if (check(hipStreamSynchronize(stream()), "rf_profile_read")) return RF_ERR_HIP;
@@ -884,6 +905,11 @@ int rf_profile_arm(rf_context* ctx, int kernel_id, int level) {
 int rf_profile_read(rf_context* ctx, double* total_ms, int* launches) {
     RF_CTX(ctx);
     return ctx->s->profile_read(total_ms, launches);
+}
+
+int rf_joint_tiling(rf_context* ctx, int* tiling, int n) {
+    RF_CTX(ctx);
+    return ctx->s->joint_tiling(tiling, n);
 }
 
 }  // extern "C"

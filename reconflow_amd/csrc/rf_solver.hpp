@@ -95,6 +95,7 @@ class Solver {
     int sync();
     int profile_arm(int kid, int level);
     int profile_read(double* total_ms, int* launches);
+    int joint_tiling(int* tiling, int n);
     Profiler& prof() { return prof_; }
 
     const Geometry& geom() const { return g_; }
