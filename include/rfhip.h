@@ -29,11 +29,22 @@
 extern "C" {
 #endif
 
-/* ---- method identifiers (Cudaflow.h:20-28, Reconflow.h:46-52) ---- */
+/* ---- method identifiers (Cudaflow.h:20-28, Reconflow.h:46-52) ----
+ * rf_solve_flow follows _solveOpticalFlow's data-term choice (Cudaflow.cu:512-557): the
+ * Charbonnier-gradient step for TVCHARBGRAD and TVCHARBGRAD_FN, the Charbonnier step for
+ * TVCHARB, patch derivatives for TVL1PATCH, the L1 step for every other method (TVL1,
+ * SCENEFLOW_KINECT_TVL1, TVL1_FN, TVCHARB_FN, TVL1_INPAINTING). The masked step of
+ * TVL1_INPAINTING runs through rf_solve_inpainting. Methods other than TVL1 run on the plain
+ * path only. */
 #define RF_METHOD_TVL1 0
 #define RF_METHOD_TVCHARB 1
 #define RF_METHOD_TVCHARBGRAD 2
 #define RF_METHOD_TVL1PATCH 3
+#define RF_SCENEFLOW_KINECT_TVL1 4
+#define RF_METHOD_TVL1_FN 5
+#define RF_METHOD_TVCHARB_FN 6
+#define RF_METHOD_TVCHARBGRAD_FN 7
+#define RF_METHOD_TVL1_INPAINTING 8
 #define RF_METHODR_TVL1_MS 14
 #define RF_METHODR_TVL1_MS_FN 17
 #define RF_METHODR_TVL1_MS_FNSPARSE 20
@@ -139,6 +150,13 @@ int rf_set_camera(rf_context* ctx, const double K0[9], const double K1[9]);
  * copySparseOpticalFlowToDevice (Cudaflow.cu:332-343); mask may be NULL (all ones). */
 int rf_copy_initial_flow(rf_context* ctx, const float* u, const float* v);
 int rf_copy_sparse_flow(rf_context* ctx, const float* u, const float* v, const float* mask);
+/* CudaFlow::copyMaskToDevice (Inpainting.cu:3-7): stride x height float masks of
+ * METHOD_TVL1_INPAINTING (0 = no data term at the pixel); mask1 is kept and, as in the
+ * reference, not used by the data step. Synchronous. */
+int rf_copy_masks(rf_context* ctx, const float* mask0, const float* mask1);
+/* CudaFlow::copyDepthToDevice (Cudaflow.cu:282-286): stride x height 16-bit depth maps of
+ * SCENEFLOW_KINECT_TVL1. Synchronous. */
+int rf_copy_depth(rf_context* ctx, const uint16_t* depth0, const uint16_t* depth1);
 
 /* ---- solve ---- */
 /* CudaFlow::solveOpticalFlow(flowScale) (Cudaflow.cu:395-621). Asynchronous on the stream. */
@@ -147,6 +165,16 @@ int rf_solve_flow(rf_context* ctx, float flow_scale);
  * for camera 0 at the origin (solveR(R1, t1), Reconflow.cu:130-144). Asynchronous. */
 int rf_solve_recon(rf_context* ctx, const double* R0, const double* t0, const double* R1, const double* t1,
                    float flow_scale);
+/* CudaFlow::solveInpainting(flowScale) (Inpainting.cu:9-152): the TV-L1 solve with the data
+ * step skipped where mask0 == 0 (METHOD_TVL1_INPAINTING, masks copied). Asynchronous. */
+int rf_solve_inpainting(rf_context* ctx, float flow_scale);
+/* CudaFlow::solveSceneFlow(fx, fy, cx, cy, flowScale, sceneFlowScale) and
+ * solveSceneFlowAndPointCloud (Cudaflow.cu:816-871): the TV-L1 solve, the depth0 map converted
+ * to float, back-projection of the depth at the pixel and at the flow target (both from depth0,
+ * as the reference passes it twice) and the scene-flow colour map. With point_cloud = 0 the
+ * point clouds are not written. SCENEFLOW_KINECT_TVL1 with depth copied. Asynchronous. */
+int rf_solve_scene_flow(rf_context* ctx, float fx, float fy, float cx, float cy, float flow_scale,
+                        float scene_flow_scale, int point_cloud);
 /* Blocks until the context's stream is idle. */
 int rf_synchronize(rf_context* ctx);
 
@@ -159,6 +187,9 @@ int rf_copy_3d(rf_context* ctx, float* X, float* Y, float* Z);
 /* Device addresses of the current result fields (valid until the next solve). */
 int rf_get_result_device(const rf_context* ctx, const float** u, const float** v, const float** X,
                          const float** Y, const float** Z);
+/* CudaFlow::copySceneFlowToHost / copySceneFlowAndPointCloudToHost (Cudaflow.cu:345-362): 3 floats
+ * (x, y, z) per pixel, stride x height; any pointer may be NULL. Synchronous. */
+int rf_copy_scene_flow(rf_context* ctx, float* sceneflow, float* sceneflow_rgb, float* pcloud0, float* pcloud1);
 /* ReconFlow::removeEdgeArtifacts3D(surfaceWidth, maxSurfaceArea) (Reconflow.cu:468-477). */
 int rf_remove_edge_artifacts_3d(rf_context* ctx, int surface_width, float max_surface_area);
 
@@ -277,6 +308,33 @@ int rf_k_flow_to_hsv(const float* u, const float* v, int w, int h, int s, float*
 int rf_k_remove_edge_artifacts_3d(const float* X0, const float* Y0, const float* Z0, int surface_width,
                                   float max_surface_area, int w, int h, int s, float* X1, float* Y1, float* Z1,
                                   void* stream);                                             /* solve3dMinimalArea.cu:380-393 */
+/* ComputeDeriv (computeDeriv.cu:6-55): 5-point gradient of one image. */
+int rf_k_compute_deriv(const float* I, int w, int h, int s, float* Ix, float* Iy, void* stream);
+/* ComputeDerivativesPatch (computeDerivativesPatch.cu:8-85): ComputeDerivatives summed over the
+ * 5x5 patch. */
+int rf_k_compute_derivatives_patch(const float* I0, const float* I1, int w, int h, int s, float* Ix, float* Iy,
+                                   float* Iz, void* stream);
+/* SolveDataCharbForTV (solveDataCharbForTV.cu:87-112) and, with Ixx..Iyt non-NULL,
+ * SolveDataCharbForTVGrad (solveDataCharbForTVGrad.cu:105-135). */
+int rf_k_solve_data_charb_for_tv(const float* du0, const float* dv0, const float* duhat0, const float* dvhat0,
+                                 const float* pu1, const float* pu2, const float* pv1, const float* pv2,
+                                 const float* Ix, const float* Iy, const float* It, const float* Ixx,
+                                 const float* Ixy, const float* Ixt, const float* Iyx, const float* Iyy,
+                                 const float* Iyt, int w, int h, int s, float lambda, float lambdagrad, float theta,
+                                 float* du1, float* dv1, float* duhat1, float* dvhat1, void* stream);
+/* SolveDataL1Inpaint (solveDataL1Inpaint.cu:100-125); mask1 is not read. */
+int rf_k_solve_data_l1_inpaint(const float* duhat0, const float* dvhat0, const float* mask0, const float* mask1,
+                               const float* pu1, const float* pu2, const float* pv1, const float* pv2,
+                               const float* Ix, const float* Iy, const float* It, int w, int h, int s, float lambda,
+                               float theta, float* duhat1, float* dvhat1, void* stream);
+/* Cv16uTo32f (rgbToGray.cu:51-64, 104-110). */
+int rf_k_cv16u_to_32f(const uint16_t* in, float* out, int w, int h, int s, void* stream);
+/* SolveSceneFlow_ (solveSceneFlow.cu:95-160): 3 floats per pixel; pcloud0 / pcloud1 may be NULL. */
+int rf_k_solve_scene_flow(const float* u, const float* v, const float* depth0, const float* depth1, float fx,
+                          float fy, float cx, float cy, int w, int h, int s, float* pcloud0, float* pcloud1,
+                          float* sceneflow, void* stream);
+/* SceneFlowToHSV (flowToHSV.cu:315-321). */
+int rf_k_scene_flow_to_hsv(const float* sceneflow, int w, int h, int s, float* rgb, float flow_scale, void* stream);
 
 #ifdef __cplusplus
 }

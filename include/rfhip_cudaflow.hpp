@@ -111,6 +111,43 @@ class CudaFlow {
         return rc ? rc : rf_synchronize(ctx_);
     }
 
+    // Masked TV-L1 (Inpainting.cu:3-152) and Kinect scene flow (Cudaflow.cu:282-286, 345-362, 816-871).
+    int copyMaskToDevice(const Mat& mask0, const Mat& mask1) {
+        return rf_copy_masks(ctx_, mask0.ptr<float>(), mask1.ptr<float>());
+    }
+    int solveInpainting(float flowScale) {
+        int rc = rf_solve_inpainting(ctx_, flowScale);
+        return rc ? rc : rf_synchronize(ctx_);
+    }
+    int copyDepthToDevice(const Mat& depth0, const Mat& depth1) {
+        return rf_copy_depth(ctx_, depth0.ptr<uint16_t>(), depth1.ptr<uint16_t>());
+    }
+    int solveSceneFlow() { return solveSceneFlow(fx_, fy_, cx_, cy_, 5.0f, 1.0f); }
+    int solveSceneFlow(float focalx, float focaly, float cx, float cy, float flowScale) {
+        return solveSceneFlow(focalx, focaly, cx, cy, flowScale, 1.0f);
+    }
+    int solveSceneFlow(float focalx, float focaly, float cx, float cy, float opticalFlowScale, float sceneFlowScale) {
+        fx_ = focalx, fy_ = focaly, cx_ = cx, cy_ = cy;
+        int rc = rf_solve_scene_flow(ctx_, focalx, focaly, cx, cy, opticalFlowScale, sceneFlowScale, 0);
+        return rc ? rc : rf_synchronize(ctx_);
+    }
+    int solveSceneFlowAndPointCloud(float focalx, float focaly, float cx, float cy, float opticalFlowScale,
+                                    float sceneFlowScale) {
+        fx_ = focalx, fy_ = focaly, cx_ = cx, cy_ = cy;
+        int rc = rf_solve_scene_flow(ctx_, focalx, focaly, cx, cy, opticalFlowScale, sceneFlowScale, 1);
+        return rc ? rc : rf_synchronize(ctx_);
+    }
+    int copySceneFlowToHost(Mat& sceneflow) {
+        return rf_copy_scene_flow(ctx_, sceneflow.ptr<float>(), nullptr, nullptr, nullptr);
+    }
+    int copySceneFlowToHost(Mat& sceneflow, Mat& sceneflowrgb) {
+        return rf_copy_scene_flow(ctx_, sceneflow.ptr<float>(), sceneflowrgb.ptr<float>(), nullptr, nullptr);
+    }
+    int copySceneFlowAndPointCloudToHost(Mat& sceneflow, Mat& pcloud0, Mat& pcloud1, Mat& sceneflowrgb) {
+        return rf_copy_scene_flow(ctx_, sceneflow.ptr<float>(), sceneflowrgb.ptr<float>(), pcloud0.ptr<float>(),
+                                  pcloud1.ptr<float>());
+    }
+
     int iAlignUp(int n) const {
         int m = StrideAlignment_, mod = n % m;
         return mod ? n + m - mod : n;
@@ -159,6 +196,48 @@ class CudaFlow {
         rf_k_solve_smooth_dual_tv_global(duhat, dvhat, pu1, pu2, pv1, pv2, w, h, s, tau, theta, pu1s, pu2s, pv1s,
                                          pv2s, nullptr);
     }
+    void ComputeDeriv(float* I0, int w, int h, int s, float* Ix0, float* Iy0) {
+        rf_k_compute_deriv(I0, w, h, s, Ix0, Iy0, nullptr);
+    }
+    void ComputeDerivativesPatch(float* I0, float* I1, int w, int h, int s, float* Ix, float* Iy, float* Iz) {
+        rf_k_compute_derivatives_patch(I0, I1, w, h, s, Ix, Iy, Iz, nullptr);
+    }
+    void SolveDataCharbForTV(float* du0, float* dv0, float* duhat0, float* dvhat0, float* pu1, float* pu2,
+                             float* pv1, float* pv2, float* Ix, float* Iy, float* Iz, int w, int h, int s,
+                             float lambda, float theta, float* du1, float* dv1, float* duhat1, float* dvhat1) {
+        rf_k_solve_data_charb_for_tv(du0, dv0, duhat0, dvhat0, pu1, pu2, pv1, pv2, Ix, Iy, Iz, nullptr, nullptr,
+                                     nullptr, nullptr, nullptr, nullptr, w, h, s, lambda, 0.0f, theta, du1, dv1,
+                                     duhat1, dvhat1, nullptr);
+    }
+    void SolveDataCharbForTVGrad(float* du0, float* dv0, float* duhat0, float* dvhat0, float* pu1, float* pu2,
+                                 float* pv1, float* pv2, float* Ix, float* Iy, float* Iz, float* Ixx, float* Ixy,
+                                 float* Ixz, float* Iyx, float* Iyy, float* Iyz, int w, int h, int s, float lambda,
+                                 float lambdagrad, float theta, float* du1, float* dv1, float* duhat1,
+                                 float* dvhat1) {
+        rf_k_solve_data_charb_for_tv(du0, dv0, duhat0, dvhat0, pu1, pu2, pv1, pv2, Ix, Iy, Iz, Ixx, Ixy, Ixz, Iyx,
+                                     Iyy, Iyz, w, h, s, lambda, lambdagrad, theta, du1, dv1, duhat1, dvhat1, nullptr);
+    }
+    void SolveDataL1Inpaint(const float* duhat0, const float* dvhat0, const float* mask0, const float* mask1,
+                            const float* pu1, const float* pu2, const float* pv1, const float* pv2, const float* Ix,
+                            const float* Iy, const float* Iz, int w, int h, int s, float lambda, float theta,
+                            float* duhat1, float* dvhat1) {
+        rf_k_solve_data_l1_inpaint(duhat0, dvhat0, mask0, mask1, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda,
+                                   theta, duhat1, dvhat1, nullptr);
+    }
+    void Cv16uTo32f(const uint16_t* d_iCv16u, float* d_iGray, int w, int h, int s) {
+        rf_k_cv16u_to_32f(d_iCv16u, d_iGray, w, h, s, nullptr);
+    }
+    void SolveSceneFlow_(float* u, float* v, float* depth0, float* depth1, float fx, float fy, float cx, float cy,
+                         int w, int h, int s, float* sceneflow) {
+        rf_k_solve_scene_flow(u, v, depth0, depth1, fx, fy, cx, cy, w, h, s, nullptr, nullptr, sceneflow, nullptr);
+    }
+    void SolveSceneFlow_(float* u, float* v, float* depth0, float* depth1, float fx, float fy, float cx, float cy,
+                         int w, int h, int s, float* pcloud0, float* pcloud1, float* sceneflow) {
+        rf_k_solve_scene_flow(u, v, depth0, depth1, fx, fy, cx, cy, w, h, s, pcloud0, pcloud1, sceneflow, nullptr);
+    }
+    void SceneFlowToHSV(float* sceneflow, int w, int h, int s, float* uRGB, float flowscale) {
+        rf_k_scene_flow_to_hsv(sceneflow, w, h, s, uRGB, flowscale, nullptr);
+    }
     void FlowToHSV(float* u, float* v, int w, int h, int s, float* uRGB, float flowscale) {
         rf_k_flow_to_hsv(u, v, w, h, s, uRGB, flowscale, nullptr);
     }
@@ -180,7 +259,10 @@ class CudaFlow {
 
   protected:
     void applyOptions(rf_params& p) const {
-        p.path = path_;
+        // the fused path runs METHOD_TVL1 and the joint methods; the others run per step
+        const bool fusable = p.method == RF_METHOD_TVL1 || p.method == RF_METHODR_TVL1_MS ||
+                             p.method == RF_METHODR_TVL1_MS_FN || p.method == RF_METHODR_TVL1_MS_FNSPARSE;
+        p.path = fusable ? path_ : RF_PATH_PLAIN;
         p.use_graph = graph_ ? 1 : 0;
         p.storage = storage_;
     }
@@ -198,6 +280,7 @@ class CudaFlow {
     int path_ = RF_PATH_FUSED;
     bool graph_ = false;
     int storage_ = RF_STORAGE_F32;
+    float fx_ = 0.0f, fy_ = 0.0f, cx_ = 0.0f, cy_ = 0.0f;  // depth camera of solveSceneFlow()
 };
 
 class ReconFlow : public CudaFlow {

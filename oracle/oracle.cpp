@@ -891,6 +891,34 @@ void scene_flow(const T* u, const T* v, const T* depth0, const T* depth1, T fx, 
         }
 }
 
+// SceneFlowToHSVKernel (flowToHSV.cu:132-190): sf holds (x, y, z) per pixel. The sector bounds
+// are double literals in the reference, so H is compared in double.
+template <typename T>
+void scene_flow_to_hsv(const T* sf, int w, int h, int s, T* rgb, T flowscale) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T um = sf[3 * p] / flowscale, vm = sf[3 * p + 1] / flowscale, wm = sf[3 * p + 2] / flowscale;
+            T H = std::atan2(vm, um) + T(3.1416f);
+            T C = std::sqrt(um * um + vm * vm);
+            T Hp = H / T(1.0472f);
+            T X = C * (T(1) - std::fabs(std::fmod(Hp, T(2)) - T(1)));
+            T m = wm - C;
+            double Hd = (double)H;
+            T r, g, b;
+            if (Hd < 6.2832 && Hd >= 5.236) r = C, g = 0, b = X;
+            else if (Hd < 5.236 && Hd >= 4.1888) r = X, g = 0, b = C;
+            else if (Hd < 4.1888 && Hd >= 3.1416) r = 0, g = X, b = C;
+            else if (Hd < 3.1416 && Hd >= 2.0944) r = 0, g = C, b = X;
+            else if (Hd < 2.0944 && Hd >= 1.0472) r = X, g = C, b = 0;
+            else r = C, g = X, b = 0;
+            rgb[3 * p] = r + m;
+            rgb[3 * p + 1] = g + m;
+            rgb[3 * p + 2] = b + m;
+        }
+}
+
 // ---------------------------------------------------------------------------------
 // Host orchestration.
 
@@ -984,15 +1012,15 @@ static void gray_input(const Params& P, const void* in, T* out, int s0) {
 template <typename T>
 // Method branches follow Cudaflow.cu:426-578: Charbonnier-gradient for methods 2 and 7 (with
 // gradient pyramids, :426-461, and three warps / derivative sets per warp, :489-504),
-// Charbonnier for method 1, patch derivatives for method 3, the L1 step otherwise; method 8 is
-// the masked L1 step of CudaFlow::_solveInpainting (Inpainting.cu:13-152).
+// Charbonnier for method 1, patch derivatives for method 3, the L1 step otherwise. With mask0_in
+// it is CudaFlow::_solveInpainting (Inpainting.cu:13-152): the masked L1 step, plain derivatives.
 int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, float* v_out, const float* mask0_in = nullptr) {
     std::vector<int> pW, pH, pS;
     pyramid(P, pW, pH, pS);
     const int w0 = pW[0], h0 = pH[0], s0 = pS[0];
     const long n = (long)s0 * h0;
     const int m = P.method;
-    const bool cg = (m == 2 || m == 7), inpaint = (m == 8);
+    const bool cg = (m == 2 || m == 7), inpaint = (mask0_in != nullptr);
     Pool<T> pool(n, s0);
     std::vector<T*> pI0(P.n_levels), pI1(P.n_levels), pIx0(P.n_levels), pIx1(P.n_levels), pIy0(P.n_levels),
         pIy1(P.n_levels), pM0(P.n_levels);
@@ -1061,7 +1089,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
                 warp(pIx1[level], w, h, s, u, v, ix1warp);
                 warp(pIy1[level], w, h, s, u, v, iy1warp);
             }
-            if (m == 3)
+            if (m == 3 && !inpaint)
                 derivatives_patch(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
             else
                 derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
@@ -1070,14 +1098,14 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
                 derivatives(pIy0[level], iy1warp, w, h, s, Iyx, Iyy, Iyz);
             }
             for (int it = 0; it < P.n_solver_iters; ++it) {
-                if (cg || m == 1) {
+                if (inpaint) {
+                    data_l1_inpaint(dumed, dvmed, pM0[level], pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta,
+                                    dumeds, dvmeds);
+                } else if (cg || m == 1) {
                     data_charb(du, dv, dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, Ixx, Ixy, Ixz, Iyx, Iyy, Iyz, w,
                                h, s, lambda, lambdagrad, theta, dus, dvs, dumeds, dvmeds);
                     std::swap(du, dus);
                     std::swap(dv, dvs);
-                } else if (inpaint) {
-                    data_l1_inpaint(dumed, dvmed, pM0[level], pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta,
-                                    dumeds, dvmeds);
                 } else {
                     data_l1(dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta, dumeds, dvmeds);
                 }
@@ -1382,6 +1410,9 @@ DEF2(double, f64)
 DEF2(float, f32)
 
 #define DEF3(T, SUF)                                                                                          \
+    void orc_scene_flow_to_hsv_##SUF(const T* sf, int w, int h, int s, T* rgb, float fs) {                     \
+        orc::scene_flow_to_hsv<T>(sf, w, h, s, rgb, (T)fs);                                                  \
+    }                                                                                                        \
     void orc_flow_to_hsv_##SUF(const T* u, const T* v, int w, int h, int s, T* rgb, float fs) {              \
         orc::flow_to_hsv<T>(u, v, w, h, s, rgb, (T)fs);                                                      \
     }                                                                                                        \

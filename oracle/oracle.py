@@ -125,6 +125,19 @@ def solve_flow(p: Params, i0: np.ndarray, i1: np.ndarray, kind: str = "f64"):
     return u, v
 
 
+def solve_flow_masked(p: Params, i0: np.ndarray, i1: np.ndarray, mask0: np.ndarray, kind: str = "f64"):
+    """CudaFlow::solveInpainting: the TV-L1 solve with the masked L1 step (mask0 (h, stride) float32)."""
+    s0 = pyramid(p)[2][0]
+    u = np.zeros((p.height, s0), np.float32)
+    v = np.zeros_like(u)
+    f = getattr(lib(), f"orc_solve_flow_masked_{kind}")
+    f.restype = C.c_int
+    m = np.ascontiguousarray(mask0, dtype=np.float32)
+    rc = f(C.byref(p), _p(np.ascontiguousarray(i0)), _p(np.ascontiguousarray(i1)), _p(m), _p(u), _p(v))
+    assert rc == 0
+    return u, v
+
+
 def solve_recon(p: Params, K0, K1, R0, t0, R1, t1, i0, i1, kind="f64", ufn=None, vfn=None, mask=None):
     s0 = pyramid(p)[2][0]
     outs = [np.zeros((p.height, s0), np.float32) for _ in range(5)]
@@ -138,10 +151,11 @@ def solve_recon(p: Params, K0, K1, R0, t0, R1, t1, i0, i1, kind="f64", ufn=None,
     return tuple(outs)
 
 
-def flow_params(w, h, n_levels, scale=2.0, n_warp=3, n_iter=30, lam=50.0, theta=0.3, tau=0.25, input_type=0):
+def flow_params(w, h, n_levels, scale=2.0, n_warp=3, n_iter=30, lam=50.0, theta=0.3, tau=0.25, input_type=0,
+                method=0, lambdagrad=400.0):
     p = Params()
-    p.width, p.height, p.n_levels, p.scale, p.method = w, h, n_levels, scale, 0
-    p.lambda_, p.lambdagrad, p.theta, p.tau = lam, 400.0, theta, tau
+    p.width, p.height, p.n_levels, p.scale, p.method = w, h, n_levels, scale, method
+    p.lambda_, p.lambdagrad, p.theta, p.tau = lam, lambdagrad, theta, tau
     p.n_warp_iters, p.n_solver_iters = n_warp, n_iter
     p.stride_alignment, p.input_type = 32, input_type
     return p

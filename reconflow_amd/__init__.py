@@ -119,10 +119,45 @@ class CudaFlow:
         check(self._lib.rf_copy_images_device(self._ctx, d_i0, d_i1), "rf_copy_images_device")
         return 0
 
+    def _plane(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.shape != (self.height, self.stride):
+            raise ValueError(f"plane must be {(self.height, self.stride)} (padded to the stride), got {a.shape}")
+        return a
+
+    def copyMaskToDevice(self, mask0, mask1) -> int:
+        """CudaFlow::copyMaskToDevice (Inpainting.cu:3-7): float masks of METHOD_TVL1_INPAINTING."""
+        m0, m1 = self._plane(mask0, np.float32), self._plane(mask1, np.float32)
+        check(self._lib.rf_copy_masks(self._ctx, m0.ctypes.data, m1.ctypes.data), "rf_copy_masks")
+        return 0
+
+    def copyDepthToDevice(self, depth0, depth1) -> int:
+        """CudaFlow::copyDepthToDevice (Cudaflow.cu:282-286): 16-bit depth of SCENEFLOW_KINECT_TVL1."""
+        d0, d1 = self._plane(depth0, np.uint16), self._plane(depth1, np.uint16)
+        check(self._lib.rf_copy_depth(self._ctx, d0.ctypes.data, d1.ctypes.data), "rf_copy_depth")
+        return 0
+
     # ---- solve -----------------------------------------------------------------------
     def solveOpticalFlow(self, flowScale: float = 5.0) -> int:
         """CudaFlow::solveOpticalFlow (Cudaflow.cu:395-621); asynchronous."""
         check(self._lib.rf_solve_flow(self._ctx, flowScale), "rf_solve_flow")
+        return 0
+
+    def solveInpainting(self, flowScale: float = 5.0) -> int:
+        """CudaFlow::solveInpainting (Inpainting.cu:9-152); asynchronous."""
+        check(self._lib.rf_solve_inpainting(self._ctx, flowScale), "rf_solve_inpainting")
+        return 0
+
+    def solveSceneFlow(self, focalx, focaly, cx, cy, opticalFlowScale=5.0, sceneFlowScale=1.0) -> int:
+        """CudaFlow::solveSceneFlow (Cudaflow.cu:816-848); asynchronous."""
+        check(self._lib.rf_solve_scene_flow(self._ctx, focalx, focaly, cx, cy, opticalFlowScale, sceneFlowScale, 0),
+              "rf_solve_scene_flow")
+        return 0
+
+    def solveSceneFlowAndPointCloud(self, focalx, focaly, cx, cy, opticalFlowScale=5.0, sceneFlowScale=1.0) -> int:
+        """CudaFlow::solveSceneFlowAndPointCloud (Cudaflow.cu:850-871); asynchronous."""
+        check(self._lib.rf_solve_scene_flow(self._ctx, focalx, focaly, cx, cy, opticalFlowScale, sceneFlowScale, 1),
+              "rf_solve_scene_flow")
         return 0
 
     def synchronize(self) -> int:
@@ -141,6 +176,22 @@ class CudaFlow:
         check(self._lib.rf_copy_flow(self._ctx, u.ctypes.data, v.ctypes.data,
                                      rgb.ctypes.data if with_rgb else None), "rf_copy_flow")
         return (u, v, rgb) if with_rgb else (u, v)
+
+    def copySceneFlowToHost(self, with_rgb: bool = False):
+        """CudaFlow::copySceneFlowToHost (Cudaflow.cu:345-354): (h, stride, 3) float (x, y, z)."""
+        sf = np.empty((self.height, self.stride, 3), np.float32)
+        rgb = np.empty_like(sf) if with_rgb else None
+        check(self._lib.rf_copy_scene_flow(self._ctx, sf.ctypes.data, rgb.ctypes.data if with_rgb else None, None,
+                                           None), "rf_copy_scene_flow")
+        return (sf, rgb) if with_rgb else sf
+
+    def copySceneFlowAndPointCloudToHost(self):
+        """CudaFlow::copySceneFlowAndPointCloudToHost (Cudaflow.cu:356-362): (sf, pc0, pc1, rgb)."""
+        arrs = [np.empty((self.height, self.stride, 3), np.float32) for _ in range(4)]
+        sf, pc0, pc1, rgb = arrs
+        check(self._lib.rf_copy_scene_flow(self._ctx, sf.ctypes.data, rgb.ctypes.data, pc0.ctypes.data,
+                                           pc1.ctypes.data), "rf_copy_scene_flow")
+        return sf, pc0, pc1, rgb
 
     def resultDevicePointers(self):
         ptrs = [C.c_void_p() for _ in range(5)]

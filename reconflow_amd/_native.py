@@ -50,6 +50,11 @@ _PROTOS = {
     "rf_copy_initial_flow": (_I, [_P, _P, _P]),
     "rf_copy_sparse_flow": (_I, [_P, _P, _P, _P]),
     "rf_solve_flow": (_I, [_P, _F]),
+    "rf_copy_masks": (_I, [_P, _P, _P]),
+    "rf_copy_depth": (_I, [_P, _P, _P]),
+    "rf_solve_inpainting": (_I, [_P, _F]),
+    "rf_solve_scene_flow": (_I, [_P, _F, _F, _F, _F, _F, _F, _I]),
+    "rf_copy_scene_flow": (_I, [_P, _P, _P, _P, _P]),
     "rf_solve_recon": (_I, [_P, _D, _D, _D, _D, _F]),
     "rf_synchronize": (_I, [_P]),
     "rf_copy_flow": (_I, [_P, _P, _P, _P]),
@@ -96,6 +101,13 @@ _PROTOS = {
     "rf_k_update_sk": (_I, [_P] * 8 + [_I, _I, _I, _P, _P, _P]),
     "rf_k_flow_to_hsv": (_I, [_P, _P, _I, _I, _I, _P, _F, _P]),
     "rf_k_remove_edge_artifacts_3d": (_I, [_P, _P, _P, _I, _F, _I, _I, _I, _P, _P, _P, _P]),
+    "rf_k_compute_deriv": (_I, [_P, _I, _I, _I, _P, _P, _P]),
+    "rf_k_compute_derivatives_patch": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P]),
+    "rf_k_solve_data_charb_for_tv": (_I, [_P] * 17 + [_I, _I, _I, _F, _F, _F] + [_P] * 5),
+    "rf_k_solve_data_l1_inpaint": (_I, [_P] * 11 + [_I, _I, _I, _F, _F, _P, _P, _P]),
+    "rf_k_cv16u_to_32f": (_I, [_P, _P, _I, _I, _I, _P]),
+    "rf_k_solve_scene_flow": (_I, [_P, _P, _P, _P, _F, _F, _F, _F, _I, _I, _I, _P, _P, _P, _P]),
+    "rf_k_scene_flow_to_hsv": (_I, [_P, _I, _I, _I, _P, _F, _P]),
 }
 
 _SYNTH_PROTOS = {

@@ -178,4 +178,55 @@ int rf_k_remove_edge_artifacts_3d(const float* X0, const float* Y0, const float*
                 "rf_k_remove_edge_artifacts_3d");
 }
 
+int rf_k_compute_deriv(const float* I, int w, int h, int s, float* Ix, float* Iy, void* stream) {
+    GEOM("rf_k_compute_deriv");
+    return done(launch_deriv_single(I, w, h, s, Ix, Iy, ST), "rf_k_compute_deriv");
+}
+int rf_k_compute_derivatives_patch(const float* I0, const float* I1, int w, int h, int s, float* Ix, float* Iy,
+                                   float* Iz, void* stream) {
+    GEOM("rf_k_compute_derivatives_patch");
+    return done(launch_derivatives_patch(I0, I1, w, h, s, Ix, Iy, Iz, ST), "rf_k_compute_derivatives_patch");
+}
+int rf_k_solve_data_charb_for_tv(const float* du0, const float* dv0, const float* duhat0, const float* dvhat0,
+                                 const float* pu1, const float* pu2, const float* pv1, const float* pv2,
+                                 const float* Ix, const float* Iy, const float* It, const float* Ixx,
+                                 const float* Ixy, const float* Ixt, const float* Iyx, const float* Iyy,
+                                 const float* Iyt, int w, int h, int s, float lambda, float lambdagrad, float theta,
+                                 float* du1, float* dv1, float* duhat1, float* dvhat1, void* stream) {
+    GEOM("rf_k_solve_data_charb_for_tv");
+    const bool any = Ixx || Ixy || Ixt || Iyx || Iyy || Iyt, all = Ixx && Ixy && Ixt && Iyx && Iyy && Iyt;
+    if (any && !all) {
+        set_error("rf_k_solve_data_charb_for_tv: pass all six gradient-image derivatives or none");
+        return RF_ERR_ARG;
+    }
+    return done(launch_data_charb(du0, dv0, duhat0, dvhat0, pu1, pu2, pv1, pv2, Ix, Iy, It, Ixx, Ixy, Ixt, Iyx, Iyy,
+                                  Iyt, w, h, s, lambda, lambdagrad, theta, du1, dv1, duhat1, dvhat1, ST),
+                "rf_k_solve_data_charb_for_tv");
+}
+int rf_k_solve_data_l1_inpaint(const float* duhat0, const float* dvhat0, const float* mask0, const float* mask1,
+                               const float* pu1, const float* pu2, const float* pv1, const float* pv2,
+                               const float* Ix, const float* Iy, const float* It, int w, int h, int s, float lambda,
+                               float theta, float* duhat1, float* dvhat1, void* stream) {
+    GEOM("rf_k_solve_data_l1_inpaint");
+    (void)mask1;
+    return done(launch_data_l1_inpaint(duhat0, dvhat0, mask0, pu1, pu2, pv1, pv2, Ix, Iy, It, w, h, s, lambda, theta,
+                                       duhat1, dvhat1, ST),
+                "rf_k_solve_data_l1_inpaint");
+}
+int rf_k_cv16u_to_32f(const uint16_t* in, float* out, int w, int h, int s, void* stream) {
+    GEOM("rf_k_cv16u_to_32f");
+    return done(launch_cv16u_to_32f(in, out, w, h, s, ST), "rf_k_cv16u_to_32f");
+}
+int rf_k_solve_scene_flow(const float* u, const float* v, const float* depth0, const float* depth1, float fx,
+                          float fy, float cx, float cy, int w, int h, int s, float* pcloud0, float* pcloud1,
+                          float* sceneflow, void* stream) {
+    GEOM("rf_k_solve_scene_flow");
+    return done(launch_scene_flow(u, v, depth0, depth1, fx, fy, cx, cy, w, h, s, pcloud0, pcloud1, sceneflow, ST),
+                "rf_k_solve_scene_flow");
+}
+int rf_k_scene_flow_to_hsv(const float* sceneflow, int w, int h, int s, float* rgb, float flow_scale, void* stream) {
+    GEOM("rf_k_scene_flow_to_hsv");
+    return done(launch_scene_flow_to_hsv(sceneflow, w, h, s, rgb, flow_scale, ST), "rf_k_scene_flow_to_hsv");
+}
+
 }  // extern "C"

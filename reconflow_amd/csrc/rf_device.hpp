@@ -147,6 +147,49 @@ __device__ __forceinline__ void l1_threshold_div(float dix, float diy, float dit
     }
 }
 
+// sqrtf(x*x + 0.001) of the Charbonnier kernels: 0.001 is a double literal there, so the sum is
+// formed in double and rounded to float before the float square root.
+__device__ __forceinline__ float charb_root(float x) { return sqrtf((float)((double)(x * x) + 0.001)); }
+
+// SolveDataCharbForTVKernel (solveDataCharbForTV.cu:3-85; GRAD = false: one IRLS pass) and
+// SolveDataCharbForTVGradKernel (solveDataCharbForTVGrad.cu:3-103; GRAD: two passes with the
+// gradient-constancy terms): the 2x2 reweighted solve for (du, dv) from (du0, dv0).
+template <bool GRAD>
+__device__ __forceinline__ void charb_px(float du, float dv, float dix, float diy, float dit, float dixx, float dixy,
+                                         float dixt, float diyx, float diyy, float diyt, float duhat, float dvhat,
+                                         float lambda, float lambdagrad, float theta, float& odu, float& odv) {
+#pragma unroll
+    for (int k = 0; k < (GRAD ? 2 : 1); ++k) {
+        const float it = 1.0f / theta;
+        const float innerSum = (dix * du + diy * dv) + dit;
+        const float beta = lambda / charb_root(innerSum);
+        float a11, a12, a22, b1, b2;
+        if (GRAD) {
+            const float innerSumx = (dixx * du + dixy * dv) + dixt;
+            const float innerSumy = (diyx * du + diyy * dv) + diyt;
+            const float betax = lambdagrad / charb_root(innerSumx);
+            const float betay = lambdagrad / charb_root(innerSumy);
+            a11 = (((dix * dix) * beta + (dixx * dixx) * betax) + (diyx * diyx) * betay) + it;
+            a12 = ((dix * diy) * beta + (dixx * dixy) * betax) + (diyx * diyy) * betay;
+            a22 = (((diy * diy) * beta + (dixy * dixy) * betax) + (diyy * diyy) * betay) + it;
+            b1 = ((it * duhat - (dix * dit) * beta) - (dixx * dixt) * betax) - (diyx * diyt) * betay;
+            b2 = ((it * dvhat - (diy * dit) * beta) - (dixy * dixt) * betax) - (diyy * diyt) * betay;
+        } else {
+            a11 = (dix * dix) * beta + it;
+            a12 = (dix * diy) * beta;
+            a22 = (diy * diy) * beta + it;
+            b1 = it * duhat - (dix * dit) * beta;
+            b2 = it * dvhat - (diy * dit) * beta;
+        }
+        const float a21 = a12;
+        const float detA = a11 * a22 - a12 * a21;
+        du = (a22 * b1 - a12 * b2) / detA;
+        dv = ((-a21) * b1 + a11 * b2) / detA;
+    }
+    odu = du;
+    odv = dv;
+}
+
 // Joint data blend (solveReconDataL1.cu:35-36): (aTv*uhat + aP*((uproj - u) - sk)) / A.
 __device__ __forceinline__ float recon_blend(float aTv, float aP, float A, float uhat, float uproj, float u, float sk) {
     return (aTv * uhat + aP * ((uproj - u) - sk)) / A;

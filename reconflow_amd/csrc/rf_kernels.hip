@@ -308,6 +308,150 @@ __global__ void __launch_bounds__(256) k_flow_to_hsv(const float* u, const float
     rgb[3 * pos + 2] = col[0];  // R
 }
 
+// ComputeDerivKernel (computeDeriv.cu:6-40): 5-point gradient of one image, mirror border.
+__global__ void __launch_bounds__(256) k_deriv_single(const float* I, int w, int h, int s, float* Ix, float* Iy) {
+    RF_PIX
+    const long r = (long)iy * s;
+    Ix[pos] = d5(I[r + refl(ix - 2, w)], I[r + refl(ix - 1, w)], I[r + refl(ix + 1, w)], I[r + refl(ix + 2, w)]);
+    Iy[pos] = d5(I[ix + (long)refl(iy - 2, h) * s], I[ix + (long)refl(iy - 1, h) * s],
+                 I[ix + (long)refl(iy + 1, h) * s], I[ix + (long)refl(iy + 2, h) * s]);
+}
+
+// ComputeDerivativesPatchKernel (computeDerivativesPatch.cu:8-67): the derivatives of
+// ComputeDerivatives at every tap of the 5x5 patch (each tap's stencil mirror-indexed on its
+// own), summed in row-major tap order.
+__global__ void __launch_bounds__(256) k_derivatives_patch(const float* I0, const float* I1, int w, int h, int s,
+                                                            float* Ix, float* Iy, float* Iz) {
+    RF_PIX
+    auto at = [&](const float* I, int x, int y) { return I[refl(x, w) + (long)refl(y, h) * s]; };
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int col = ix + i - 2, row = iy + j - 2;
+            float t0 = d5(at(I0, col - 2, row), at(I0, col - 1, row), at(I0, col + 1, row), at(I0, col + 2, row));
+            float t1 = d5(at(I1, col - 2, row), at(I1, col - 1, row), at(I1, col + 1, row), at(I1, col + 2, row));
+            sx += (t0 + t1) * 0.5f;
+            sz += at(I1, col, row) - at(I0, col, row);
+            t0 = d5(at(I0, col, row - 2), at(I0, col, row - 1), at(I0, col, row + 1), at(I0, col, row + 2));
+            t1 = d5(at(I1, col, row - 2), at(I1, col, row - 1), at(I1, col, row + 1), at(I1, col, row + 2));
+            sy += (t0 + t1) * 0.5f;
+        }
+    Ix[pos] = sx;
+    Iy[pos] = sy;
+    Iz[pos] = sz;
+}
+
+// SolveDataCharbForTVKernel / SolveDataCharbForTVGradKernel (solveDataCharbForTV.cu:3-85,
+// solveDataCharbForTVGrad.cu:3-103): reweighted solve, then the primal step with the
+// divergence of SolveDataL1's border rule.
+template <bool GRAD>
+__global__ void __launch_bounds__(256) k_data_charb(const float* du0, const float* dv0, const float* duhat0,
+                                                     const float* dvhat0, const float* pu1, const float* pu2,
+                                                     const float* pv1, const float* pv2, const float* Ix,
+                                                     const float* Iy, const float* It, const float* Ixx,
+                                                     const float* Ixy, const float* Ixt, const float* Iyx,
+                                                     const float* Iyy, const float* Iyt, int w, int h, int s,
+                                                     float lambda, float lambdagrad, float theta, float* du1,
+                                                     float* dv1, float* duhat1, float* dvhat1) {
+    RF_PIX
+    float du, dv;
+    if (GRAD)
+        charb_px<true>(du0[pos], dv0[pos], Ix[pos], Iy[pos], It[pos], Ixx[pos], Ixy[pos], Ixt[pos], Iyx[pos],
+                       Iyy[pos], Iyt[pos], duhat0[pos], dvhat0[pos], lambda, lambdagrad, theta, du, dv);
+    else
+        charb_px<false>(du0[pos], dv0[pos], Ix[pos], Iy[pos], It[pos], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
+                        duhat0[pos], dvhat0[pos], lambda, lambdagrad, theta, du, dv);
+    du1[pos] = du;
+    dv1[pos] = dv;
+    bool hl = ix > 0, hu = iy > 0;
+    float divu = div_p(pu1[pos], hl ? pu1[pos - 1] : 0.0f, pu2[pos], hu ? pu2[pos - s] : 0.0f, hl, hu);
+    float divv = div_p(pv1[pos], hl ? pv1[pos - 1] : 0.0f, pv2[pos], hu ? pv2[pos - s] : 0.0f, hl, hu);
+    duhat1[pos] = du + theta * divu;
+    dvhat1[pos] = dv + theta * divv;
+}
+
+// SolveDataL1InpaintKernel (solveDataL1Inpaint.cu:4-98): the L1 step, kept at uhat where
+// mask0 == 0.
+__global__ void __launch_bounds__(256) k_data_l1_inpaint(const float* du0, const float* dv0, const float* mask0,
+                                                          const float* pu1, const float* pu2, const float* pv1,
+                                                          const float* pv2, const float* Ix, const float* Iy,
+                                                          const float* Iz, int w, int h, int s, float lambda,
+                                                          float theta, float* du1, float* dv1) {
+    RF_PIX
+    float du, dv;
+    const float duhat = du0[pos], dvhat = dv0[pos];
+    l1_threshold(Ix[pos], Iy[pos], Iz[pos], duhat, dvhat, lambda, theta, du, dv);
+    if (mask0[pos] == 0.0f) {
+        du = duhat;
+        dv = dvhat;
+    }
+    bool hl = ix > 0, hu = iy > 0;
+    float divu = div_p(pu1[pos], hl ? pu1[pos - 1] : 0.0f, pu2[pos], hu ? pu2[pos - s] : 0.0f, hl, hu);
+    float divv = div_p(pv1[pos], hl ? pv1[pos - 1] : 0.0f, pv2[pos], hu ? pv2[pos - s] : 0.0f, hl, hu);
+    du1[pos] = du + theta * divu;
+    dv1[pos] = dv + theta * divv;
+}
+
+// Cv16uTo32fKernel (rgbToGray.cu:51-64): depth in sensor units, unscaled.
+__global__ void __launch_bounds__(256) k_cv16u_to_32f(const uint16_t* in, float* out, int w, int h, int s) {
+    RF_PIX
+    out[pos] = (float)in[pos];
+}
+
+// SolveSceneFlowAndPointCloudKernel (solveSceneFlow.cu:49-91): depth at the pixel and, bilinear
+// with mirror border, at the flow target; both back-projected; scene flow zeroed outside
+// 500 <= d0 <= 2500. Outputs are (x, y, z) triples per pixel; pc0 / pc1 may be null.
+__global__ void __launch_bounds__(256) k_scene_flow(const float* u, const float* v, const float* depth0,
+                                                     const float* depth1, float fx, float fy, float cx, float cy,
+                                                     int w, int h, int s, float* pc0, float* pc1, float* sf) {
+    RF_PIX
+    const float d0 = depth0[pos];
+    const float d1 = warp_sample(depth1, w, h, s, ix, iy, u[pos], v[pos]);
+    const float p0z = d0, p0x = (((float)ix - cx) * p0z) / fx, p0y = (((float)iy - cy) * p0z) / fy;
+    const float p1z = d1, p1x = (((float)ix - cx) * p1z) / fx, p1y = (((float)iy - cy) * p1z) / fy;
+    float sx = p0x - p1x, sy = p0y - p1y, sz = p0z - p1z;
+    if ((d0 < 500.0f) || (d0 > 2500.0f)) sx = sy = sz = 0.0f;
+    if (pc0) {
+        pc0[3 * pos] = p0x;
+        pc0[3 * pos + 1] = p0y;
+        pc0[3 * pos + 2] = p0z;
+    }
+    if (pc1) {
+        pc1[3 * pos] = p1x;
+        pc1[3 * pos + 1] = p1y;
+        pc1[3 * pos + 2] = p1z;
+    }
+    sf[3 * pos] = sx;
+    sf[3 * pos + 1] = sy;
+    sf[3 * pos + 2] = sz;
+}
+
+// SceneFlowToHSVKernel (flowToHSV.cu:132-190): hue from the (x, y) direction, brightness
+// offset by z; the sector bounds are double literals in the reference, compared in double.
+__global__ void __launch_bounds__(256) k_scene_flow_to_hsv(const float* sf, int w, int h, int s, float* rgb,
+                                                            float flowscale) {
+    RF_PIX
+    const float um = sf[3 * pos] / flowscale, vm = sf[3 * pos + 1] / flowscale, wm = sf[3 * pos + 2] / flowscale;
+    const float H = atan2f(vm, um) + 3.1416f;
+    const float C = sqrtf(um * um + vm * vm);
+    const float Hp = H / 1.0472f;
+    const float X = C * (1.0f - fabsf(fmodf(Hp, 2.0f) - 1.0f));
+    const float m = wm - C;
+    const double Hd = H;
+    float r, g, b;
+    if (Hd < 6.2832 && Hd >= 5.236) r = C, g = 0.0f, b = X;
+    else if (Hd < 5.236 && Hd >= 4.1888) r = X, g = 0.0f, b = C;
+    else if (Hd < 4.1888 && Hd >= 3.1416) r = 0.0f, g = X, b = C;
+    else if (Hd < 3.1416 && Hd >= 2.0944) r = 0.0f, g = C, b = X;
+    else if (Hd < 2.0944 && Hd >= 1.0472) r = X, g = C, b = 0.0f;
+    else r = C, g = X, b = 0.0f;
+    rgb[3 * pos] = r + m;
+    rgb[3 * pos + 1] = g + m;
+    rgb[3 * pos + 2] = b + m;
+}
+
 // solve3dMinimalArea.cu:320-378. The forward difference at ix + range == w (or row h) reads
 // the padding column (or the next row / guard row), as in the reference.
 __global__ void __launch_bounds__(256) k_remove_edge(const float* X0, const float* Y0, const float* Z0, int range,
@@ -499,6 +643,55 @@ hipError_t launch_update_sk(const float* sku, const float* skv, const float* u, 
 hipError_t launch_flow_to_hsv(const float* u, const float* v, int w, int h, int s, float* uvrgb, float flowscale,
                               hipStream_t st) {
     RF_L2(k_flow_to_hsv, u, v, w, h, s, uvrgb, flowscale);
+}
+hipError_t launch_deriv_single(const float* I, int w, int h, int s, float* Ix, float* Iy, hipStream_t st) {
+    hipLaunchKernelGGL(k_deriv_single, grid2(w, h), dim3(BX, BY), 0, st, I, w, h, s, Ix, Iy);
+    return hipGetLastError();
+}
+hipError_t launch_derivatives_patch(const float* I0, const float* I1, int w, int h, int s, float* Ix, float* Iy,
+                                    float* Iz, hipStream_t st) {
+    hipLaunchKernelGGL(k_derivatives_patch, grid2(w, h), dim3(BX, BY), 0, st, I0, I1, w, h, s, Ix, Iy, Iz);
+    return hipGetLastError();
+}
+hipError_t launch_data_charb(const float* du0, const float* dv0, const float* duhat0, const float* dvhat0,
+                             const float* pu1, const float* pu2, const float* pv1, const float* pv2, const float* Ix,
+                             const float* Iy, const float* It, const float* Ixx, const float* Ixy, const float* Ixt,
+                             const float* Iyx, const float* Iyy, const float* Iyt, int w, int h, int s, float lambda,
+                             float lambdagrad, float theta, float* du1, float* dv1, float* duhat1, float* dvhat1,
+                             hipStream_t st) {
+    if (Ixx)
+        hipLaunchKernelGGL(k_data_charb<true>, grid2(w, h), dim3(BX, BY), 0, st, du0, dv0, duhat0, dvhat0, pu1, pu2,
+                           pv1, pv2, Ix, Iy, It, Ixx, Ixy, Ixt, Iyx, Iyy, Iyt, w, h, s, lambda, lambdagrad, theta,
+                           du1, dv1, duhat1, dvhat1);
+    else
+        hipLaunchKernelGGL(k_data_charb<false>, grid2(w, h), dim3(BX, BY), 0, st, du0, dv0, duhat0, dvhat0, pu1,
+                           pu2, pv1, pv2, Ix, Iy, It, Ixx, Ixy, Ixt, Iyx, Iyy, Iyt, w, h, s, lambda, lambdagrad,
+                           theta, du1, dv1, duhat1, dvhat1);
+    return hipGetLastError();
+}
+hipError_t launch_data_l1_inpaint(const float* du0, const float* dv0, const float* mask0, const float* pu1,
+                                  const float* pu2, const float* pv1, const float* pv2, const float* Ix,
+                                  const float* Iy, const float* Iz, int w, int h, int s, float lambda, float theta,
+                                  float* du1, float* dv1, hipStream_t st) {
+    hipLaunchKernelGGL(k_data_l1_inpaint, grid2(w, h), dim3(BX, BY), 0, st, du0, dv0, mask0, pu1, pu2, pv1, pv2, Ix,
+                       Iy, Iz, w, h, s, lambda, theta, du1, dv1);
+    return hipGetLastError();
+}
+hipError_t launch_cv16u_to_32f(const uint16_t* in, float* out, int w, int h, int s, hipStream_t st) {
+    hipLaunchKernelGGL(k_cv16u_to_32f, grid2(w, h), dim3(BX, BY), 0, st, in, out, w, h, s);
+    return hipGetLastError();
+}
+hipError_t launch_scene_flow(const float* u, const float* v, const float* depth0, const float* depth1, float fx,
+                             float fy, float cx, float cy, int w, int h, int s, float* pc0, float* pc1, float* sf,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_scene_flow, grid2(w, h), dim3(BX, BY), 0, st, u, v, depth0, depth1, fx, fy, cx, cy, w, h, s,
+                       pc0, pc1, sf);
+    return hipGetLastError();
+}
+hipError_t launch_scene_flow_to_hsv(const float* sf, int w, int h, int s, float* rgb, float flowscale,
+                                    hipStream_t st) {
+    hipLaunchKernelGGL(k_scene_flow_to_hsv, grid2(w, h), dim3(BX, BY), 0, st, sf, w, h, s, rgb, flowscale);
+    return hipGetLastError();
 }
 hipError_t launch_remove_edge(const float* X0, const float* Y0, const float* Z0, int range, float maxarea, int w,
                               int h, int s, float* X1, float* Y1, float* Z1, hipStream_t st) {

@@ -65,6 +65,27 @@ hipError_t launch_flow_to_hsv(const float* u, const float* v, int w, int h, int 
 hipError_t launch_remove_edge(const float* X0, const float* Y0, const float* Z0, int range, float maxarea, int w,
                               int h, int s, float* X1, float* Y1, float* Z1, hipStream_t st);
 
+hipError_t launch_deriv_single(const float* I, int w, int h, int s, float* Ix, float* Iy, hipStream_t st);
+hipError_t launch_derivatives_patch(const float* I0, const float* I1, int w, int h, int s, float* Ix, float* Iy,
+                                    float* Iz, hipStream_t st);
+// Ixx == nullptr: the Charbonnier form; otherwise the gradient-constancy form.
+hipError_t launch_data_charb(const float* du0, const float* dv0, const float* duhat0, const float* dvhat0,
+                             const float* pu1, const float* pu2, const float* pv1, const float* pv2, const float* Ix,
+                             const float* Iy, const float* It, const float* Ixx, const float* Ixy, const float* Ixt,
+                             const float* Iyx, const float* Iyy, const float* Iyt, int w, int h, int s, float lambda,
+                             float lambdagrad, float theta, float* du1, float* dv1, float* duhat1, float* dvhat1,
+                             hipStream_t st);
+hipError_t launch_data_l1_inpaint(const float* du0, const float* dv0, const float* mask0, const float* pu1,
+                                  const float* pu2, const float* pv1, const float* pv2, const float* Ix,
+                                  const float* Iy, const float* Iz, int w, int h, int s, float lambda, float theta,
+                                  float* du1, float* dv1, hipStream_t st);
+hipError_t launch_cv16u_to_32f(const uint16_t* in, float* out, int w, int h, int s, hipStream_t st);
+hipError_t launch_scene_flow(const float* u, const float* v, const float* depth0, const float* depth1, float fx,
+                             float fy, float cx, float cy, int w, int h, int s, float* pc0, float* pc1, float* sf,
+                             hipStream_t st);
+hipError_t launch_scene_flow_to_hsv(const float* sf, int w, int h, int s, float* rgb, float flowscale,
+                                    hipStream_t st);
+
 PQ make_pq(const double* P, const double* Q);
 PQd make_pqd(const double* P, const double* Q);
 UcParams make_uc(const double* R, const double* t, const double* K1);

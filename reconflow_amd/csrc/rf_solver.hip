@@ -87,6 +87,11 @@ int Solver::initialize(const rf_params& p) {
         set_error("rf_initialize: unknown storage");
         return RF_ERR_ARG;
     }
+    if (p.path == RF_PATH_FUSED && p.method != RF_METHOD_TVL1 && p.method != RF_METHODR_TVL1_MS &&
+        p.method != RF_METHODR_TVL1_MS_FN && p.method != RF_METHODR_TVL1_MS_FNSPARSE) {
+        set_error("rf_initialize: the fused path runs METHOD_TVL1 and the joint methods; use the plain path");
+        return RF_ERR_UNSUPPORTED;
+    }
     if (p.storage != RF_STORAGE_F32 && p.path != RF_PATH_FUSED) {
         set_error("rf_initialize: FP16 storage is an option of the fused path");
         return RF_ERR_UNSUPPORTED;
@@ -133,15 +138,22 @@ int Solver::initialize(const rf_params& p) {
 
     if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return RF_ERR_HIP;
 
-    // count field buffers: 2 per level (image pyramids) + the fixed set + FP16 shadows
-    const int L = g_.L;
-    const int nfixed = 5 + 8 + 7 + 9 + 8 + 6 + 3 /*uvrgb*/ + 4 /*ping-pong extras*/;
-    const long nf = (long)(2 * L + nfixed);
-    const long halfn = (g_.bufn + 1) / 2;                  // floats holding one FP16 field
-    const long nhalf = (long)(2 * L + 3 + 8);              // FP16 shadows: pyramids, derivatives, duals
-    const long inbytes = g_.n * 4;                         // raw input (up to 4 B per pixel)
+    // field buffers: 2 per level (image pyramids), the fixed set, the uvrgb triple, the raw inputs,
+    // and the buffers of the alternate data terms the method uses (Cudaflow.cu:181-266)
+    const int L = g_.L, m = p.method;
+    const bool cg = (m == RF_METHOD_TVCHARBGRAD || m == RF_METHOD_TVCHARBGRAD_FN);
+    const bool charb = cg || m == RF_METHOD_TVCHARB;
+    const bool inpaint = (m == RF_METHOD_TVL1_INPAINTING), scene = (m == RF_SCENEFLOW_KINECT_TVL1);
+    const int nfixed = 43;
+    long nf = (long)(2 * L + nfixed + 3);
+    if (cg) nf += 4L * L + 8;
+    if (charb) nf += 4;
+    if (inpaint) nf += 2L * L + 2;
+    if (scene) nf += 2 + 4 * 3;
+    const long inbytes = g_.n * 4;  // raw input (up to 4 B per pixel)
     const long infl = (inbytes / 4 + 63) / 64 * 64;
-    slab_bytes_ = (size_t)(nf * g_.bufn + nhalf * halfn + 2 * infl) * sizeof(float) + 256 * 64;
+    const long nin = 2 + (scene ? 2 : 0);
+    slab_bytes_ = (size_t)(nf * ((g_.bufn + 63) / 64 * 64) + nin * infl) * sizeof(float) + 256 * 64;
     if (hipMalloc(&slab_, slab_bytes_) != hipSuccess) {
         slab_ = nullptr;
         set_error("rf_initialize: device allocation of " + std::to_string(slab_bytes_) + " bytes failed");
@@ -171,32 +183,52 @@ int Solver::initialize(const rf_params& p) {
                        &f.us,     &f.vs,    &f.X,     &f.Y,      &f.Z,      &f.Xs,   &f.Ys,   &f.Zs,   &f.Xm,
                        &f.Ym,     &f.Zm,    &f.uproj, &f.vproj,  &f.uc,     &f.vc,   &f.sku,  &f.skv,  &f.skus,
                        &f.skvs,   &f.ufn,   &f.vfn,   &f.ufn_l,  &f.vfn_l,  &f.mask, &f.mask_l};
+    static_assert(sizeof fixed / sizeof fixed[0] == 43, "fixed field count");
     for (float** pp : fixed) *pp = take(g_.bufn);
     f.uvrgb = take(3 * g_.bufn);
-    f.uprojs = take(g_.bufn);
-    f.vprojs = take(g_.bufn);
-    f.ucs = take(g_.bufn);
-    f.vcs = take(g_.bufn);
-    for (int l = 0; l < L; ++l) {
-        f.h_I0[l] = take(halfn);
-        f.h_I1[l] = take(halfn);
-    }
-    f.h_Ix = take(halfn);
-    f.h_Iy = take(halfn);
-    f.h_Iz = take(halfn);
-    for (int i = 0; i < 8; ++i) f.h_p[i] = take(halfn);
     f.in0 = take(infl);
     f.in1 = take(infl);
+    if (cg) {
+        for (int l = 0; l < L; ++l) {
+            f.pIx0[l] = take(g_.bufn);
+            f.pIx1[l] = take(g_.bufn);
+            f.pIy0[l] = take(g_.bufn);
+            f.pIy1[l] = take(g_.bufn);
+        }
+        for (float** pp : {&f.ix1warp, &f.iy1warp, &f.Ixx, &f.Ixy, &f.Ixz, &f.Iyx, &f.Iyy, &f.Iyz})
+            *pp = take(g_.bufn);
+    }
+    if (charb)
+        for (float** pp : {&f.du, &f.dv, &f.dus, &f.dvs}) *pp = take(g_.bufn);
+    if (inpaint) {
+        for (int l = 0; l < L; ++l) {
+            f.pM0[l] = take(g_.bufn);
+            f.pM1[l] = take(g_.bufn);
+        }
+        f.mask0 = take(g_.bufn);
+        f.mask1 = take(g_.bufn);
+    }
+    if (scene) {
+        f.depth032f = take(g_.bufn);
+        f.depth132f = take(g_.bufn);
+        f.sceneflow = take(3 * g_.bufn);
+        f.sceneflowrgb = take(3 * g_.bufn);
+        f.pcloud0 = take(3 * g_.bufn);
+        f.pcloud1 = take(3 * g_.bufn);
+        f.depth0_16 = take(infl);
+        f.depth1_16 = take(infl);
+    }
     if ((char*)cursor > (char*)slab_ + slab_bytes_) {
         set_error("rf_initialize: internal slab accounting error");
         release();
         return RF_ERR_STATE;
     }
     init_ = cur_ = f;
-    gfinal_[0] = gfinal_[1] = f;
+    gfinal_[0] = gfinal_[1] = gfinal_[2] = f;
     inited_ = true;
     prior_ = false;
     camera_ = false;
+    masks_ = depth_ = scene_ = false;
     return RF_OK;
 }
 
@@ -333,32 +365,85 @@ hipError_t Solver::enqueue_gray_pyramid(Fields& f) {
             RF_TRY(launch_gray_f32((const float*)f.in0, f.pI0[0], w0, h0, s0, st_));
             RF_TRY(launch_gray_f32((const float*)f.in1, f.pI1[0], w0, h0, s0, st_));
     }
+    const bool cg = f.pIx0[0] != nullptr;
+    if (cg) {  // gradient images of both frames (Cudaflow.cu:426-429)
+        RF_TRY(launch_deriv_single(f.pI0[0], w0, h0, s0, f.pIx0[0], f.pIy0[0], st_));
+        RF_TRY(launch_deriv_single(f.pI1[0], w0, h0, s0, f.pIx1[0], f.pIy1[0], st_));
+    }
+    if (masked_run_) {  // masks (Inpainting.cu:36-37)
+        RF_TRY(launch_gray_f32(f.mask0, f.pM0[0], w0, h0, s0, st_));
+        RF_TRY(launch_gray_f32(f.mask1, f.pM1[0], w0, h0, s0, st_));
+    }
     for (int l = 1; l < g_.L; ++l) {
-        RF_TRY(launch_downscale(f.pI0[l - 1], g_.W[l - 1], g_.H[l - 1], g_.S[l - 1], g_.W[l], g_.H[l], g_.S[l], false,
-                                1.0f, f.pI0[l], st_));
-        RF_TRY(launch_downscale(f.pI1[l - 1], g_.W[l - 1], g_.H[l - 1], g_.S[l - 1], g_.W[l], g_.H[l], g_.S[l], false,
-                                1.0f, f.pI1[l], st_));
+        auto down = [&](float* const* py) {
+            return launch_downscale(py[l - 1], g_.W[l - 1], g_.H[l - 1], g_.S[l - 1], g_.W[l], g_.H[l], g_.S[l], false,
+                                    1.0f, py[l], st_);
+        };
+        RF_TRY(down(f.pI0));
+        RF_TRY(down(f.pI1));
+        if (cg) {
+            RF_TRY(down(f.pIx0));
+            RF_TRY(down(f.pIx1));
+            RF_TRY(down(f.pIy0));
+            RF_TRY(down(f.pIy1));
+        }
+        if (masked_run_) {
+            RF_TRY(down(f.pM0));
+            RF_TRY(down(f.pM1));
+        }
     }
     return hipSuccess;
 }
 
-// CudaFlow::_solveOpticalFlow, METHOD_TVL1 (Cudaflow.cu:403-621).
+// CudaFlow::_solveOpticalFlow (Cudaflow.cu:403-621) with its data-term branches (:489-557), and
+// CudaFlow::_solveInpainting (Inpainting.cu:13-152) when masked_run_ is set.
 hipError_t Solver::enqueue_flow_plain(Fields& f) {
     RF_TRY(enqueue_gray_pyramid(f));
     {
         float* z[] = {f.u, f.v};
         RF_TRY(launch_fill_zero(z, 2, g_.n, st_));
     }
+    const int m = p_.method;
+    const bool cg = (m == RF_METHOD_TVCHARBGRAD || m == RF_METHOD_TVCHARBGRAD_FN);
+    const bool charb = (m == RF_METHOD_TVCHARB);
     for (int level = g_.L - 1; level >= 0; --level) {
         const int w = g_.W[level], h = g_.H[level], s = g_.S[level];
         for (int wi = 0; wi < p_.n_warp_iters; ++wi) {
             float* z[] = {f.dumed, f.dvmed, f.dumeds, f.dvmeds, f.pu1, f.pu2, f.pv1, f.pv2};
             RF_TRY(launch_fill_zero(z, 8, g_.n, st_));
+            if (f.du) {
+                float* zd[] = {f.du, f.dv, f.dus, f.dvs};
+                RF_TRY(launch_fill_zero(zd, 4, g_.n, st_));
+            }
             RF_TRY(launch_warp(f.pI1[level], w, h, s, f.u, f.v, f.i1warp, st_));
-            RF_TRY(launch_derivatives(f.pI0[level], f.i1warp, w, h, s, f.Ix, f.Iy, f.Iz, st_));
+            if (cg) {
+                RF_TRY(launch_warp(f.pIx1[level], w, h, s, f.u, f.v, f.ix1warp, st_));
+                RF_TRY(launch_warp(f.pIy1[level], w, h, s, f.u, f.v, f.iy1warp, st_));
+            }
+            if (m == RF_METHOD_TVL1PATCH && !masked_run_)
+                RF_TRY(launch_derivatives_patch(f.pI0[level], f.i1warp, w, h, s, f.Ix, f.Iy, f.Iz, st_));
+            else
+                RF_TRY(launch_derivatives(f.pI0[level], f.i1warp, w, h, s, f.Ix, f.Iy, f.Iz, st_));
+            if (cg) {
+                RF_TRY(launch_derivatives(f.pIx0[level], f.ix1warp, w, h, s, f.Ixx, f.Ixy, f.Ixz, st_));
+                RF_TRY(launch_derivatives(f.pIy0[level], f.iy1warp, w, h, s, f.Iyx, f.Iyy, f.Iyz, st_));
+            }
             for (int it = 0; it < p_.n_solver_iters; ++it) {
-                RF_PROF(RF_KID_DATA, launch_data_l1(f.dumed, f.dvmed, f.pu1, f.pu2, f.pv1, f.pv2, f.Ix, f.Iy, f.Iz, w,
-                                                    h, s, p_.lambda, p_.theta, f.dumeds, f.dvmeds, st_));
+                if (masked_run_) {
+                    RF_PROF(RF_KID_DATA, launch_data_l1_inpaint(f.dumed, f.dvmed, f.pM0[level], f.pu1, f.pu2, f.pv1,
+                                                                f.pv2, f.Ix, f.Iy, f.Iz, w, h, s, p_.lambda,
+                                                                p_.theta, f.dumeds, f.dvmeds, st_));
+                } else if (cg || charb) {
+                    RF_PROF(RF_KID_DATA, launch_data_charb(f.du, f.dv, f.dumed, f.dvmed, f.pu1, f.pu2, f.pv1, f.pv2,
+                                                           f.Ix, f.Iy, f.Iz, cg ? f.Ixx : nullptr, f.Ixy, f.Ixz,
+                                                           f.Iyx, f.Iyy, f.Iyz, w, h, s, p_.lambda, p_.lambdagrad,
+                                                           p_.theta, f.dus, f.dvs, f.dumeds, f.dvmeds, st_));
+                    std::swap(f.du, f.dus);
+                    std::swap(f.dv, f.dvs);
+                } else {
+                    RF_PROF(RF_KID_DATA, launch_data_l1(f.dumed, f.dvmed, f.pu1, f.pu2, f.pv1, f.pv2, f.Ix, f.Iy,
+                                                        f.Iz, w, h, s, p_.lambda, p_.theta, f.dumeds, f.dvmeds, st_));
+                }
                 std::swap(f.dumed, f.dumeds);
                 std::swap(f.dvmed, f.dvmeds);
                 RF_PROF(RF_KID_DUAL, launch_dual_tv(f.dumed, f.dvmed, f.pu1, f.pu2, f.pv1, f.pv2, w, h, s, p_.tau,
@@ -545,9 +630,11 @@ void Solver::result(const float** u, const float** v, const float** X, const flo
 
 // Runs one solve. The pointer table restarts from its initial assignment every solve, so the
 // schedule (and a graph captured from it) is identical from solve to solve.
-int Solver::run_schedule(bool recon) {
-    const int kind = recon ? 1 : 0;
-    auto enqueue = [&](Fields& f) -> hipError_t { return recon ? enqueue_recon_plain(f) : enqueue_flow_plain(f); };
+int Solver::run_schedule(int kind) {
+    auto enqueue = [&](Fields& f) -> hipError_t {
+        masked_run_ = (kind == 2);
+        return kind == 1 ? enqueue_recon_plain(f) : enqueue_flow_plain(f);
+    };
     if (!p_.use_graph || prof_.armed) {
         Fields f = init_;
         if (check(enqueue(f), "solve")) return RF_ERR_HIP;
@@ -583,8 +670,8 @@ int Solver::solve_flow(float flow_scale) {
         set_error("rf_solve_flow: not initialized");
         return RF_ERR_STATE;
     }
-    if (p_.method != RF_METHOD_TVL1) {
-        set_error("rf_solve_flow: only METHOD_TVL1 is implemented");
+    if (p_.method < RF_METHOD_TVL1 || p_.method > RF_METHOD_TVL1_INPAINTING) {
+        set_error("rf_solve_flow: not a CudaFlow method");
         return RF_ERR_UNSUPPORTED;
     }
     if (batch_) return batch_->solve(false, flow_scale);
@@ -593,7 +680,7 @@ int Solver::solve_flow(float flow_scale) {
         gexec_[0] = nullptr;
     }
     flow_scale_ = flow_scale;
-    return run_schedule(false);
+    return run_schedule(0);
 }
 
 int Solver::solve_recon(const double* R0, const double* t0, const double* R1, const double* t1, float flow_scale) {
@@ -659,7 +746,7 @@ int Solver::solve_recon(const double* R0, const double* t0, const double* R1, co
         gexec_[1] = nullptr;
     }
     flow_scale_ = flow_scale;
-    return run_schedule(true);
+    return run_schedule(1);
 }
 
 int Solver::sync() { return check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
@@ -680,6 +767,104 @@ int Solver::copy_flow(float* u, float* v, float* uvrgb) {
     if (uvrgb && check(hipMemcpyAsync(uvrgb, drgb, bytes * 3, hipMemcpyDeviceToHost, st), "copy uvrgb"))
         return RF_ERR_HIP;
     return check(hipStreamSynchronize(st), "copy flow");
+}
+
+// CudaFlow::copyMaskToDevice (Inpainting.cu:3-7).
+int Solver::copy_masks(const float* m0, const float* m1) {
+    if (!inited_ || !m0 || !m1) {
+        set_error("rf_copy_masks: not initialized or null mask");
+        return inited_ ? RF_ERR_ARG : RF_ERR_STATE;
+    }
+    if (!init_.mask0) {
+        set_error("rf_copy_masks: the context was not initialized with METHOD_TVL1_INPAINTING");
+        return RF_ERR_STATE;
+    }
+    size_t bytes = (size_t)g_.n * 4;
+    if (check(hipMemcpyAsync(init_.mask0, m0, bytes, hipMemcpyHostToDevice, st_), "copy mask0") ||
+        check(hipMemcpyAsync(init_.mask1, m1, bytes, hipMemcpyHostToDevice, st_), "copy mask1"))
+        return RF_ERR_HIP;
+    masks_ = true;
+    return check(hipStreamSynchronize(st_), "copy masks");
+}
+
+// CudaFlow::copyDepthToDevice (Cudaflow.cu:282-286).
+int Solver::copy_depth(const uint16_t* d0, const uint16_t* d1) {
+    if (!inited_ || !d0 || !d1) {
+        set_error("rf_copy_depth: not initialized or null depth");
+        return inited_ ? RF_ERR_ARG : RF_ERR_STATE;
+    }
+    if (!init_.depth0_16) {
+        set_error("rf_copy_depth: the context was not initialized with SCENEFLOW_KINECT_TVL1");
+        return RF_ERR_STATE;
+    }
+    size_t bytes = (size_t)g_.n * 2;
+    if (check(hipMemcpyAsync(init_.depth0_16, d0, bytes, hipMemcpyHostToDevice, st_), "copy depth0") ||
+        check(hipMemcpyAsync(init_.depth1_16, d1, bytes, hipMemcpyHostToDevice, st_), "copy depth1"))
+        return RF_ERR_HIP;
+    depth_ = true;
+    return check(hipStreamSynchronize(st_), "copy depth");
+}
+
+// CudaFlow::solveInpainting (Inpainting.cu:9-152).
+int Solver::solve_inpainting(float flow_scale) {
+    if (!inited_ || !init_.mask0) {
+        set_error("rf_solve_inpainting: needs a context initialized with METHOD_TVL1_INPAINTING");
+        return RF_ERR_STATE;
+    }
+    if (!masks_) {
+        set_error("rf_solve_inpainting: call rf_copy_masks first");
+        return RF_ERR_STATE;
+    }
+    if (p_.use_graph && p_.with_visualization && flow_scale != flow_scale_ && gexec_[2]) {
+        hipGraphExecDestroy(gexec_[2]);
+        gexec_[2] = nullptr;
+    }
+    flow_scale_ = flow_scale;
+    return run_schedule(2);
+}
+
+// CudaFlow::_solveSceneFlow / _solveSceneFlowAndPointCloud (Cudaflow.cu:836-871): the flow solve,
+// then depth0 converted to float and passed as both depth maps (Cudaflow.cu:843, 865).
+int Solver::solve_scene_flow(float fx, float fy, float cx, float cy, float flow_scale, float sf_scale,
+                             bool point_cloud) {
+    if (!inited_ || !init_.depth0_16) {
+        set_error("rf_solve_scene_flow: needs a context initialized with SCENEFLOW_KINECT_TVL1");
+        return RF_ERR_STATE;
+    }
+    if (!depth_) {
+        set_error("rf_solve_scene_flow: call rf_copy_depth first");
+        return RF_ERR_STATE;
+    }
+    int rc = solve_flow(flow_scale);
+    if (rc) return rc;
+    const int w = g_.W[0], h = g_.H[0], s = g_.S[0];
+    const Fields& f = init_;
+    if (check(launch_cv16u_to_32f((const uint16_t*)f.depth0_16, f.depth032f, w, h, s, st_), "depth0 to float") ||
+        check(launch_cv16u_to_32f((const uint16_t*)f.depth1_16, f.depth132f, w, h, s, st_), "depth1 to float") ||
+        check(launch_scene_flow(cur_.u, cur_.v, f.depth032f, f.depth032f, fx, fy, cx, cy, w, h, s,
+                                point_cloud ? f.pcloud0 : nullptr, point_cloud ? f.pcloud1 : nullptr, f.sceneflow,
+                                st_),
+              "scene flow") ||
+        check(launch_scene_flow_to_hsv(f.sceneflow, w, h, s, f.sceneflowrgb, sf_scale, st_), "scene flow colour"))
+        return RF_ERR_HIP;
+    scene_ = true;
+    return RF_OK;
+}
+
+// CudaFlow::copySceneFlowToHost / copySceneFlowAndPointCloudToHost (Cudaflow.cu:345-362).
+int Solver::copy_scene_flow(float* sf, float* sfrgb, float* pc0, float* pc1) {
+    if (!inited_ || !scene_) {
+        set_error("rf_copy_scene_flow: no scene flow solved");
+        return RF_ERR_STATE;
+    }
+    const size_t bytes = (size_t)g_.n * 12;
+    const Fields& f = init_;
+    const float* src[4] = {f.sceneflow, f.sceneflowrgb, f.pcloud0, f.pcloud1};
+    float* dst[4] = {sf, sfrgb, pc0, pc1};
+    for (int i = 0; i < 4; ++i)
+        if (dst[i] && check(hipMemcpyAsync(dst[i], src[i], bytes, hipMemcpyDeviceToHost, st_), "copy scene flow"))
+            return RF_ERR_HIP;
+    return check(hipStreamSynchronize(st_), "copy scene flow");
 }
 
 // ReconFlow::copy3dToHost (Reconflow.cu:65-70).
@@ -862,6 +1047,32 @@ int rf_copy_sparse_flow(rf_context* ctx, const float* u, const float* v, const f
 int rf_solve_flow(rf_context* ctx, float flow_scale) {
     RF_CTX(ctx);
     return ctx->s->solve_flow(flow_scale);
+}
+
+int rf_copy_masks(rf_context* ctx, const float* mask0, const float* mask1) {
+    RF_CTX(ctx);
+    return ctx->s->copy_masks(mask0, mask1);
+}
+
+int rf_copy_depth(rf_context* ctx, const uint16_t* depth0, const uint16_t* depth1) {
+    RF_CTX(ctx);
+    return ctx->s->copy_depth(depth0, depth1);
+}
+
+int rf_solve_inpainting(rf_context* ctx, float flow_scale) {
+    RF_CTX(ctx);
+    return ctx->s->solve_inpainting(flow_scale);
+}
+
+int rf_solve_scene_flow(rf_context* ctx, float fx, float fy, float cx, float cy, float flow_scale,
+                        float scene_flow_scale, int point_cloud) {
+    RF_CTX(ctx);
+    return ctx->s->solve_scene_flow(fx, fy, cx, cy, flow_scale, scene_flow_scale, point_cloud != 0);
+}
+
+int rf_copy_scene_flow(rf_context* ctx, float* sceneflow, float* sceneflow_rgb, float* pcloud0, float* pcloud1) {
+    RF_CTX(ctx);
+    return ctx->s->copy_scene_flow(sceneflow, sceneflow_rgb, pcloud0, pcloud1);
 }
 
 int rf_solve_recon(rf_context* ctx, const double* R0, const double* t0, const double* R1, const double* t1,

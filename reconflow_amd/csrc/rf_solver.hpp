@@ -37,11 +37,21 @@ struct Fields {
     float *uproj, *vproj, *uc, *vc, *sku, *skv, *skus, *skvs;
     float *ufn, *vfn, *ufn_l, *vfn_l, *mask, *mask_l;
     float* uvrgb;
-    // fused-path extras: ping-pong partners and FP16 shadows
-    float *uprojs, *vprojs, *ucs, *vcs;
-    void *h_I0[kMaxLevels], *h_I1[kMaxLevels];
-    void *h_Ix, *h_Iy, *h_Iz, *h_p[8];
     void *in0, *in1;
+    // alternate data terms (allocated for the methods that use them): gradient-image pyramids and
+    // their warped / differentiated fields (Charbonnier-gradient), the IRLS iterate du, dv
+    // (Charbonnier), mask pyramids (masked TV-L1), depth and scene-flow fields (Kinect scene flow)
+    float* pIx0[kMaxLevels];
+    float* pIx1[kMaxLevels];
+    float* pIy0[kMaxLevels];
+    float* pIy1[kMaxLevels];
+    float* pM0[kMaxLevels];
+    float* pM1[kMaxLevels];
+    float *ix1warp, *iy1warp, *Ixx, *Ixy, *Ixz, *Iyx, *Iyy, *Iyz;
+    float *du, *dv, *dus, *dvs;
+    float *mask0, *mask1;
+    void *depth0_16, *depth1_16;
+    float *depth032f, *depth132f, *sceneflow, *sceneflowrgb, *pcloud0, *pcloud1;
 };
 
 // HIP-event timing of one kernel kind at one level (eager solves only).
@@ -87,7 +97,12 @@ class Solver {
     int copy_images(const void* i0, const void* i1);
     int copy_images_device(const float* i0, const float* i1);
     int copy_prior(const float* u, const float* v, const float* mask, bool sparse);
+    int copy_masks(const float* m0, const float* m1);
+    int copy_depth(const uint16_t* d0, const uint16_t* d1);
     int solve_flow(float flow_scale);
+    int solve_inpainting(float flow_scale);
+    int solve_scene_flow(float fx, float fy, float cx, float cy, float flow_scale, float sf_scale, bool point_cloud);
+    int copy_scene_flow(float* sf, float* sfrgb, float* pc0, float* pc1);
     int solve_recon(const double* R0, const double* t0, const double* R1, const double* t1, float flow_scale);
     int copy_flow(float* u, float* v, float* uvrgb);
     int copy_3d(float* X, float* Y, float* Z);
@@ -105,7 +120,7 @@ class Solver {
 
   private:
     int release();
-    int run_schedule(bool recon);
+    int run_schedule(int kind);  // 0: flow, 1: joint, 2: masked flow
     hipError_t enqueue_flow_plain(Fields& f);
     hipError_t enqueue_recon_plain(Fields& f);
     hipError_t enqueue_gray_pyramid(Fields& f);
@@ -114,6 +129,7 @@ class Solver {
     int bw_, bh_, align_;
     rf_params p_{};
     bool inited_ = false, prior_ = false, sparse_ = false, camera_ = false;
+    bool masks_ = false, depth_ = false, scene_ = false, masked_run_ = false;
     Geometry g_;
     hipStream_t st_ = nullptr;
     void* slab_ = nullptr;
@@ -124,9 +140,9 @@ class Solver {
     std::vector<std::vector<double>> K0_, K1_;
     double R0_[9], t0_[3], R1_[9], t1_[3];
     float flow_scale_ = 5.0f;
-    // graph cache: one per schedule kind (flow / recon)
-    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
-    Fields gfinal_[2];
+    // graph cache: one per schedule kind (flow / joint / masked flow)
+    hipGraphExec_t gexec_[3] = {nullptr, nullptr, nullptr};
+    Fields gfinal_[3];
     Profiler prof_;
     std::unique_ptr<Batch> batch_;  // the fused path
     double K0raw_[9], K1raw_[9];

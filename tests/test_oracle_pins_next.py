@@ -181,3 +181,48 @@ def test_scene_flow_back_projection(orc):
                                           500.0])
     np.testing.assert_allclose(sf[0, 0], [0, 0, 0])
     np.testing.assert_allclose(sf[2, 5], [0, 0, 0])
+
+
+def test_scene_flow_to_hsv_is_the_hsv_sector_map(orc, rng):
+    # hue H = atan2(y, x) + 3.1416 in [0, 2pi), sectors of 1.0472; C = |(x, y)|, m = z - C; the
+    # textbook HSV-to-RGB sector table (C, X, 0), (X, C, 0), (0, C, X), (0, X, C), (X, 0, C), (C, 0, X)
+    n = 64
+    ang = rng.uniform(-3.1, 3.1, n)
+    mag = rng.uniform(0.1, 2.0, n)
+    z = rng.uniform(-1, 1, n)
+    sf = np.stack([mag * np.cos(ang), mag * np.sin(ang), z], -1).ravel()
+    rgb = np.zeros(3 * n)
+    orc.call("scene_flow_to_hsv", "f64", sf, n, 1, n, rgb, 1.0)
+    got = rgb.reshape(n, 3)
+    for k in range(n):
+        x, y, zz = sf[3 * k:3 * k + 3]
+        Hh = np.arctan2(y, x) + float(np.float32(3.1416))
+        C = np.hypot(x, y)
+        hp = Hh / float(np.float32(1.0472))
+        X = C * (1 - abs(hp % 2 - 1))
+        sector = min(int(Hh // 1.0472), 5)
+        table = [(C, X, 0), (X, C, 0), (0, C, X), (0, X, C), (X, 0, C), (C, 0, X)]
+        if abs(Hh - round(Hh / 1.0472) * 1.0472) < 1e-3:
+            continue  # the reference's sector bounds are rounded literals; skip angles next to them
+        np.testing.assert_allclose(got[k], np.array(table[sector]) + (zz - C), atol=1e-12)
+
+
+def test_flow_methods_dispatch(orc):
+    # methods without a data-term branch of their own run the L1 step (Cudaflow.cu:540-557), and
+    # the masked solve with an all-ones mask is the plain solve
+    import reconflow_amd as rf
+    i0, i1, _, _ = rf.make_flow_pair(64, 48, 3)
+    base = orc.solve_flow(orc.flow_params(64, 48, 2, n_warp=1, n_iter=5), i0, i1, "f64")
+    for m in (4, 5, 6, 8):
+        got = orc.solve_flow(orc.flow_params(64, 48, 2, n_warp=1, n_iter=5, method=m), i0, i1, "f64")
+        assert all(np.array_equal(a, b) for a, b in zip(got, base)), m
+    ones = np.ones_like(i0)
+    got = orc.solve_flow_masked(orc.flow_params(64, 48, 2, n_warp=1, n_iter=5, method=8), i0, i1, ones, "f64")
+    assert all(np.array_equal(a, b) for a, b in zip(got, base))
+    zeros = np.zeros_like(i0)
+    got = orc.solve_flow_masked(orc.flow_params(64, 48, 2, n_warp=1, n_iter=5, method=8), i0, i1, zeros, "f64")
+    assert not np.any(got[0]) and not np.any(got[1])  # no data term anywhere: the flow stays 0
+    # the Charbonnier and patch solves move the flow (they are not the L1 solve)
+    for m in (1, 2, 3, 7):
+        got = orc.solve_flow(orc.flow_params(64, 48, 2, n_warp=1, n_iter=5, method=m), i0, i1, "f64")
+        assert not np.array_equal(got[0], base[0]), m
