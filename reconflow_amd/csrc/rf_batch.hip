@@ -261,20 +261,24 @@ __global__ void __launch_bounds__(256) kb_upscale(Launch L, int l, int pu, int p
 
 __global__ void __launch_bounds__(256) kb_hsv(Launch L, int pu, float flowscale);
 
-// The fields one joint iteration reads (state n-1) and writes (state n) on one pyramid level,
-// resolved from the pair descriptor (the compiler loads only what is used).
+// The fields one joint iteration reads (state n-1) and writes (state n) on one pyramid level:
+// built on the host for a single pair and passed by value, or resolved in the kernel from the
+// pair descriptor for a batch.
 struct JointView {
     const float *du0, *dv0, *p1u0, *p2u0, *p1v0, *p2v0, *U, *V, *sku0, *skv0, *up0, *vp0, *uc0, *vc0, *X0, *Y0, *Z0;
     const float *Ix, *Iy, *Iz, *ufn, *vfn, *mask;
     float *du1, *dv1, *p1u1, *p2u1, *p1v1, *p2v1, *sku1, *skv1, *up1, *vp1, *uc1, *vc1, *X1, *Y1, *Z1;
     int w, h, s, tiles_x;
     float inv_tiles_x;  // RN(1/tiles_x): tile row of a tile index without an integer division
-    PQ pq;
-    PQd pqd;
-    UcParams ucp;
+    // camera data stay in the device descriptor, so a captured graph sees pose updates
+    const PQ* pq;
+    const PQd* pqd;
+    const UcParams* ucp;
 };
 
-__device__ __forceinline__ JointView joint_view(const PairFields& F, int l, int pd, int pf, int px, int pu, int tw) {
+// F: the pair's descriptor as the caller sees it; dF: the same descriptor's device address.
+__host__ __device__ inline JointView joint_view(const PairFields& F, const PairFields* dF, int l, int pd, int pf,
+                                                int px, int pu, int tw) {
     JointView v;
     v.du0 = F.du[pd], v.dv0 = F.dv[pd], v.p1u0 = F.p1u[pd], v.p2u0 = F.p2u[pd], v.p1v0 = F.p1v[pd],
     v.p2v0 = F.p2v[pd];
@@ -290,9 +294,9 @@ __device__ __forceinline__ JointView joint_view(const PairFields& F, int l, int 
     v.w = F.W[l], v.h = F.H[l], v.s = F.S[l];
     v.tiles_x = (v.w + tw - 1) / tw;
     v.inv_tiles_x = 1.0f / (float)v.tiles_x;
-    v.pq = F.pq[l];
-    v.pqd = F.pqd[l];
-    v.ucp = F.ucp[l];
+    v.pq = &dF->pq[l];
+    v.pqd = &dF->pqd[l];
+    v.ucp = &dF->ucp[l];
     return v;
 }
 
@@ -301,6 +305,7 @@ struct IterArgs {
     int level, pd, pf, px, pu, first;
     float lambda, theta, tau, aTv, aP, aFn, lf, lms;
     int sparse;
+    JointView v;  // one pair's fields on this level, by value (one-pixel kernels, SINGLE)
 };
 
 // Tile index -> (column, row) of tiles for small indices (< 2^24) with a float reciprocal of the
@@ -512,7 +517,8 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
 //   interior lanes: dual step -> p_n, 3x3 XYZ median of X'_n, cleanup -> X_n, reprojection ->
 //   uc_n, Sk update, stores.
 // Same per-pixel arithmetic as kb_joint_iter, so the results are identical. SINGLE: the launch
-// covers one pair whose tiles are the dispatch indices (no tile list read).
+// covers one pair whose tiles are the dispatch indices (no tile list read) and whose field
+// table is in the kernel arguments.
 // OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
 // schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
 // the FP64 triangulation, met without spilling, for occupancy on large levels).
@@ -522,7 +528,8 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
     const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
     const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const JointView V = joint_view(a.L.P[pt.x], a.level, a.pd, a.pf, a.px, a.pu, TW);
+    const JointView V =
+        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW);
     const int w = V.w, h = V.h, s = V.s;
     const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
     const int x0 = txy.x * TW, y0 = txy.y * TH;
@@ -590,10 +597,10 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     // ---- triangulation
     float oX, oY, oZ;
     if (F64TRI)
-        solve3d_px<double>(V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd,
+        solve3d_px<double>(*V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd,
                            Xl, Yl, Zl, Xu, Yu, Zu, hl, hu, oX, oY, oZ);
     else
-        solve3d_px<float>(V.pq, a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
+        solve3d_px<float>(*V.pq, a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
                           Yu, Zu, hl, hu, oX, oY, oZ);
     s_du[e] = nduh;
     s_dv[e] = ndvh;
@@ -644,7 +651,7 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     float Xn, Yn, Zn;
     cleanup_px(oX, oY, oZ, xm, ym, zm, Xn, Yn, Zn);
     float ucn, vcn;
-    uc_px(V.ucp, Xn, Yn, Zn, x, y, ucn, vcn);
+    uc_px(*V.ucp, Xn, Yn, Zn, x, y, ucn, vcn);
     el(V.X1, q) = Xn;
     el(V.Y1, q) = Yn;
     el(V.Z1, q) = Zn;
@@ -1212,7 +1219,10 @@ hipError_t Batch::enqueue_t(bool recon) {
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
-                           (fn && sparse_) ? 1 : 0};
+                           (fn && sparse_) ? 1 : 0, JointView{}};
+                // one pair: its fields travel in the kernel arguments (no dependent descriptor load)
+                if (recon && n() == 1 && cls != TC_J64)
+                    a.v = joint_view(hp_[0], dp_, level, pr.d, pr.f, pr.x, pr.u, kTileW[cls]);
                 if (ts.ntiles > 0) {
                     if (!recon)
                         hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), dim3(ts.ntiles), dim3(TXW, TRW), 0, st_, a);

@@ -176,6 +176,19 @@ def test_recon_joint_tilings_equal_oracle(orc, tile, tri):
         assert _same(_run_recon(r, d), ref, 200), rep
 
 
+@pytest.mark.parametrize("path", [rf.PATH_PLAIN, rf.PATH_FUSED], ids=["plain-graph", "fused-graph"])
+def test_recon_graph_replay_follows_pose_changes(orc, path):
+    # the captured schedule reads the per-level camera matrices from device memory: a new pose
+    # between replays must give the new pose's result
+    print(f"case: joint 160x120 graph replay with two poses path={path}", flush=True)
+    a, b = rf.make_recon_pair(160, 120, 21), rf.make_recon_pair(160, 120, 22)
+    r = _recon(160, 120, 3, path=path, graph=True, tri=1, nw=2, ni=6)
+    for d in (a, b, a):
+        got = _run_recon(r, d)
+        ref = _oracle_recon(orc, d, 160, 120, 3, "f32", tri=1, nw=2, ni=6)
+        assert _same(got, ref, 160)
+
+
 def test_rgb_input_and_prior_flow(orc):
     print("case: joint 8UC3 input + sparse prior", flush=True)
     w, h, L = 160, 120, 3
