@@ -13,6 +13,8 @@ launch over its average launch time at level 0, timed with HIP events on the sol
 the FP64 CPU oracle's time on the same frame (the reported CPU baseline), the mean EPE of the
 GPU flow against that FP64 result, and the per-step (plain) GPU path's frames/s beside it.
 
+--workload tvl1-320: the 320x240 TV-L1 case (fused, fused FP16, plain; EPE and time of the FP64
+oracle). --workload methods: each CudaFlow data term at 320x240 on the per-step path.
 --workload batch64: 64 seeded pairs of uneven sizes (W in 160..1280, H in 120..720, multiples of
 16), even seeds TV-L1 (4 levels, 3 warps, 30 iterations; FP16 images, derivatives and duals),
 odd seeds joint (as above; FP16 derivatives, FP32 images); the two batches run on their own
@@ -21,6 +23,7 @@ streams.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -46,6 +49,7 @@ FLOW_ITER_BYTES = 60.0  # one fused TV-L1 iteration: I(3) read; uhat(2), p(4) re
 KID = {"fused_iter": 0, "data": 1, "dual": 2, "uvproj": 3, "solve3d": 4, "median3d": 5, "cleanup": 6,
        "solve_uc": 7, "update_sk": 8}
 TILING = {0: "auto", 1: "64x8-rpt2", 2: "32x16", 3: "16x16", 4: "16x16-occ5"}
+FLOW_TILING = {0: "auto", 1: "64x16-rpt4", 2: "16x16", 3: "32x16"}
 
 
 def make_solver(rf, path, graph, tri, joint_tile=0):
@@ -260,16 +264,108 @@ def run_batch(args, rf, torch):
     }
 
 
+def _time(fn, sync, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_tvl1(args, rf, torch):
+    """TV-L1 at 320x240 (4 levels, 3 warps, 30 iterations, lambda 50, theta 0.3, tau 0.25):
+    fused graph replays, the plain path beside it, EPE and time of the FP64 oracle."""
+    w, h, L, nw, ni = 320, 240, 4, 3, 30
+    i0, i1, _, _ = rf.make_flow_pair(w, h, args.seed)
+
+    def solver(path, graph, storage=rf.STORAGE_F32):
+        f = rf.CudaFlow()
+        f.initialize(w, h, 1, rf.INPUT_32F, L, 2.0, rf.CudaFlow.METHOD_TVL1, 50.0, 0.0, 0.3, 0.25, nw, ni, False,
+                     path=path, use_graph=graph, storage=storage)
+        f.copyImagesToDevice(i0, i1)
+        return f
+
+    out, tiling = {}, {}
+    for name, path, storage in (("fused", rf.PATH_FUSED, rf.STORAGE_F32), ("fused_fp16", rf.PATH_FUSED,
+                                                                          rf.STORAGE_F16),
+                                ("plain", rf.PATH_PLAIN, rf.STORAGE_F32)):
+        f = solver(path, not args.no_graph, storage)
+        dt = _time(lambda: f.solveOpticalFlow(5.0), f.synchronize, args.steps, args.warmup)
+        out[name] = (dt, f.copyOpticalFlowToHost())
+        if path == rf.PATH_FUSED:
+            tl = (ctypes.c_int * L)()
+            assert f._lib.rf_flow_tiling(f._ctx, tl, L) == 0
+            tiling[name] = [FLOW_TILING[t] for t in tl]
+    from oracle import oracle as orc
+    tc = time.perf_counter()
+    ref = orc.solve_flow(orc.flow_params(w, h, L), i0, i1, "f64")
+    tcpu = time.perf_counter() - tc
+    epe = {k: float(np.hypot(v[1][0][:, :w] - ref[0][:, :w], v[1][1][:, :w] - ref[1][:, :w]).mean())
+           for k, v in out.items()}
+    px_iter = sum((w >> l) * (h >> l) for l in range(L)) * nw * ni
+    fps = 1.0 / out["fused"][0]
+    return {
+        "metric": "320x240 TV-L1 frames/s (4 levels x 3 warps x 30 iterations)", "value": round(fps, 2),
+        "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(out["fused"][0] * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded affine-flow pair)",
+        "config": {"workload": "TV-L1 320x240", "graph": not args.no_graph},
+        "frames_per_s": {k: round(1.0 / v[0], 2) for k, v in out.items()},
+        "mpix_iter_per_s": round(fps * px_iter / 1e6, 1),
+        "epe_vs_fp64_oracle": {k: round(v, 6) for k, v in epe.items()},
+        "flow_tiling_per_level": tiling,
+        "cpu_baseline": {"value": round(1.0 / tcpu, 3), "unit": "frames/s", "cores": orc.get_threads(),
+                         "kind": "port", "sample": f"one 320x240 TV-L1 frame of the FP64 oracle, {tcpu:.3f} s"},
+    }
+
+
+def run_methods(args, rf, torch):
+    """Each CudaFlow data term at 320x240 (4 levels, 3 warps, 30 iterations) on the per-step
+    path, graph-replayed: frames/s per method."""
+    w, h, L, nw, ni = 320, 240, 4, 3, 30
+    i0, i1, _, _ = rf.make_flow_pair(w, h, args.seed)
+    mask = (np.random.default_rng(1).random(i0.shape) > 0.2).astype(np.float32)
+    depth = np.random.default_rng(2).integers(500, 2500, i0.shape).astype(np.uint16)
+    res = {}
+    for name, method in (("tvl1", 0), ("charbonnier", 1), ("charbonnier_gradient", 2), ("patch_derivatives", 3),
+                         ("masked_tvl1", 8), ("scene_flow", 4)):
+        f = rf.CudaFlow()
+        f.initialize(w, h, 1, rf.INPUT_32F, L, 2.0, method, 50.0, 20.0, 0.3, 0.25, nw, ni, False,
+                     path=rf.PATH_PLAIN, use_graph=not args.no_graph)
+        f.copyImagesToDevice(i0, i1)
+        if method == 8:
+            f.copyMaskToDevice(mask, mask)
+            fn = lambda f=f: f.solveInpainting(5.0)  # noqa: E731
+        elif method == 4:
+            f.copyDepthToDevice(depth, depth)
+            fn = lambda f=f: f.solveSceneFlowAndPointCloud(500.0, 500.0, 160.0, 120.0, 5.0, 1.0)  # noqa: E731
+        else:
+            fn = lambda f=f: f.solveOpticalFlow(5.0)  # noqa: E731
+        dt = _time(fn, f.synchronize, args.steps, args.warmup)
+        res[name] = round(1.0 / dt, 2)
+    return {
+        "metric": "320x240 CudaFlow frames/s per data term (per-step path)", "value": res["tvl1"],
+        "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 / res["tvl1"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded affine-flow pair, random masks/depth)",
+        "config": {"workload": "CudaFlow methods 320x240, 4 levels x 3 warps x 30 iterations"},
+        "frames_per_s": res, "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["joint640", "batch64"], default="joint640")
+    ap.add_argument("--workload", choices=["joint640", "batch64", "tvl1-320", "methods"], default="joint640")
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
-    ap.add_argument("--joint-tile", type=int, default=0, choices=[0, 1, 2, 3, 4],
+    ap.add_argument("--joint-tile", type=int, default=0, choices=sorted(TILING),
                     help="fused joint tiling: 0 autotuned per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16, "
                          "4 16x16 capped at 96 VGPRs")
     ap.add_argument("--seed", type=int, default=2)
@@ -297,7 +393,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "launches": 5}))
         return 0
-    line = run_batch(args, rf, torch) if args.workload == "batch64" else run_joint(args, rf, torch)
+    run = {"batch64": run_batch, "tvl1-320": run_tvl1, "methods": run_methods}.get(args.workload, run_joint)
+    line = run(args, rf, torch)
     print(json.dumps(line))
     return 0
 

@@ -96,16 +96,26 @@ typedef struct rf_params {
                               levels >= 1, Reconflow.cu:99-110, which breaks the coarse triangulation) */
     int tri_f64;           /* 1: evaluate the triangulation (Solve3dMinimalArea) in FP64 */
     int joint_tile;        /* fused joint iteration tiling: RF_JOINT_TILE_* (results are identical) */
+    int flow_tile;         /* fused TV-L1 iteration tiling: RF_FLOW_TILE_* (results are identical) */
 } rf_params;
 
 /* Tilings of the fused joint iteration (all give the same results). AUTO: the first joint solve
  * times 16x16, 16x16_OCC5 and 32x16 on alternating warps of every pyramid level and keeps the
- * fastest per level (needs >= 3 warps; otherwise 16x16). That first solve synchronises. */
+ * fastest per level (with fewer than 3 warps, the first ones of that list; with one warp,
+ * 16x16). That first solve synchronises. */
 #define RF_JOINT_TILE_AUTO 0
 #define RF_JOINT_TILE_64x8_RPT2 1  /* 64x4 lanes, two rows per lane plus a ring pass */
 #define RF_JOINT_TILE_32x16 2      /* one pixel per lane, 30x14 pixels per workgroup */
 #define RF_JOINT_TILE_16x16 3      /* one pixel per lane, 14x14 pixels per workgroup */
 #define RF_JOINT_TILE_16x16_OCC5 4 /* as 16x16, registers capped for five waves per SIMD */
+
+/* Tilings of the fused TV-L1 iteration (all give the same results). AUTO: the first solve times
+ * 16x16, 32x16 and 64x16_RPT4 on alternating warps of every level and keeps the fastest per level
+ * (with fewer than 3 warps, the first ones of that list). That first solve synchronises. */
+#define RF_FLOW_TILE_AUTO 0
+#define RF_FLOW_TILE_64x16_RPT4 1 /* 64x4 lanes, four rows per lane plus a ring pass */
+#define RF_FLOW_TILE_16x16 2      /* one pixel per lane, 15x15 pixels per workgroup */
+#define RF_FLOW_TILE_32x16 3      /* one pixel per lane, 31x15 pixels per workgroup */
 
 /* Fills p with the reference defaults (Cudaflow.cu:99-101, Reconflow.cu:27-30) for
  * width x height: 1 warp, 100 iterations, lambda 100, theta 0.33, tau 0.25, alignment 32,
@@ -227,6 +237,8 @@ int rf_profile_read(rf_context* ctx, double* total_ms, int* launches);
  * fixed choice, or what the tuning solve picked (RF_JOINT_TILE_AUTO before it ran). Fused path
  * only (no counterpart in the reference). */
 int rf_joint_tiling(rf_context* ctx, int* tiling, int n);
+/* The same for the fused TV-L1 iteration (RF_FLOW_TILE_*). */
+int rf_flow_tiling(rf_context* ctx, int* tiling, int n);
 
 /* ---- batch API: many pairs of uneven sizes per launch ---- */
 typedef struct rf_batch rf_batch;

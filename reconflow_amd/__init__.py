@@ -345,9 +345,9 @@ class Batch:
 
 def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50.0, theta=0.3, tau=0.25,
                  alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1, lambdams=100.0, use_graph=True, tri_f64=0,
-                 storage=STORAGE_F32) -> RFParams:
-    """Parameters for a Batch: configuration-0 style TV-L1 (method 0) or configuration-1 style
-    joint solve (method 14), float gray input."""
+                 storage=STORAGE_F32, joint_tile=0, flow_tile=0) -> RFParams:
+    """Parameters for a Batch: TV-L1 flow (method 0) or the joint flow/depth solve (method 14),
+    float gray input, scale 2 pyramid."""
     lib = _native.lib()
     p = RFParams()
     lib.rf_params_default(C.byref(p), 1, 1)
@@ -356,12 +356,14 @@ def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50
     p.n_warp_iters, p.n_solver_iters = n_warp, n_iter
     p.lambdaf, p.lambdams, p.alpha_tv, p.alpha_proj = lambdaf, lambdams, alpha_tv, alpha_proj
     p.path, p.use_graph, p.tri_f64, p.storage = PATH_FUSED, int(use_graph), tri_f64, storage
+    p.joint_tile, p.flow_tile = joint_tile, flow_tile
     return p
 
 
 # ---- seeded synthetic pairs ---------------------------------------------------------
 def make_flow_pair(w: int, h: int, seed: int, stride: int | None = None):
-    """Config-0 style pair: returns (i0, i1, gt_u, gt_v), each (h, stride) float32."""
+    """Synthetic textured pair moved by a smooth flow field: returns (i0, i1, gt_u, gt_v), each
+    (h, stride) float32."""
     s = stride or align_up(w)
     arrs = [np.zeros((h, s), np.float32) for _ in range(4)]
     check(_native.synth().rfs_make_flow_pair(w, h, s, seed, *[a.ctypes.data for a in arrs]), "rfs_make_flow_pair")
@@ -369,7 +371,8 @@ def make_flow_pair(w: int, h: int, seed: int, stride: int | None = None):
 
 
 def make_recon_pair(w: int, h: int, seed: int, stride: int | None = None):
-    """Config-1 style pair: returns dict with i0, i1, K, R, t, gt_u, gt_v, gt_Z."""
+    """Synthetic rigid scene seen by two posed pinhole cameras: returns dict with i0, i1, K, R, t,
+    gt_u, gt_v, gt_Z."""
     s = stride or align_up(w)
     i0, i1, gu, gv, gz = (np.zeros((h, s), np.float32) for _ in range(5))
     K, R, t = np.zeros(9), np.zeros(9), np.zeros(3)

@@ -23,7 +23,7 @@ class RFParams(C.Structure):
         ("lambdaf", C.c_float), ("lambdams", C.c_float), ("alpha_tv", C.c_float), ("alpha_proj", C.c_float),
         ("alpha_fn", C.c_float), ("stride_alignment", C.c_int), ("path", C.c_int), ("storage", C.c_int),
         ("use_graph", C.c_int), ("fix_coarse_k", C.c_int), ("tri_f64", C.c_int),
-        ("joint_tile", C.c_int),
+        ("joint_tile", C.c_int), ("flow_tile", C.c_int),
     ]
 
 
@@ -67,6 +67,7 @@ _PROTOS = {
     "rf_profile_arm": (_I, [_P, _I, _I]),
     "rf_profile_read": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_I)]),
     "rf_joint_tiling": (_I, [_P, C.POINTER(_I), _I]),
+    "rf_flow_tiling": (_I, [_P, C.POINTER(_I), _I]),
     "rf_batch_create": (_I, [C.POINTER(_P), C.POINTER(RFParams), _I, _P, _P]),
     "rf_batch_destroy": (_I, [_P]),
     "rf_batch_stride": (_I, [_P, _I]),

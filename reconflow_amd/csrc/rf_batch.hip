@@ -508,36 +508,16 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
     }
 }
 
-// One joint iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
-// outer ring of lanes only feeds its (EXW-2) x (EXH-2) interior. Every global load of a lane is
-// issued up front (coordinates clamped into the image so that the loads are in bounds; lanes off
-// the image compute values nobody reads), then
-//   data step -> uhat_n, projection blend -> uproj_n (registers), triangulation -> X'_n;
-//   uhat_n and X'_n to LDS, one barrier;
-//   interior lanes: dual step -> p_n, 3x3 XYZ median of X'_n, cleanup -> X_n, reprojection ->
-//   uc_n, Sk update, stores.
-// Same per-pixel arithmetic as kb_joint_iter, so the results are identical. SINGLE: the launch
-// covers one pair whose tiles are the dispatch indices (no tile list read) and whose field
-// table is in the kernel arguments.
-// OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
-// schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
-// the FP64 triangulation, met without spilling, for occupancy on large levels).
-template <int EXW, int EXH, int OCC, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
-__global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
-    constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
-    __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
-    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
-    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const JointView V =
-        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW);
-    const int w = V.w, h = V.h, s = V.s;
-    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
-    const int x0 = txy.x * TW, y0 = txy.y * TH;
-    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
-    const int x = x0 - 1 + ex, y = y0 - 1 + ey;
-    const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
-    // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
-    // offset
+// The front half of a joint iteration at pixel (xc, yc) (in the image): every global load, the
+// data step -> uhat_n, the projection blend -> uproj_n and the triangulation -> X'_n, with the
+// values the back half needs.
+struct JointFront {
+    float p1u, p2u, p1v, p2v, uu, vv, sku, skv, nduh, ndvh, upn, vpn, oX, oY, oZ;
+};
+
+template <bool F64TRI, bool PRIOR, typename TS>
+__device__ __forceinline__ JointFront joint_front(const IterArgs& a, const JointView& V, int xc, int yc) {
+    const int s = V.s;
     const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
@@ -602,6 +582,44 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     else
         solve3d_px<float>(*V.pq, a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
                           Yu, Zu, hl, hu, oX, oY, oZ);
+    return JointFront{p1u, p2u, p1v, p2v, uu, vv, sku, skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
+}
+
+// One joint iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
+// outer ring of lanes only feeds its (EXW-2) x (EXH-2) interior. Every global load of a lane is
+// issued up front (coordinates clamped into the image so that the loads are in bounds; lanes off
+// the image compute values nobody reads), then
+//   data step -> uhat_n, projection blend -> uproj_n (registers), triangulation -> X'_n;
+//   uhat_n and X'_n to LDS, one barrier;
+//   interior lanes: dual step -> p_n, 3x3 XYZ median of X'_n, cleanup -> X_n, reprojection ->
+//   uc_n, Sk update, stores.
+// Same per-pixel arithmetic as kb_joint_iter, so the results are identical. SINGLE: the launch
+// covers one pair whose tiles are the dispatch indices (no tile list read) and whose field
+// table is in the kernel arguments.
+// OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
+// schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
+// the FP64 triangulation, met without spilling, for occupancy on large levels).
+template <int EXW, int EXH, int OCC, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
+__global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
+    constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
+    __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
+    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
+    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
+    const JointView V =
+        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW);
+    const int w = V.w, h = V.h, s = V.s;
+    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int x0 = txy.x * TW, y0 = txy.y * TH;
+    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
+    const int x = x0 - 1 + ex, y = y0 - 1 + ey;
+    const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+    // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
+    // offset
+    const JointFront J = joint_front<F64TRI, PRIOR, TS>(a, V, xc, yc);
+    const float p1u = J.p1u, p2u = J.p2u, p1v = J.p1v, p2v = J.p2v, uu = J.uu, vv = J.vv, sku = J.sku,
+                skv = J.skv, nduh = J.nduh, ndvh = J.ndvh, upn = J.upn, vpn = J.vpn, oX = J.oX, oY = J.oY,
+                oZ = J.oZ;
+    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
     s_du[e] = nduh;
     s_dv[e] = ndvh;
     s_X[e] = oX;
@@ -754,6 +772,73 @@ __global__ void __launch_bounds__(256) kb_flow_iter(IterArgs a) {
     }
 }
 
+// One TV-L1 iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
+// right column and bottom row of lanes only feed the (EXW-1) x (EXH-1) interior (the dual step
+// reads uhat_n to the right and below). Every lane loads its inputs up front (coordinates clamped
+// into the image), runs the data step -> uhat_n into LDS, passes one barrier; interior lanes run
+// the dual step and store. Same per-pixel arithmetic as kb_flow_iter. SINGLE: one pair whose
+// tiles are the dispatch indices and whose field table is in the kernel arguments.
+template <int EXW, int EXH, bool SINGLE, typename TS>
+__global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
+    constexpr int TW = EXW - 1, TH = EXH - 1, NE = EXW * EXH;
+    __shared__ float s_du[NE], s_dv[NE];
+    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
+    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
+    const JointView V =
+        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW);
+    const int w = V.w, h = V.h, s = V.s;
+    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
+    const int x = txy.x * TW + ex, y = txy.y * TH + ey;
+    const int xc = min(x, w - 1), yc = min(y, h - 1);
+    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
+    const bool hl = xc > 0, hu = yc > 0;
+    const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
+    const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
+    float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
+          p1vl = 0.0f, p2vd = 0.0f;
+    if (!a.first) {
+        duh = el(V.du0, q);
+        dvh = el(V.dv0, q);
+        p1u = (float)el(p1u0, q);
+        p2u = (float)el(p2u0, q);
+        p1v = (float)el(p1v0, q);
+        p2v = (float)el(p2v0, q);
+        p1ul = (float)el(p1u0, ql);
+        p1vl = (float)el(p1v0, ql);
+        p2ud = (float)el(p2u0, qu);
+        p2vd = (float)el(p2v0, qu);
+    }
+    const float ix_ = (float)el(as<TS>(V.Ix), q), iy_ = (float)el(as<TS>(V.Iy), q), iz_ = (float)el(as<TS>(V.Iz), q);
+    float du, dv;
+    l1_threshold(ix_, iy_, iz_, duh, dvh, a.lambda, a.theta, du, dv);
+    const float nduh = du + a.theta * div_p(p1u, p1ul, p2u, p2ud, hl, hu);
+    const float ndvh = dv + a.theta * div_p(p1v, p1vl, p2v, p2vd, hl, hu);
+    s_du[e] = nduh;
+    s_dv[e] = ndvh;
+    __syncthreads();
+    if (ex == EXW - 1 || ey == EXH - 1 || x >= w || y >= h) return;
+    const float fac = a.tau / a.theta;
+    float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
+    if (x + 1 < w) {
+        dux = s_du[e + 1] - nduh;
+        dvx = s_dv[e + 1] - ndvh;
+    }
+    if (y + 1 < h) {
+        duy = s_du[e + EXW] - nduh;
+        dvy = s_dv[e + EXW] - ndvh;
+    }
+    float o1, o2;
+    dual_px(dux, duy, p1u, p2u, fac, o1, o2);
+    el(as<TS>(V.p1u1), q) = st<TS>(o1);
+    el(as<TS>(V.p2u1), q) = st<TS>(o2);
+    dual_px(dvx, dvy, p1v, p2v, fac, o1, o2);
+    el(as<TS>(V.p1v1), q) = st<TS>(o1);
+    el(as<TS>(V.p2v1), q) = st<TS>(o2);
+    el(V.du1, q) = nduh;
+    el(V.dv1, q) = ndvh;
+}
+
 // Zero uhat (a warp with no solver iterations leaves the zeroed uhat of the reference).
 __global__ void __launch_bounds__(256) kb_zero_uhat(Launch L, int l, int pd) {
     BT_PIX(L, l)
@@ -806,8 +891,9 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         set_error("rf_batch_create: unknown input type");
         return RF_ERR_ARG;
     }
-    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_OCC5) {
-        set_error("rf_batch_create: unknown joint tiling");
+    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_OCC5 ||
+        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x16) {
+        set_error("rf_batch_create: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
     }
     if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV) {
@@ -846,7 +932,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
     geo_ = geo;
     if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return RF_ERR_HIP;
     const int L = p.n_levels;
-    const long per_fields = 2L * L + 52;  // pyramids + state/double buffers + prior + uvrgb(3) + scratch + inputs(2)
+    const long per_fields = 2L * L + 50;  // pyramids, inputs, derivatives, state double buffers, prior, uvrgb(3), scratch
     size_t total = 0;
     for (auto& g : geo_) total += (size_t)per_fields * g.bufn * sizeof(float);
     if (hipMalloc(&slab_, total) != hipSuccess) {
@@ -944,37 +1030,61 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         if (offs[k] >= 0) sets_[k].map = maps_ + offs[k];
     prior_ = false;
     dirty_ = true;
-    tuned_ = false;
-    jcls_.clear();
+    for (int k = 0; k < 2; ++k) {
+        tuned_[k] = false;
+        cls_[k].clear();
+    }
     return upload_pairs();
 }
 
 TileSet& Batch::tiles(int level, int cls) { return sets_[level * TC_COUNT + cls]; }
 
-// Joint iteration tiling of a level for a warp: a fixed choice; during the tuning solve the
-// candidates in turn; afterwards the tuned choice (14x14 tiles until then).
-static const int kTuneCand[3] = {TC_J16, TC_J16O, TC_J32};
+// Iteration tiling of a level for a warp (TV-L1 or joint): a fixed choice; during the tuning
+// solve the candidates in turn (the first min(3, warps per level)); afterwards the tuned choice
+// (the first candidate until then).
 constexpr int kNumCand = 3;
+static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_F32, TC_FLOW}, {TC_J16, TC_J16O, TC_J32}};
+static int num_cand(int warps) { return warps < kNumCand ? warps : kNumCand; }
 
-int Batch::joint_class(int level, int warp) const {
-    switch (p_.joint_tile) {
-        case RF_JOINT_TILE_64x8_RPT2: return TC_J64;
-        case RF_JOINT_TILE_32x16: return TC_J32;
-        case RF_JOINT_TILE_16x16: return TC_J16;
-        case RF_JOINT_TILE_16x16_OCC5: return TC_J16O;
-        default: break;
+int Batch::tile_class(bool recon, int level, int warp) const {
+    if (recon) {
+        switch (p_.joint_tile) {
+            case RF_JOINT_TILE_64x8_RPT2: return TC_J64;
+            case RF_JOINT_TILE_32x16: return TC_J32;
+            case RF_JOINT_TILE_16x16: return TC_J16;
+            case RF_JOINT_TILE_16x16_OCC5: return TC_J16O;
+            default: break;
+        }
+    } else {
+        switch (p_.flow_tile) {
+            case RF_FLOW_TILE_64x16_RPT4: return TC_FLOW;
+            case RF_FLOW_TILE_16x16: return TC_F16;
+            case RF_FLOW_TILE_32x16: return TC_F32;
+            default: break;
+        }
     }
-    if (tuning_) return kTuneCand[warp % kNumCand];
-    return tuned_ ? jcls_[level] : TC_J16;
+    const int k = recon ? 1 : 0;
+    if (tuning_) return kTuneCand[k][warp % num_cand(p_.n_warp_iters)];
+    return tuned_[k] ? cls_[k][level] : kTuneCand[k][0];
 }
 
 int Batch::joint_tiling(int level) const {
     if (p_.joint_tile != RF_JOINT_TILE_AUTO) return p_.joint_tile;
-    if (!tuned_ || level < 0 || level >= (int)jcls_.size()) return RF_JOINT_TILE_AUTO;
-    switch (jcls_[level]) {
+    if (!tuned_[1] || level < 0 || level >= (int)cls_[1].size()) return RF_JOINT_TILE_AUTO;
+    switch (cls_[1][level]) {
         case TC_J32: return RF_JOINT_TILE_32x16;
         case TC_J16O: return RF_JOINT_TILE_16x16_OCC5;
         default: return RF_JOINT_TILE_16x16;
+    }
+}
+
+int Batch::flow_tiling(int level) const {
+    if (p_.flow_tile != RF_FLOW_TILE_AUTO) return p_.flow_tile;
+    if (!tuned_[0] || level < 0 || level >= (int)cls_[0].size()) return RF_FLOW_TILE_AUTO;
+    switch (cls_[0][level]) {
+        case TC_FLOW: return RF_FLOW_TILE_64x16_RPT4;
+        case TC_F32: return RF_FLOW_TILE_32x16;
+        default: return RF_FLOW_TILE_16x16;
     }
 }
 
@@ -982,7 +1092,7 @@ int Batch::joint_tiling(int level) const {
 // per (level, warp) with events; each level keeps its faster candidate. Its results are those of
 // any other solve.
 int Batch::tune_tiling(bool recon) {
-    const int L = p_.n_levels, nw = p_.n_warp_iters;
+    const int L = p_.n_levels, nw = p_.n_warp_iters, nc = num_cand(nw);
     tev_.assign(2 * L * nw, nullptr);
     int rc = RF_OK;
     for (auto& ev : tev_)
@@ -994,10 +1104,11 @@ int Batch::tune_tiling(bool recon) {
         rc = check(e, "tuning solve");
     }
     if (!rc) rc = check(hipStreamSynchronize(st_), "tuning solve");
-    jcls_.assign(L, TC_J16);
+    const int k = recon ? 1 : 0;
+    cls_[k].assign(L, kTuneCand[k][0]);
     for (int l = 0; l < L && !rc; ++l) {
-        double t[kNumCand] = {0, 0, 0};
-        int n[kNumCand] = {0, 0, 0};
+        double t[kNumCand] = {};
+        int n[kNumCand] = {};
         for (int wi = 0; wi < nw; ++wi) {
             float ms = 0;
             if (check(hipEventElapsedTime(&ms, tev_[2 * (l * nw + wi)], tev_[2 * (l * nw + wi) + 1]),
@@ -1005,20 +1116,20 @@ int Batch::tune_tiling(bool recon) {
                 rc = RF_ERR_HIP;
                 break;
             }
-            t[wi % kNumCand] += ms;
-            ++n[wi % kNumCand];
+            t[wi % nc] += ms;
+            ++n[wi % nc];
         }
         double best = 0;
-        for (int c = 0; c < kNumCand && !rc; ++c)
+        for (int c = 0; c < nc && !rc; ++c)
             if (n[c] && (c == 0 || t[c] / n[c] < best)) {
                 best = t[c] / n[c];
-                jcls_[l] = kTuneCand[c];
+                cls_[k][l] = kTuneCand[k][c];
             }
     }
     for (auto& ev : tev_)
         if (ev) (void)hipEventDestroy(ev);
     tev_.clear();
-    tuned_ = rc == RF_OK;
+    tuned_[k] = rc == RF_OK;
     return rc;
 }
 
@@ -1156,6 +1267,22 @@ static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
 }
 
 template <typename TS>
+static void launch_flow(int cls, bool single, hipStream_t st, const IterArgs& a) {
+    const dim3 grid(a.L.ntiles);
+    switch (cls) {
+        case TC_F16:
+            if (single) hipLaunchKernelGGL((kb_flow_px<16, 16, true, TS>), grid, dim3(16, 16), 0, st, a);
+            else hipLaunchKernelGGL((kb_flow_px<16, 16, false, TS>), grid, dim3(16, 16), 0, st, a);
+            break;
+        case TC_F32:
+            if (single) hipLaunchKernelGGL((kb_flow_px<32, 16, true, TS>), grid, dim3(32, 16), 0, st, a);
+            else hipLaunchKernelGGL((kb_flow_px<32, 16, false, TS>), grid, dim3(32, 16), 0, st, a);
+            break;
+        default: hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), grid, dim3(TXW, TRW), 0, st, a); break;
+    }
+}
+
+template <typename TS>
 static void launch_joint(int cls, bool single, bool f64, bool prior, hipStream_t st, const IterArgs& a) {
     const int k = (single ? 4 : 0) | (f64 ? 2 : 0) | (prior ? 1 : 0);
     switch (k) {
@@ -1215,17 +1342,17 @@ hipError_t Batch::enqueue_t(bool recon) {
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi)], st_));
             for (int it = 0; it < p_.n_solver_iters; ++it) {
-                const int cls = recon ? joint_class(level, wi) : TC_FLOW;
+                const int cls = tile_class(recon, level, wi);
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
                            (fn && sparse_) ? 1 : 0, JointView{}};
                 // one pair: its fields travel in the kernel arguments (no dependent descriptor load)
-                if (recon && n() == 1 && cls != TC_J64)
+                if (n() == 1 && cls != TC_J64 && cls != TC_FLOW)
                     a.v = joint_view(hp_[0], dp_, level, pr.d, pr.f, pr.x, pr.u, kTileW[cls]);
                 if (ts.ntiles > 0) {
                     if (!recon)
-                        hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), dim3(ts.ntiles), dim3(TXW, TRW), 0, st_, a);
+                        launch_flow<TS>(cls, n() == 1, st_, a);
                     else
                         launch_joint<TS>(cls, n() == 1, p_.tri_f64 != 0, fn, st_, a);
                     RF_TRY(hipGetLastError());
@@ -1267,8 +1394,8 @@ int Batch::solve(bool recon, float flow_scale) {
         gexec_[kind] = nullptr;
     }
     flow_scale_ = flow_scale;
-    if (recon && p_.joint_tile == RF_JOINT_TILE_AUTO && !tuned_ && p_.n_warp_iters >= kNumCand && p_.n_solver_iters > 0 &&
-        !prof_.armed)
+    const bool auto_tile = recon ? p_.joint_tile == RF_JOINT_TILE_AUTO : p_.flow_tile == RF_FLOW_TILE_AUTO;
+    if (auto_tile && !tuned_[kind] && p_.n_warp_iters >= 2 && p_.n_solver_iters > 0 && !prof_.armed)
         return tune_tiling(recon);
     if (!p_.use_graph || prof_.armed) {
         if (check(enqueue(recon), "batch solve")) return RF_ERR_HIP;

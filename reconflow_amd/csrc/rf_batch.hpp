@@ -26,7 +26,7 @@ struct PairFields {
     float *u[2], *v[2];
     float *ufn, *vfn, *ufn_l, *vfn_l, *mask, *mask_l;
     float* uvrgb;
-    float* scratch;  // spare field (edge-artifact filter output)
+    float* scratch;          // spare field (edge-artifact filter output)
     PQ pq[kMaxLevels];
     PQd pqd[kMaxLevels];
     UcParams ucp[kMaxLevels];
@@ -34,11 +34,22 @@ struct PairFields {
 
 // Tile shapes (pixels per workgroup) of the batched launches: pointwise 64x4; joint iteration
 // 64x8 (two rows per lane), 30x14 and 14x14 (one pixel per lane of a 32x16 / 16x16 extended
-// tile); TV-L1 iteration 64x16.
-// (TC_J16O: the 14x14 tiling compiled for occupancy rather than latency.)
-enum TileClass { TC_PIX = 0, TC_J64 = 1, TC_FLOW = 2, TC_J32 = 3, TC_J16 = 4, TC_J16O = 5, TC_COUNT = 6 };
-constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14};
-constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14};
+// tile; TC_J16O: the 14x14 tiling compiled for occupancy rather than latency); TV-L1 iteration
+// 64x16 (four rows per lane), 15x15 and 31x15 (one pixel per lane of a 16x16 / 32x16 extended
+// tile).
+enum TileClass {
+    TC_PIX = 0,
+    TC_J64 = 1,
+    TC_FLOW = 2,
+    TC_J32 = 3,
+    TC_J16 = 4,
+    TC_J16O = 5,
+    TC_F16 = 6,
+    TC_F32 = 7,
+    TC_COUNT = 8
+};
+constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14, 15, 31};
+constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14, 15, 15};
 
 // One launch's tile list: (pair, tile index within the pair).
 struct TileSet {
@@ -76,6 +87,7 @@ class Batch {
     Profiler& prof() { return prof_; }
     const rf_params& params() const { return p_; }
     int joint_tiling(int level) const;  // RF_JOINT_TILE_* in use at a level
+    int flow_tiling(int level) const;   // RF_FLOW_TILE_* in use at a level
 
   private:
     struct Geo {
@@ -88,7 +100,7 @@ class Batch {
     int check(hipError_t e, const char* what);
     int upload_pairs();
     TileSet& tiles(int level, int cls);
-    int joint_class(int level, int warp) const;
+    int tile_class(bool recon, int level, int warp) const;
     int tune_tiling(bool recon);
     void release();
 
@@ -105,10 +117,11 @@ class Batch {
     int2* maps_ = nullptr;
     Parity final_{};
     bool dirty_ = true;
-    // joint tiling per level: fixed by the parameters, or chosen by timing the candidates on
-    // alternating warps of the first solve (every tiling gives the same results)
-    std::vector<int> jcls_;
-    bool tuned_ = false, tuning_ = false;
+    // iteration tiling per level, [0] TV-L1 and [1] joint: fixed by the parameters, or chosen by
+    // timing the candidates on alternating warps of the first solve (every tiling gives the same
+    // results)
+    std::vector<int> cls_[2];
+    bool tuned_[2] = {false, false}, tuning_ = false;
     std::vector<hipEvent_t> tev_;
     hipGraphExec_t gexec_[2] = {nullptr, nullptr};
     Parity gfinal_[2];

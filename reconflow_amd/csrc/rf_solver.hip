@@ -79,8 +79,9 @@ int Solver::initialize(const rf_params& p) {
         set_error("rf_initialize: unknown path");
         return RF_ERR_ARG;
     }
-    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_OCC5) {
-        set_error("rf_initialize: unknown joint tiling");
+    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_OCC5 ||
+        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x16) {
+        set_error("rf_initialize: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
     }
     if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV) {
@@ -586,16 +587,17 @@ int Solver::profile_arm(int kid, int level) {
     return RF_OK;
 }
 
-int Solver::joint_tiling(int* tiling, int n) {
+int Solver::tiling(bool recon, int* tiling, int n) {
+    const char* name = recon ? "rf_joint_tiling" : "rf_flow_tiling";
     if (!tiling || n < 0) {
-        set_error("rf_joint_tiling: null output or negative count");
+        set_error(std::string(name) + ": null output or negative count");
         return RF_ERR_ARG;
     }
     if (!batch_) {
-        set_error("rf_joint_tiling: the fused path is not in use");
+        set_error(std::string(name) + ": the fused path is not in use");
         return RF_ERR_STATE;
     }
-    for (int l = 0; l < n; ++l) tiling[l] = batch_->joint_tiling(l);
+    for (int l = 0; l < n; ++l) tiling[l] = recon ? batch_->joint_tiling(l) : batch_->flow_tiling(l);
     return RF_OK;
 }
 
@@ -1120,7 +1122,12 @@ int rf_profile_read(rf_context* ctx, double* total_ms, int* launches) {
 
 int rf_joint_tiling(rf_context* ctx, int* tiling, int n) {
     RF_CTX(ctx);
-    return ctx->s->joint_tiling(tiling, n);
+    return ctx->s->tiling(true, tiling, n);
+}
+
+int rf_flow_tiling(rf_context* ctx, int* tiling, int n) {
+    RF_CTX(ctx);
+    return ctx->s->tiling(false, tiling, n);
 }
 
 }  // extern "C"

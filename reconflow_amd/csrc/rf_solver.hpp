@@ -110,7 +110,7 @@ class Solver {
     int sync();
     int profile_arm(int kid, int level);
     int profile_read(double* total_ms, int* launches);
-    int joint_tiling(int* tiling, int n);
+    int tiling(bool recon, int* tiling, int n);  // joint (recon) or TV-L1 tilings per level
     Profiler& prof() { return prof_; }
 
     const Geometry& geom() const { return g_; }

@@ -19,10 +19,11 @@ def _fetch(ptr, n):
     return fetch(ptr, n)
 
 
+@pytest.mark.parametrize("tile", [0, 1, 3], ids=["auto", "64x16-rpt4", "32x16"])
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
-def test_batch_flow(orc, graph):
-    print(f"case: batch TV-L1 {len(SIZES)} pairs graph={graph}", flush=True)
-    p = rf.batch_params(0, 3, 2, 10, use_graph=graph)
+def test_batch_flow(orc, graph, tile):
+    print(f"case: batch TV-L1 {len(SIZES)} pairs graph={graph} tile={tile}", flush=True)
+    p = rf.batch_params(0, 3, 2, 10, use_graph=graph, flow_tile=tile)
     b = rf.Batch(p, [w for w, _ in SIZES], [h for _, h in SIZES])
     pairs, keep = [], []
     for i, (w, h) in enumerate(SIZES):
@@ -44,10 +45,11 @@ def test_batch_flow(orc, graph):
             assert np.array_equal(u[:, :w], ru[:, :w]) and np.array_equal(v[:, :w], rv[:, :w]), (rep, i)
 
 
+@pytest.mark.parametrize("tile", [0, 2], ids=["auto", "32x16"])
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
-def test_batch_recon(orc, graph):
-    print(f"case: batch joint {len(SIZES)} pairs graph={graph}", flush=True)
-    p = rf.batch_params(14, 3, 2, 10, tau=0.125, use_graph=graph)
+def test_batch_recon(orc, graph, tile):
+    print(f"case: batch joint {len(SIZES)} pairs graph={graph} tile={tile}", flush=True)
+    p = rf.batch_params(14, 3, 2, 10, tau=0.125, use_graph=graph, joint_tile=tile)
     b = rf.Batch(p, [w for w, _ in SIZES], [h for _, h in SIZES])
     data, keep = [], []
     for i, (w, h) in enumerate(SIZES):

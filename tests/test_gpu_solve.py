@@ -11,6 +11,8 @@ Bars:
     triangulation the FP32-vs-FP64 gap is a property of the arithmetic, not of this code: the
     FP32 restatement itself misses the bar on some seeds; the bit-exact check covers it.)
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -19,10 +21,13 @@ import reconflow_amd as rf
 pytestmark = pytest.mark.gpu
 
 
-def _flow(path=rf.PATH_PLAIN, graph=False, w=320, h=240, L=4, nw=3, ni=30, storage=rf.STORAGE_F32):
+def _flow(path=rf.PATH_PLAIN, graph=False, w=320, h=240, L=4, nw=3, ni=30, storage=rf.STORAGE_F32, flow_tile=0):
     f = rf.CudaFlow()
     f.initialize(w, h, 1, rf.INPUT_32F, L, 2.0, rf.CudaFlow.METHOD_TVL1, 50.0, 0.0, 0.3, 0.25, nw, ni, False,
                  path=path, use_graph=graph, storage=storage)
+    if flow_tile:
+        f.params.flow_tile = flow_tile
+        f._init_params(f.params)
     return f
 
 
@@ -61,6 +66,31 @@ def test_flow_320x240_paths_equal_plain(tvl1_320, path, graph):
         f.solveOpticalFlow(5.0)
         u, v = f.copyOpticalFlowToHost()
         assert np.array_equal(u[:, :320], u32[:, :320]) and np.array_equal(v[:, :320], v32[:, :320]), rep
+
+
+@pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16], ids=["f32", "f16"])
+def test_flow_tilings_equal(orc, storage):
+    # every tiling of the fused TV-L1 iteration gives the same bits (uneven level sizes:
+    # 200x152 -> 100x76 -> 50x38, partial tiles on every level); FP32 storage equals the oracle
+    print(f"case: TV-L1 200x152 fused graph tilings storage={storage}", flush=True)
+    i0, i1, _, _ = rf.make_flow_pair(200, 152, 5)
+    res = {}
+    for tile in (1, 2, 3, 0):
+        f = _flow(path=rf.PATH_FUSED, graph=True, w=200, h=152, L=3, nw=3, ni=8, storage=storage, flow_tile=tile)
+        for rep in range(2):
+            f.copyImagesToDevice(i0, i1)
+            f.solveOpticalFlow(5.0)
+            res[(tile, rep)] = f.copyOpticalFlowToHost()
+        if tile == 0:
+            tl = (C.c_int * 3)()
+            assert f._lib.rf_flow_tiling(f._ctx, tl, 3) == 0
+            assert all(t in (1, 2, 3) for t in tl), list(tl)
+    base = res[(1, 0)]
+    for k, (u, v) in res.items():
+        assert np.array_equal(u[:, :200], base[0][:, :200]) and np.array_equal(v[:, :200], base[1][:, :200]), k
+    if storage == rf.STORAGE_F32:
+        ru, rv = orc.solve_flow(orc.flow_params(200, 152, 3, n_warp=3, n_iter=8), i0, i1, "f32")
+        assert np.array_equal(base[0][:, :200], ru[:, :200]) and np.array_equal(base[1][:, :200], rv[:, :200])
 
 
 def test_flow_320x240_fp16_storage_epe(tvl1_320):
