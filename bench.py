@@ -12,6 +12,8 @@ Prints one JSON line: frames/s, the roofline of the dominant kernel (its algorit
 launch over its average launch time at level 0, timed with HIP events on the solver's stream),
 the FP64 CPU oracle's time on the same frame (the reported CPU baseline), the mean EPE of the
 GPU flow against that FP64 result, and the per-step (plain) GPU path's frames/s beside it.
+`batched_640x480` adds the same solve over --batch-pairs seeded 640x480 pairs per batched launch
+(FP32 storage, and FP16 derivatives), frames/s and pair 0's EPE against the FP64 result.
 
 --workload tvl1-320: the 320x240 TV-L1 case (fused, fused FP16, plain; EPE and time of the FP64
 oracle). --workload methods: each CudaFlow data term at 320x240 on the per-step path.
@@ -62,6 +64,9 @@ def make_solver(rf, path, graph, tri, joint_tile=0):
     return r
 
 
+REF64 = {}  # the FP64 oracle's flow of the single-pair workload, kept for the batched EPE
+
+
 def cpu_baseline(d, u, v):
     """The FP64 restatement on the host: median of 5 timed frames after one warm-up at the
     default OpenMP thread count, one frame on one thread, one FP32 frame; and the GPU flow's
@@ -78,6 +83,7 @@ def cpu_baseline(d, u, v):
         return time.perf_counter() - t0, res
 
     _, ref = timed("f64")
+    REF64["joint"] = ref
     runs = [timed("f64")[0] for _ in range(5)]
     t_med = float(np.median(runs))
     t32, _ = timed("f32")
@@ -91,6 +97,48 @@ def cpu_baseline(d, u, v):
                      f"1 thread {t1:.2f} s/frame; FP32 restatement {t32:.2f} s/frame",
            "one_thread_frames_per_s": round(1.0 / t1, 4), "fp32_frames_per_s": round(1.0 / t32, 4)}
     return cpu, epe
+
+
+def _fetch(dptr, n):
+    """n floats at a raw device address -> host numpy (after a device synchronisation)."""
+    hip = ctypes.CDLL("libamdhip64.so.7")  # by soname: the runtime torch and librfhip already share
+    hip.hipDeviceSynchronize.restype = ctypes.c_int
+    hip.hipMemcpy.restype = ctypes.c_int
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipDeviceSynchronize() == 0
+    out = np.empty(n, np.float32)
+    assert hip.hipMemcpy(out.ctypes.data, dptr, n * 4, 2) == 0  # device to host
+    return out
+
+
+def run_joint_batched(args, rf, torch, n, storage):
+    """n seeded 640x480 pairs (pair 0 is the single-pair workload's scene) in one batch: every
+    launch covers the tiles of all n pairs. Returns frames/s and pair 0's flow."""
+    p = rf.batch_params(14, LEVELS, WARPS, ITERS, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0,
+                        lambdaf=0.1, lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage)
+    b = rf.Batch(p, [W] * n, [H] * n)
+    keep = []
+    for i in range(n):
+        d = rf.make_recon_pair(W, H, args.seed + i, stride=b.stride(i))
+        t0, t1 = torch.from_numpy(d["i0"]).cuda(), torch.from_numpy(d["i1"]).cuda()
+        keep += [t0, t1]
+        b.set_images_device(i, t0.data_ptr(), t1.data_ptr())
+        b.set_camera(i, d["K"], d["K"], d["R"], d["t"])
+    torch.cuda.synchronize()
+    for _ in range(max(1, args.warmup)):  # the first solve is the tuning solve
+        b.solve()
+    b.synchronize()
+    steps = max(2, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.solve()
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    s = b.stride(0)
+    u_ptr, v_ptr = b.result_ptrs(0)[:2]
+    u, v = _fetch(u_ptr, H * s).reshape(H, s), _fetch(v_ptr, H * s).reshape(H, s)
+    del b, keep
+    return n * steps / dt, u, v
 
 
 def run_joint(args, rf, torch):
@@ -166,6 +214,18 @@ def run_joint(args, rf, torch):
         del rp
 
     cpu, epe = (None, None) if args.no_cpu_baseline else cpu_baseline(d, u, v)
+    batched = None
+    if path == rf.PATH_FUSED and args.batch_pairs > 0:
+        batched = {"pairs_per_launch": args.batch_pairs}
+        for name, storage in (("f32", rf.STORAGE_F32), ("f16_derivatives", rf.STORAGE_F16_DERIV)):
+            bfps, bu, bv = run_joint_batched(args, rf, torch, args.batch_pairs, storage)
+            ent = {"frames_per_s": round(bfps, 2)}
+            if plain_fps:
+                ent["speedup_vs_plain_path"] = round(bfps / plain_fps, 3)
+            if "joint" in REF64:
+                ru, rv = REF64["joint"][0], REF64["joint"][1]
+                ent["epe_vs_fp64_oracle"] = round(float(np.hypot(bu[:, :W] - ru[:, :W], bv[:, :W] - rv[:, :W]).mean()), 5)
+            batched[name] = ent
     px_iter = sum((W >> l) * (H >> l) for l in range(LEVELS)) * WARPS * ITERS
     return {
         "metric": "640x480 flow+depth frames/s at EPE<=0.05px vs FP64 oracle; % of HBM peak",
@@ -193,6 +253,7 @@ def run_joint(args, rf, torch):
         "plain_fp32_path_fps": None if plain_fps is None else round(plain_fps, 3),
         "speedup_vs_plain_path": None if plain_fps is None else round(fps / plain_fps, 3),
         "mpix_iter_per_s": round(fps * px_iter / 1e6, 1),
+        "batched_640x480": batched,
     }
 
 
@@ -363,6 +424,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["joint640", "batch64", "tvl1-320", "methods"], default="joint640")
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--batch-pairs", type=int, default=8,
+                    help="joint640: also time this many 640x480 pairs per batched launch (0: skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
     ap.add_argument("--joint-tile", type=int, default=0, choices=sorted(TILING),
