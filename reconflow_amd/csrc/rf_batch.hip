@@ -42,6 +42,7 @@ constexpr int TXW = 64;
 constexpr int TRW = 4;
 constexpr int RPT_JOINT = 2;  // 64 x 8 joint tiles
 constexpr int RPT_FLOW = 4;   // 64 x 16 TV-L1 tiles
+
 constexpr int FLAT = kMaxLevels;
 
 __device__ __forceinline__ int xcd_tile(int b, int n) {
@@ -277,8 +278,10 @@ struct JointView {
 };
 
 // F: the pair's descriptor as the caller sees it; dF: the same descriptor's device address.
+// first: the first iteration of a warp, which starts from zero uhat and duals (the pair's zero
+// field stands in for them).
 __host__ __device__ inline JointView joint_view(const PairFields& F, const PairFields* dF, int l, int pd, int pf,
-                                                int px, int pu, int tw) {
+                                                int px, int pu, int tw, bool first) {
     JointView v;
     v.du0 = F.du[pd], v.dv0 = F.dv[pd], v.p1u0 = F.p1u[pd], v.p2u0 = F.p2u[pd], v.p1v0 = F.p1v[pd],
     v.p2v0 = F.p2v[pd];
@@ -291,6 +294,7 @@ __host__ __device__ inline JointView joint_view(const PairFields& F, const PairF
     v.sku1 = F.sku[pf ^ 1], v.skv1 = F.skv[pf ^ 1], v.up1 = F.up[pf ^ 1], v.vp1 = F.vp[pf ^ 1], v.uc1 = F.uc[pf ^ 1],
     v.vc1 = F.vc[pf ^ 1];
     v.X1 = F.X[px ^ 1], v.Y1 = F.Y[px ^ 1], v.Z1 = F.Z[px ^ 1];
+    if (first) v.du0 = v.dv0 = v.p1u0 = v.p2u0 = v.p1v0 = v.p2v0 = F.zero;
     v.w = F.W[l], v.h = F.H[l], v.s = F.S[l];
     v.tiles_x = (v.w + tw - 1) / tw;
     v.inv_tiles_x = 1.0f / (float)v.tiles_x;
@@ -522,22 +526,11 @@ __device__ __forceinline__ JointFront joint_front(const IterArgs& a, const Joint
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
 
-    // ---- loads
-    float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
-          p1vl = 0.0f, p2vd = 0.0f;
-    if (!a.first) {
-        const float *p1u0 = V.p1u0, *p2u0 = V.p2u0, *p1v0 = V.p1v0, *p2v0 = V.p2v0;
-        duh = el(V.du0, q);
-        dvh = el(V.dv0, q);
-        p1u = el(p1u0, q);
-        p2u = el(p2u0, q);
-        p1v = el(p1v0, q);
-        p2v = el(p2v0, q);
-        p1ul = el(p1u0, ql);
-        p1vl = el(p1v0, ql);
-        p2ud = el(p2u0, qu);
-        p2vd = el(p2v0, qu);
-    }
+    // ---- loads (unconditional: on the first iteration of a warp the view's uhat and dual
+    // inputs are the zero field)
+    const float duh = el(V.du0, q), dvh = el(V.dv0, q);
+    const float p1u = el(V.p1u0, q), p2u = el(V.p2u0, q), p1v = el(V.p1v0, q), p2v = el(V.p2v0, q);
+    const float p1ul = el(V.p1u0, ql), p1vl = el(V.p1v0, ql), p2ud = el(V.p2u0, qu), p2vd = el(V.p2v0, qu);
     const float uu = el(V.U, q), vv = el(V.V, q);
     const float sku = el(V.sku0, q), skv = el(V.skv0, q), up0 = el(V.up0, q), vp0 = el(V.vp0, q);
     const float uc0 = el(V.uc0, q), vc0 = el(V.vc0, q);
@@ -585,65 +578,32 @@ __device__ __forceinline__ JointFront joint_front(const IterArgs& a, const Joint
     return JointFront{p1u, p2u, p1v, p2v, uu, vv, sku, skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
 }
 
-// One joint iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
-// outer ring of lanes only feeds its (EXW-2) x (EXH-2) interior. Every global load of a lane is
-// issued up front (coordinates clamped into the image so that the loads are in bounds; lanes off
-// the image compute values nobody reads), then
-//   data step -> uhat_n, projection blend -> uproj_n (registers), triangulation -> X'_n;
-//   uhat_n and X'_n to LDS, one barrier;
-//   interior lanes: dual step -> p_n, 3x3 XYZ median of X'_n, cleanup -> X_n, reprojection ->
-//   uc_n, Sk update, stores.
-// Same per-pixel arithmetic as kb_joint_iter, so the results are identical. SINGLE: the launch
-// covers one pair whose tiles are the dispatch indices (no tile list read) and whose field
-// table is in the kernel arguments.
-// OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
-// schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
-// the FP64 triangulation, met without spilling, for occupancy on large levels).
-template <int EXW, int EXH, int OCC, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
-__global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
-    constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
-    __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
-    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
-    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const JointView V =
-        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW);
-    const int w = V.w, h = V.h, s = V.s;
-    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
-    const int x0 = txy.x * TW, y0 = txy.y * TH;
-    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
-    const int x = x0 - 1 + ex, y = y0 - 1 + ey;
-    const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
-    // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
-    // offset
-    const JointFront J = joint_front<F64TRI, PRIOR, TS>(a, V, xc, yc);
-    const float p1u = J.p1u, p2u = J.p2u, p1v = J.p1v, p2v = J.p2v, uu = J.uu, vv = J.vv, sku = J.sku,
-                skv = J.skv, nduh = J.nduh, ndvh = J.ndvh, upn = J.upn, vpn = J.vpn, oX = J.oX, oY = J.oY,
-                oZ = J.oZ;
-    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
-    s_du[e] = nduh;
-    s_dv[e] = ndvh;
-    s_X[e] = oX;
-    s_Y[e] = oY;
-    s_Z[e] = oZ;
-    __syncthreads();
-    if (ex == 0 || ey == 0 || ex == EXW - 1 || ey == EXH - 1 || x >= w || y >= h) return;
-
-    // ---- dual step (interior)
+// The back half of a joint iteration at the in-image pixel (x, y), element q, whose
+// extended-tile slot is e (EXW slots per row; the tile's first interior pixel is (x0, y0) in slot
+// (1, 1)): dual step from uhat_n in LDS, 3x3 XYZ median of X'_n in LDS, cleanup, reprojection,
+// Sk update, stores.
+template <int EXW>
+__device__ __forceinline__ void joint_back(const IterArgs& a, const JointView& V, const UcParams& ucp,
+                                           const JointFront& J, int x, int y, int x0, int y0, int e, unsigned q,
+                                           const float* s_du, const float* s_dv, const float* s_X, const float* s_Y,
+                                           const float* s_Z) {
+    const int w = V.w, h = V.h;
+    // ---- dual step
     const float fac = a.tau / a.theta;
     float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
     if (x + 1 < w) {
-        dux = s_du[e + 1] - nduh;
-        dvx = s_dv[e + 1] - ndvh;
+        dux = s_du[e + 1] - J.nduh;
+        dvx = s_dv[e + 1] - J.ndvh;
     }
     if (y + 1 < h) {
-        duy = s_du[e + EXW] - nduh;
-        dvy = s_dv[e + EXW] - ndvh;
+        duy = s_du[e + EXW] - J.nduh;
+        dvy = s_dv[e + EXW] - J.ndvh;
     }
     float o1, o2;
-    dual_px(dux, duy, p1u, p2u, fac, o1, o2);
+    dual_px(dux, duy, J.p1u, J.p2u, fac, o1, o2);
     el(V.p1u1, q) = o1;
     el(V.p2u1, q) = o2;
-    dual_px(dvx, dvy, p1v, p2v, fac, o1, o2);
+    dual_px(dvx, dvy, J.p1v, J.p2v, fac, o1, o2);
     el(V.p1v1, q) = o1;
     el(V.p2v1, q) = o2;
 
@@ -667,20 +627,63 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
         }
     const float xm = median9_finite(mX), ym = median9_finite(mY), zm = median9_finite(mZ);
     float Xn, Yn, Zn;
-    cleanup_px(oX, oY, oZ, xm, ym, zm, Xn, Yn, Zn);
+    cleanup_px(J.oX, J.oY, J.oZ, xm, ym, zm, Xn, Yn, Zn);
     float ucn, vcn;
-    uc_px(*V.ucp, Xn, Yn, Zn, x, y, ucn, vcn);
+    uc_px(ucp, Xn, Yn, Zn, x, y, ucn, vcn);
     el(V.X1, q) = Xn;
     el(V.Y1, q) = Yn;
     el(V.Z1, q) = Zn;
     el(V.uc1, q) = ucn;
     el(V.vc1, q) = vcn;
-    el(V.du1, q) = nduh;
-    el(V.dv1, q) = ndvh;
-    el(V.up1, q) = upn;
-    el(V.vp1, q) = vpn;
-    el(V.sku1, q) = sk_px(sku, uu, nduh, upn);
-    el(V.skv1, q) = sk_px(skv, vv, ndvh, vpn);
+    el(V.du1, q) = J.nduh;
+    el(V.dv1, q) = J.ndvh;
+    el(V.up1, q) = J.upn;
+    el(V.vp1, q) = J.vpn;
+    el(V.sku1, q) = sk_px(J.sku, J.uu, J.nduh, J.upn);
+    el(V.skv1, q) = sk_px(J.skv, J.vv, J.ndvh, J.vpn);
+}
+
+// One joint iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
+// outer ring of lanes only feeds its (EXW-2) x (EXH-2) interior. Every global load of a lane is issued up front
+// (coordinates clamped into the image so that the loads are in bounds; lanes off the image
+// compute values nobody reads), then
+//   data step -> uhat_n, projection blend -> uproj_n (registers), triangulation -> X'_n;
+//   uhat_n and X'_n to LDS, one barrier;
+//   interior lanes: dual step -> p_n, 3x3 XYZ median of X'_n, cleanup -> X_n, reprojection ->
+//   uc_n, Sk update, stores.
+// Same per-pixel arithmetic as kb_joint_iter, so the results are identical. SINGLE: the launch
+// covers one pair whose tiles are the dispatch indices (no tile list read) and whose field
+// table is in the kernel arguments.
+// OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
+// schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
+// the FP64 triangulation, met without spilling, for occupancy on large levels).
+template <int EXW, int EXH, int OCC, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
+__global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
+    constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
+    __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
+    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
+    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
+    const JointView V =
+        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
+    const int w = V.w, h = V.h, s = V.s;
+    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int x0 = txy.x * TW, y0 = txy.y * TH;
+    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
+    const int x = x0 - 1 + ex, y = y0 - 1 + ey;
+    // read before any store of the kernel, so that it compiles to scalar loads
+    const UcParams ucp = *V.ucp;
+    const JointFront J = joint_front<F64TRI, PRIOR, TS>(a, V, min(max(x, 0), w - 1), min(max(y, 0), h - 1));
+    s_du[e] = J.nduh;
+    s_dv[e] = J.ndvh;
+    s_X[e] = J.oX;
+    s_Y[e] = J.oY;
+    s_Z[e] = J.oZ;
+    __syncthreads();
+    if (ex == 0 || ey == 0 || ex == EXW - 1 || ey == EXH - 1 || x >= w || y >= h) return;
+    // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
+    // offset
+    const unsigned q = (unsigned)x + (unsigned)y * (unsigned)s;
+    joint_back<EXW>(a, V, ucp, J, x, y, x0, y0, e, q, s_du, s_dv, s_X, s_Y, s_Z);
 }
 
 // One TV-L1 iteration: data step (tile + right column + bottom row), dual step (tile).
@@ -785,7 +788,7 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
     const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
     const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
     const JointView V =
-        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW);
+        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
     const int w = V.w, h = V.h, s = V.s;
     const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
     const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
@@ -795,20 +798,13 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
     const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
-    float duh = 0.0f, dvh = 0.0f, p1u = 0.0f, p2u = 0.0f, p1v = 0.0f, p2v = 0.0f, p1ul = 0.0f, p2ud = 0.0f,
-          p1vl = 0.0f, p2vd = 0.0f;
-    if (!a.first) {
-        duh = el(V.du0, q);
-        dvh = el(V.dv0, q);
-        p1u = (float)el(p1u0, q);
-        p2u = (float)el(p2u0, q);
-        p1v = (float)el(p1v0, q);
-        p2v = (float)el(p2v0, q);
-        p1ul = (float)el(p1u0, ql);
-        p1vl = (float)el(p1v0, ql);
-        p2ud = (float)el(p2u0, qu);
-        p2vd = (float)el(p2v0, qu);
-    }
+    // unconditional loads: on the first iteration of a warp the view's uhat and dual inputs are
+    // the zero field
+    const float duh = el(V.du0, q), dvh = el(V.dv0, q);
+    const float p1u = (float)el(p1u0, q), p2u = (float)el(p2u0, q), p1v = (float)el(p1v0, q),
+                p2v = (float)el(p2v0, q);
+    const float p1ul = (float)el(p1u0, ql), p1vl = (float)el(p1v0, ql), p2ud = (float)el(p2u0, qu),
+                p2vd = (float)el(p2v0, qu);
     const float ix_ = (float)el(as<TS>(V.Ix), q), iy_ = (float)el(as<TS>(V.Iy), q), iz_ = (float)el(as<TS>(V.Iz), q);
     float du, dv;
     l1_threshold(ix_, iy_, iz_, duh, dvh, a.lambda, a.theta, du, dv);
@@ -932,7 +928,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
     geo_ = geo;
     if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return RF_ERR_HIP;
     const int L = p.n_levels;
-    const long per_fields = 2L * L + 50;  // pyramids, inputs, derivatives, state double buffers, prior, uvrgb(3), scratch
+    const long per_fields = 2L * L + 51;  // pyramids, inputs, derivatives, state double buffers, prior, uvrgb(3), scratch, zero
     size_t total = 0;
     for (auto& g : geo_) total += (size_t)per_fields * g.bufn * sizeof(float);
     if (hipMalloc(&slab_, total) != hipSuccess) {
@@ -992,6 +988,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         F.mask_l = take(1);
         F.uvrgb = take(3);
         F.scratch = take(1);
+        F.zero = take(1);  // zero from the slab's clearing, never written
     }
     if (hipMalloc((void**)&dp_, sizeof(PairFields) * n) != hipSuccess) {
         dp_ = nullptr;
@@ -1349,7 +1346,7 @@ hipError_t Batch::enqueue_t(bool recon) {
                            (fn && sparse_) ? 1 : 0, JointView{}};
                 // one pair: its fields travel in the kernel arguments (no dependent descriptor load)
                 if (n() == 1 && cls != TC_J64 && cls != TC_FLOW)
-                    a.v = joint_view(hp_[0], dp_, level, pr.d, pr.f, pr.x, pr.u, kTileW[cls]);
+                    a.v = joint_view(hp_[0], dp_, level, pr.d, pr.f, pr.x, pr.u, kTileW[cls], it == 0);
                 if (ts.ntiles > 0) {
                     if (!recon)
                         launch_flow<TS>(cls, n() == 1, st_, a);

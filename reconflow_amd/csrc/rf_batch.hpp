@@ -27,6 +27,7 @@ struct PairFields {
     float *ufn, *vfn, *ufn_l, *vfn_l, *mask, *mask_l;
     float* uvrgb;
     float* scratch;          // spare field (edge-artifact filter output)
+    float* zero;             // all zero: the uhat and dual inputs of a warp's first iteration
     PQ pq[kMaxLevels];
     PQd pqd[kMaxLevels];
     UcParams ucp[kMaxLevels];
