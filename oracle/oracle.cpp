@@ -567,6 +567,114 @@ void solve3d(const T* uproj0, const T* vproj0, const T* X0, const T* Y0, const T
         }
 }
 
+// One pixel of the FP64 FMA triangulation (see solve3d_fma). r, d, l, u: X0 at ix+1, iy+1, ix-1
+// (used when hl), iy-1 (used when hu). On a non-finite solution the pixel keeps X0.
+static inline void tri_fma(const double* P, const double* Q, double lambdaf, double lambdams, int ix, int iy,
+                           double up, double vp, double X0, double Y0, double Z0, double Xr, double Yr, double Zr,
+                           double Xd, double Yd, double Zd, double Xl, double Yl, double Zl, double Xu, double Yu,
+                           double Zu, bool hl, bool hu, double& oX, double& oY, double& oZ) {
+    using std::fma;
+    const double x1 = (double)ix, y1 = (double)iy, x2 = (double)ix + up, y2 = (double)iy + vp;
+    double pp[4], qq[4], rr[4], ss[4];
+    for (int k = 0; k < 4; ++k) {
+        pp[k] = fma(x1, P[8 + k], -P[k]);
+        qq[k] = fma(y1, P[8 + k], -P[4 + k]);
+        rr[k] = fma(x2, Q[8 + k], -Q[k]);
+        ss[k] = fma(y2, Q[8 + k], -Q[4 + k]);
+    }
+    auto dot = [&](int i, int j) {
+        return lambdaf * fma(ss[i], ss[j], fma(rr[i], rr[j], fma(qq[i], qq[j], pp[i] * pp[j])));
+    };
+    const double ax = dot(0, 0), bx = dot(0, 1), cx = dot(0, 2), dx = dot(0, 3);
+    const double by = dot(1, 1), cy = dot(1, 2), dy = dot(1, 3);
+    const double cz = dot(2, 2), dz = dot(2, 3);
+    const double Xx = Xr - X0, Yx = Yr - Y0, Zx = Zr - Z0;
+    const double Xy = Xd - X0, Yy = Yd - Y0, Zy = Zd - Z0;
+    const double c1 = fma(Yx, Zy, -(Zx * Yy)), c2 = fma(Zx, Xy, -(Xx * Zy)), c3 = fma(Xx, Yy, -(Yx * Xy));
+    const double lm = lambdams / std::sqrt(fma(c3, c3, fma(c2, c2, c1 * c1)) + (double)0.001f);
+    const double Xib = hl ? fma(0.5, Xl, 0.5 * Xr) : Xr, Yib = hl ? fma(0.5, Yl, 0.5 * Yr) : Yr,
+                 Zib = hl ? fma(0.5, Zl, 0.5 * Zr) : Zr;
+    const double Xjb = hu ? fma(0.5, Xu, 0.5 * Xd) : Xd, Yjb = hu ? fma(0.5, Yu, 0.5 * Yd) : Yd,
+                 Zjb = hu ? fma(0.5, Zu, 0.5 * Zd) : Zd;
+    // products and squares of the forward differences
+    const double xxy = Xx * Yx, xyy = Xy * Yy, xxz = Xx * Zx, xyz = Xy * Zy, yxz = Yx * Zx, yyz = Yy * Zy;
+    const double syx = fma(Zx, Zx, Yx * Yx), syy = fma(Zy, Zy, Yy * Yy);  // Yx^2 + Zx^2, Yy^2 + Zy^2
+    const double sxx = fma(Zx, Zx, Xx * Xx), sxy = fma(Zy, Zy, Xy * Xy);  // Xx^2 + Zx^2, Xy^2 + Zy^2
+    const double txx = fma(Yx, Yx, Xx * Xx), txy = fma(Yy, Yy, Xy * Xy);  // Xx^2 + Yx^2, Xy^2 + Yy^2
+    const double a11 = fma(lm, syx + syy, ax), a22 = fma(lm, sxx + sxy, by), a33 = fma(lm, txx + txy, cz);
+    const double a12 = fma(-lm, fma(Xy, Yy, xxy), bx), a13 = fma(-lm, fma(Xy, Zy, xxz), cx),
+                 a23 = fma(-lm, fma(Yy, Zy, yxz), cy);
+    // adjugate (symmetric) and determinant
+    const double E11 = fma(a22, a33, -(a23 * a23)), E12 = fma(a13, a23, -(a12 * a33)),
+                 E13 = fma(a12, a23, -(a13 * a22));
+    const double E22 = fma(a11, a33, -(a13 * a13)), E23 = fma(a13, a12, -(a11 * a23)),
+                 E33 = fma(a11, a22, -(a12 * a12));
+    const double det = fma(a13, E13, fma(a12, E12, a11 * E11));
+    double b1 = syy * Xib;
+    b1 = fma(syx, Xjb, b1);
+    b1 = fma(-xyy, Yib, b1);
+    b1 = fma(-xxy, Yjb, b1);
+    b1 = fma(-xyz, Zib, b1);
+    b1 = fma(-xxz, Zjb, b1);
+    b1 = fma(lm, b1, -dx);
+    double b2 = -xyy * Xib;
+    b2 = fma(-xxy, Xjb, b2);
+    b2 = fma(sxy, Yib, b2);
+    b2 = fma(sxx, Yjb, b2);
+    b2 = fma(-yyz, Zib, b2);
+    b2 = fma(-yxz, Zjb, b2);
+    b2 = fma(lm, b2, -dy);
+    double b3 = -xyz * Xib;
+    b3 = fma(-xxz, Xjb, b3);
+    b3 = fma(-yyz, Yib, b3);
+    b3 = fma(-yxz, Yjb, b3);
+    b3 = fma(txy, Zib, b3);
+    b3 = fma(txx, Zjb, b3);
+    b3 = fma(lm, b3, -dz);
+    const double inv = 1.0 / det;
+    const double Xs = fma(E13, b3, fma(E12, b2, E11 * b1)) * inv;
+    const double Ys = fma(E23, b3, fma(E22, b2, E12 * b1)) * inv;
+    const double Zs = fma(E33, b3, fma(E23, b2, E13 * b1)) * inv;
+    if (std::isfinite(Xs) && std::isfinite(Ys) && std::isfinite(Zs)) {
+        oX = Xs;
+        oY = Ys;
+        oZ = Zs;
+    } else {
+        oX = X0;
+        oY = Y0;
+        oZ = Z0;
+    }
+}
+
+// The FP64 triangulation mode: the normal equations of Solve3dMinimalAreaKernel
+// (solve3dMinimalArea.cu:21-185) built and solved in FP64 with fused multiply-adds, on T-typed
+// fields (inputs widened, outputs rounded). Same terms as solve3d; the evaluation differs: the
+// 4x3 products and all sums of products are FMA chains, the symmetric normal matrix and its
+// adjugate are formed once (a21 = a12, a31 = a13, a32 = a23 hold bit for bit in solve3d too),
+// the determinant is the cofactor expansion along the first row, and the solution is the
+// adjugate times b scaled by one reciprocal of the determinant. Differs from solve3d<double> by
+// FP64 rounding only (pinned against it in the tests).
+template <typename T>
+void solve3d_fma(const T* uproj0, const T* vproj0, const T* X0, const T* Y0, const T* Z0, const double* P,
+                 const double* Q, T lambdaf_t, T lambdams_t, int w, int h, int s, T* X, T* Y, T* Z) {
+    const double lambdaf = (double)lambdaf_t, lambdams = (double)lambdams_t;
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            const long pos = ix + (long)iy * s;
+            const bool hl = ix > 0, hu = iy > 0;
+            const long pl = hl ? pos - 1 : pos, pu = hu ? pos - s : pos;
+            double oX, oY, oZ;
+            tri_fma(P, Q, lambdaf, lambdams, ix, iy, (double)uproj0[pos], (double)vproj0[pos], (double)X0[pos],
+                    (double)Y0[pos], (double)Z0[pos], (double)X0[pos + 1], (double)Y0[pos + 1], (double)Z0[pos + 1],
+                    (double)X0[pos + s], (double)Y0[pos + s], (double)Z0[pos + s], (double)X0[pl], (double)Y0[pl],
+                    (double)Z0[pl], (double)X0[pu], (double)Y0[pu], (double)Z0[pu], hl, hu, oX, oY, oZ);
+            X[pos] = (T)oX;
+            Y[pos] = (T)oY;
+            Z[pos] = (T)oZ;
+        }
+}
+
 // A16: CleanUp3DKernel (solve3dMinimalArea.cu:248-297).
 template <typename T>
 void cleanup3d(const T* X0, const T* Y0, const T* Z0, const T* Xm, const T* Ym, const T* Zm, int w, int h,
@@ -937,27 +1045,11 @@ struct Params {
     int tri_f64;       // 1: the triangulation step runs in FP64 whatever T is
 };
 
-// Solve3dMinimalArea evaluated in FP64 on T-typed fields (inputs widened, outputs rounded).
+// The triangulation step of the FP64-triangulation mode on T-typed fields: solve3d_fma.
 template <typename T>
 void solve3d_wide(const T* up, const T* vp, const T* X0, const T* Y0, const T* Z0, const double* P,
-                  const double* Q, T lambdaf, T lambdams, int w, int h, int s, long n, T* X, T* Y, T* Z) {
-    std::vector<double> a(n), b(n), c(n), d(n), e(n), o1(n), o2(n), o3(n);
-    for (long i = 0; i < n; ++i) {
-        a[i] = up[i];
-        b[i] = vp[i];
-        c[i] = X0[i];
-        d[i] = Y0[i];
-        e[i] = Z0[i];
-    }
-    solve3d<double>(a.data(), b.data(), c.data(), d.data(), e.data(), P, Q, (double)lambdaf, (double)lambdams, w,
-                    h, s, o1.data(), o2.data(), o3.data());
-    for (int iy = 0; iy < h; ++iy)
-        for (int ix = 0; ix < w; ++ix) {
-            long p = ix + (long)iy * s;
-            X[p] = (T)o1[p];
-            Y[p] = (T)o2[p];
-            Z[p] = (T)o3[p];
-        }
+                  const double* Q, T lambdaf, T lambdams, int w, int h, int s, T* X, T* Y, T* Z) {
+    solve3d_fma<T>(up, vp, X0, Y0, Z0, P, Q, lambdaf, lambdams, w, h, s, X, Y, Z);
 }
 
 // iAlignUp (Cudaflow.cu:985-994).
@@ -1259,7 +1351,7 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                 std::swap(pv2, pv2s);
                 uvproj(u, v, dumed, dvmed, uc, vc, sku, skv, w, h, s, lambdaf, alphaProj, uproj, vproj);
                 if (P.tri_f64)
-                    solve3d_wide(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, n + s0, Xs, Ys, Zs);
+                    solve3d_wide(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, Xs, Ys, Zs);
                 else
                     solve3d(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, Xs, Ys, Zs);
                 std::swap(X, Xs);
@@ -1383,6 +1475,11 @@ void orc_gray_u16_f32(const uint16_t* in, float* out, int w, int h, int s) { orc
     void orc_solve3d_##SUF(const T* up, const T* vp, const T* X0, const T* Y0, const T* Z0, const double* P,  \
                            const double* Q, float lf, float lms, int w, int h, int s, T* X, T* Y, T* Z) {    \
         orc::solve3d<T>(up, vp, X0, Y0, Z0, P, Q, (T)lf, (T)lms, w, h, s, X, Y, Z);                          \
+    }                                                                                                        \
+    void orc_solve3d_fma_##SUF(const T* up, const T* vp, const T* X0, const T* Y0, const T* Z0,             \
+                               const double* P, const double* Q, float lf, float lms, int w, int h, int s,   \
+                               T* X, T* Y, T* Z) {                                                           \
+        orc::solve3d_fma<T>(up, vp, X0, Y0, Z0, P, Q, (T)lf, (T)lms, w, h, s, X, Y, Z);                      \
     }                                                                                                        \
     void orc_cleanup3d_##SUF(const T* X0, const T* Y0, const T* Z0, const T* Xm, const T* Ym, const T* Zm,   \
                              int w, int h, int s, T* X1, T* Y1, T* Z1) {                                     \

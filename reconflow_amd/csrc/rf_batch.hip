@@ -414,9 +414,9 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
         const int e = ex + ey * EX;
         float oX, oY, oZ;
         if (F64TRI)
-            solve3d_px<double>(F.pqd[l], (double)a.lf, (double)a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q],
-                               X0[q + 1], Y0[q + 1], Z0[q + 1], X0[q + s], Y0[q + s], Z0[q + s], X0[ql], Y0[ql],
-                               Z0[ql], X0[qu], Y0[qu], Z0[qu], hl, hu, oX, oY, oZ);
+            solve3d_px_f64(F.pqd[l], (double)a.lf, (double)a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q],
+                           X0[q + 1], Y0[q + 1], Z0[q + 1], X0[q + s], Y0[q + s], Z0[q + s], X0[ql], Y0[ql], Z0[ql],
+                           X0[qu], Y0[qu], Z0[qu], hl, hu, oX, oY, oZ);
         else
             solve3d_px<float>(F.pq[l], a.lf, a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q], X0[q + 1],
                               Y0[q + 1], Z0[q + 1], X0[q + s], Y0[q + s], Z0[q + s], X0[ql], Y0[ql], Z0[ql], X0[qu],
@@ -570,8 +570,8 @@ __device__ __forceinline__ JointFront joint_front(const IterArgs& a, const Joint
     // ---- triangulation
     float oX, oY, oZ;
     if (F64TRI)
-        solve3d_px<double>(*V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd,
-                           Xl, Yl, Zl, Xu, Yu, Zu, hl, hu, oX, oY, oZ);
+        solve3d_px_f64(*V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl,
+                       Yl, Zl, Xu, Yu, Zu, hl, hu, oX, oY, oZ);
     else
         solve3d_px<float>(*V.pq, a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
                           Yu, Zu, hl, hu, oX, oY, oZ);

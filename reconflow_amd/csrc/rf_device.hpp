@@ -336,6 +336,92 @@ __device__ __forceinline__ void solve3d_px(const PQT<R>& m, R lambdaf, R lambdam
     }
 }
 
+// The FP64 triangulation mode for one pixel: the normal equations of Solve3dMinimalAreaKernel
+// (solve3dMinimalArea.cu:21-185) built and solved in FP64 with fused multiply-adds. Same terms
+// as solve3d_px; the 4x3 products and every sum of products are FMA chains, the symmetric normal
+// matrix and its adjugate are formed once, the determinant is the cofactor expansion along the
+// first row and the solution is adjugate times b times one reciprocal of the determinant.
+// Neighbours and the fallback as in solve3d_px.
+__device__ __forceinline__ void solve3d_px_f64(const PQd& m, double lambdaf, double lambdams, int ix, int iy,
+                                               float uproj_f, float vproj_f, float X0f, float Y0f, float Z0f,
+                                               float Xrf, float Yrf, float Zrf, float Xdf, float Ydf, float Zdf,
+                                               float Xlf, float Ylf, float Zlf, float Xuf, float Yuf, float Zuf,
+                                               bool hl, bool hu, float& oX, float& oY, float& oZ) {
+    const double* P = m.P;
+    const double* Q = m.Q;
+    const double x1 = (double)ix, y1 = (double)iy, x2 = (double)ix + (double)uproj_f,
+                 y2 = (double)iy + (double)vproj_f;
+    double pp[4], qq[4], rr[4], ss[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        pp[k] = fma(x1, P[8 + k], -P[k]);
+        qq[k] = fma(y1, P[8 + k], -P[4 + k]);
+        rr[k] = fma(x2, Q[8 + k], -Q[k]);
+        ss[k] = fma(y2, Q[8 + k], -Q[4 + k]);
+    }
+    auto dot = [&](int i, int j) {
+        return lambdaf * fma(ss[i], ss[j], fma(rr[i], rr[j], fma(qq[i], qq[j], pp[i] * pp[j])));
+    };
+    const double ax = dot(0, 0), bx = dot(0, 1), cx = dot(0, 2), dx = dot(0, 3);
+    const double by = dot(1, 1), cy = dot(1, 2), dy = dot(1, 3);
+    const double cz = dot(2, 2), dz = dot(2, 3);
+    const double X0 = X0f, Y0 = Y0f, Z0 = Z0f, Xr = Xrf, Yr = Yrf, Zr = Zrf, Xd = Xdf, Yd = Ydf, Zd = Zdf;
+    const double Xx = Xr - X0, Yx = Yr - Y0, Zx = Zr - Z0;
+    const double Xy = Xd - X0, Yy = Yd - Y0, Zy = Zd - Z0;
+    const double c1 = fma(Yx, Zy, -(Zx * Yy)), c2 = fma(Zx, Xy, -(Xx * Zy)), c3 = fma(Xx, Yy, -(Yx * Xy));
+    const double lm = lambdams / sqrt(fma(c3, c3, fma(c2, c2, c1 * c1)) + (double)0.001f);
+    const double Xib = hl ? fma(0.5, (double)Xlf, 0.5 * Xr) : Xr, Yib = hl ? fma(0.5, (double)Ylf, 0.5 * Yr) : Yr,
+                 Zib = hl ? fma(0.5, (double)Zlf, 0.5 * Zr) : Zr;
+    const double Xjb = hu ? fma(0.5, (double)Xuf, 0.5 * Xd) : Xd, Yjb = hu ? fma(0.5, (double)Yuf, 0.5 * Yd) : Yd,
+                 Zjb = hu ? fma(0.5, (double)Zuf, 0.5 * Zd) : Zd;
+    const double xxy = Xx * Yx, xyy = Xy * Yy, xxz = Xx * Zx, xyz = Xy * Zy, yxz = Yx * Zx, yyz = Yy * Zy;
+    const double syx = fma(Zx, Zx, Yx * Yx), syy = fma(Zy, Zy, Yy * Yy);
+    const double sxx = fma(Zx, Zx, Xx * Xx), sxy = fma(Zy, Zy, Xy * Xy);
+    const double txx = fma(Yx, Yx, Xx * Xx), txy = fma(Yy, Yy, Xy * Xy);
+    const double a11 = fma(lm, syx + syy, ax), a22 = fma(lm, sxx + sxy, by), a33 = fma(lm, txx + txy, cz);
+    const double a12 = fma(-lm, fma(Xy, Yy, xxy), bx), a13 = fma(-lm, fma(Xy, Zy, xxz), cx),
+                 a23 = fma(-lm, fma(Yy, Zy, yxz), cy);
+    const double E11 = fma(a22, a33, -(a23 * a23)), E12 = fma(a13, a23, -(a12 * a33)),
+                 E13 = fma(a12, a23, -(a13 * a22));
+    const double E22 = fma(a11, a33, -(a13 * a13)), E23 = fma(a13, a12, -(a11 * a23)),
+                 E33 = fma(a11, a22, -(a12 * a12));
+    const double det = fma(a13, E13, fma(a12, E12, a11 * E11));
+    double b1 = syy * Xib;
+    b1 = fma(syx, Xjb, b1);
+    b1 = fma(-xyy, Yib, b1);
+    b1 = fma(-xxy, Yjb, b1);
+    b1 = fma(-xyz, Zib, b1);
+    b1 = fma(-xxz, Zjb, b1);
+    b1 = fma(lm, b1, -dx);
+    double b2 = -xyy * Xib;
+    b2 = fma(-xxy, Xjb, b2);
+    b2 = fma(sxy, Yib, b2);
+    b2 = fma(sxx, Yjb, b2);
+    b2 = fma(-yyz, Zib, b2);
+    b2 = fma(-yxz, Zjb, b2);
+    b2 = fma(lm, b2, -dy);
+    double b3 = -xyz * Xib;
+    b3 = fma(-xxz, Xjb, b3);
+    b3 = fma(-yyz, Yib, b3);
+    b3 = fma(-yxz, Yjb, b3);
+    b3 = fma(txy, Zib, b3);
+    b3 = fma(txx, Zjb, b3);
+    b3 = fma(lm, b3, -dz);
+    const double inv = 1.0 / det;
+    const double Xs = fma(E13, b3, fma(E12, b2, E11 * b1)) * inv;
+    const double Ys = fma(E23, b3, fma(E22, b2, E12 * b1)) * inv;
+    const double Zs = fma(E33, b3, fma(E23, b2, E13 * b1)) * inv;
+    if (isfinite(Xs) && isfinite(Ys) && isfinite(Zs)) {
+        oX = (float)Xs;
+        oY = (float)Ys;
+        oZ = (float)Zs;
+    } else {
+        oX = X0f;
+        oY = Y0f;
+        oZ = Z0f;
+    }
+}
+
 // CleanUp3DKernel for one pixel (solve3dMinimalArea.cu:258-295).
 __device__ __forceinline__ void cleanup_px(float X, float Y, float Z, float xm, float ym, float zm, float& oX,
                                            float& oY, float& oZ) {

@@ -229,9 +229,14 @@ __global__ void __launch_bounds__(256) k_solve3d(const float* up, const float* v
     bool hl = ix > 0, hu = iy > 0;
     long pl = hl ? pos - 1 : pos, pu = hu ? pos - s : pos;
     float oX, oY, oZ;
-    solve3d_px<R>(pq, (R)lf, (R)lms, ix, iy, up[pos], vp[pos], X0[pos], Y0[pos], Z0[pos], X0[pos + 1], Y0[pos + 1],
-               Z0[pos + 1], X0[pos + s], Y0[pos + s], Z0[pos + s], X0[pl], Y0[pl], Z0[pl], X0[pu], Y0[pu], Z0[pu],
-               hl, hu, oX, oY, oZ);
+    if constexpr (sizeof(R) == 8)  // the FP64 triangulation mode
+        solve3d_px_f64(pq, (double)lf, (double)lms, ix, iy, up[pos], vp[pos], X0[pos], Y0[pos], Z0[pos],
+                       X0[pos + 1], Y0[pos + 1], Z0[pos + 1], X0[pos + s], Y0[pos + s], Z0[pos + s], X0[pl], Y0[pl],
+                       Z0[pl], X0[pu], Y0[pu], Z0[pu], hl, hu, oX, oY, oZ);
+    else
+        solve3d_px<R>(pq, (R)lf, (R)lms, ix, iy, up[pos], vp[pos], X0[pos], Y0[pos], Z0[pos], X0[pos + 1],
+                      Y0[pos + 1], Z0[pos + 1], X0[pos + s], Y0[pos + s], Z0[pos + s], X0[pl], Y0[pl], Z0[pl], X0[pu],
+                      Y0[pu], Z0[pu], hl, hu, oX, oY, oZ);
     X[pos] = oX;
     Y[pos] = oY;
     Z[pos] = oZ;

@@ -433,6 +433,31 @@ def test_solve3d_regulariser_fixes_affine_surfaces(orc, rng):
         np.testing.assert_allclose(get(o, W, H)[core], ref[core], rtol=1e-9, atol=1e-9)
 
 
+def test_solve3d_fma_tracks_plain_f64(orc, rng):
+    # the FP64-triangulation mode evaluates the same normal equations with FMA chains, a
+    # symmetric adjugate and one reciprocal of the determinant: on the same float inputs its
+    # FP32 output is the plain FP64 restatement's result rounded to FP32 (the two differ by FP64
+    # rounding only). lambdaf > 0 in every case: the minimal-area term alone is rank deficient
+    # and its solve is arbitrary in any evaluation order.
+    K = np.array([[500.0, 0, 320], [0, 500, 240], [0, 0, 1]])
+    P, Q = _PQ(K, _rot([0.2, 1, 0.1], 0.02), np.array([0.1, 0.01, 0.0]))
+    up, vp = (rng.standard_normal((H, W)) * 0.5 for _ in range(2))
+    Xs = [np.full((H, W), c) + 0.05 * rng.standard_normal((H, W)) for c in (0.3, -0.2, 5.0)]
+    for lf, lms in ((0.1, 100.0), (0.1, 0.0), (1.0, 10.0), (0.01, 1000.0)):
+        a32 = [put(x, "f32") for x in (up, vp, *Xs)]
+        a64 = [put(x.astype(np.float32).astype(np.float64), "f64") for x in (up, vp, *Xs)]
+        o32 = [field("f32") for _ in range(3)]
+        o64 = [field("f64") for _ in range(3)]
+        orc.call("solve3d_fma", "f32", *a32, P.ravel().copy(), Q.ravel().copy(), lf, lms, W, H, S, *o32)
+        orc.call("solve3d", "f64", *a64, P.ravel().copy(), Q.ravel().copy(), lf, lms, W, H, S, *o64)
+        for g, r in zip(o32, o64):
+            g, r = get(g, W, H).astype(np.float64), get(r, W, H)
+            # within one FP32 ulp of the FP64 value everywhere (half an ulp: correctly rounded)
+            ulp = np.spacing(np.abs(r).astype(np.float32)).astype(np.float64)
+            d = np.abs(g - r) / ulp
+            assert d.max() <= 1.0, (lf, lms, d.max())
+
+
 # ---------------------------------------------------------------- f32 restatement vs f64
 @pytest.mark.parametrize("name", ["data_l1", "dual_tv", "recon_data_l1", "solve3d"])
 def test_f32_restatement_tracks_f64(orc, rng, name):
