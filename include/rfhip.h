@@ -100,14 +100,18 @@ typedef struct rf_params {
 } rf_params;
 
 /* Tilings of the fused joint iteration (all give the same results). AUTO: the first joint solve
- * times 16x16, 16x16_OCC5 and 32x16 on alternating warps of every pyramid level and keeps the
- * fastest per level (with fewer than 3 warps, the first ones of that list; with one warp,
- * 16x16). That first solve synchronises. */
+ * times 16x16, 32x16, 16x16_K2, 16x16_OCC5 and 16x16_K3 on alternating warps of every pyramid
+ * level and keeps the fastest per level (with fewer than 5 warps, the first ones of that list;
+ * with one warp, 16x16). That first solve synchronises. A K2/K3 launch runs two/three solver
+ * iterations on an extended tile that recomputes a ring of 3/5 pixels (for latency-bound small
+ * levels); an iteration count that K does not divide ends with one-iteration launches. */
 #define RF_JOINT_TILE_AUTO 0
 #define RF_JOINT_TILE_64x8_RPT2 1  /* 64x4 lanes, two rows per lane plus a ring pass */
 #define RF_JOINT_TILE_32x16 2      /* one pixel per lane, 30x14 pixels per workgroup */
 #define RF_JOINT_TILE_16x16 3      /* one pixel per lane, 14x14 pixels per workgroup */
 #define RF_JOINT_TILE_16x16_OCC5 4 /* as 16x16, registers capped for five waves per SIMD */
+#define RF_JOINT_TILE_16x16_K2 5   /* two iterations per launch, 10x10 pixels per 16x16 workgroup */
+#define RF_JOINT_TILE_16x16_K3 6   /* three iterations per launch, 6x6 pixels per 16x16 workgroup */
 
 /* Tilings of the fused TV-L1 iteration (all give the same results). AUTO: the first solve times
  * 16x16, 32x16 and 64x16_RPT4 on alternating warps of every level and keeps the fastest per level

@@ -519,93 +519,113 @@ struct JointFront {
     float p1u, p2u, p1v, p2v, uu, vv, sku, skv, nduh, ndvh, upn, vpn, oX, oY, oZ;
 };
 
-template <bool F64TRI, bool PRIOR, typename TS>
-__device__ __forceinline__ JointFront joint_front(const IterArgs& a, const JointView& V, int xc, int yc) {
+// What the front half reads at a pixel: its own state, the duals to the left and above and X0 at
+// the four neighbours (X0 to the left and above is the pixel's own on column 0 and row 0).
+struct JointIn {
+    float duh, dvh, p1u, p2u, p1v, p2v, p1ul, p1vl, p2ud, p2vd;
+    float uu, vv, sku, skv, up0, vp0, uc0, vc0, ix, iy, iz;
+    float Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu, Yu, Zu;
+    float ufn, vfn, msk;
+};
+
+template <bool PRIOR, typename TS>
+__device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView& V, int xc, int yc) {
     const int s = V.s;
     const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
-
-    // ---- loads (unconditional: on the first iteration of a warp the view's uhat and dual
-    // inputs are the zero field)
-    const float duh = el(V.du0, q), dvh = el(V.dv0, q);
-    const float p1u = el(V.p1u0, q), p2u = el(V.p2u0, q), p1v = el(V.p1v0, q), p2v = el(V.p2v0, q);
-    const float p1ul = el(V.p1u0, ql), p1vl = el(V.p1v0, ql), p2ud = el(V.p2u0, qu), p2vd = el(V.p2v0, qu);
-    const float uu = el(V.U, q), vv = el(V.V, q);
-    const float sku = el(V.sku0, q), skv = el(V.skv0, q), up0 = el(V.up0, q), vp0 = el(V.vp0, q);
-    const float uc0 = el(V.uc0, q), vc0 = el(V.vc0, q);
-    const float ix_ = (float)el(as<TS>(V.Ix), q), iy_ = (float)el(as<TS>(V.Iy), q), iz_ = (float)el(as<TS>(V.Iz), q);
+    // unconditional loads: on the first iteration of a warp the view's uhat and dual inputs are
+    // the zero field
+    JointIn in;
+    in.duh = el(V.du0, q), in.dvh = el(V.dv0, q);
+    in.p1u = el(V.p1u0, q), in.p2u = el(V.p2u0, q), in.p1v = el(V.p1v0, q), in.p2v = el(V.p2v0, q);
+    in.p1ul = el(V.p1u0, ql), in.p1vl = el(V.p1v0, ql), in.p2ud = el(V.p2u0, qu), in.p2vd = el(V.p2v0, qu);
+    in.uu = el(V.U, q), in.vv = el(V.V, q);
+    in.sku = el(V.sku0, q), in.skv = el(V.skv0, q), in.up0 = el(V.up0, q), in.vp0 = el(V.vp0, q);
+    in.uc0 = el(V.uc0, q), in.vc0 = el(V.vc0, q);
+    in.ix = (float)el(as<TS>(V.Ix), q), in.iy = (float)el(as<TS>(V.Iy), q), in.iz = (float)el(as<TS>(V.Iz), q);
     const unsigned qr = q + 1u, qd = q + (unsigned)s;
-    const float Xc = el(V.X0, q), Yc = el(V.Y0, q), Zc = el(V.Z0, q), Xr = el(V.X0, qr), Yr = el(V.Y0, qr),
-                Zr = el(V.Z0, qr);
-    const float Xd = el(V.X0, qd), Yd = el(V.Y0, qd), Zd = el(V.Z0, qd), Xl = el(V.X0, ql), Yl = el(V.Y0, ql),
-                Zl = el(V.Z0, ql);
-    const float Xu = el(V.X0, qu), Yu = el(V.Y0, qu), Zu = el(V.Z0, qu);
-    float ufn = 0.0f, vfn = 0.0f, msk = 1.0f;
+    in.Xc = el(V.X0, q), in.Yc = el(V.Y0, q), in.Zc = el(V.Z0, q);
+    in.Xr = el(V.X0, qr), in.Yr = el(V.Y0, qr), in.Zr = el(V.Z0, qr);
+    in.Xd = el(V.X0, qd), in.Yd = el(V.Y0, qd), in.Zd = el(V.Z0, qd);
+    in.Xl = el(V.X0, ql), in.Yl = el(V.Y0, ql), in.Zl = el(V.Z0, ql);
+    in.Xu = el(V.X0, qu), in.Yu = el(V.Y0, qu), in.Zu = el(V.Z0, qu);
+    in.ufn = 0.0f, in.vfn = 0.0f, in.msk = 1.0f;
     if (PRIOR) {
-        ufn = el(V.ufn, q);
-        vfn = el(V.vfn, q);
-        if (a.sparse) msk = el(V.mask, q);
+        in.ufn = el(V.ufn, q);
+        in.vfn = el(V.vfn, q);
+        if (a.sparse) in.msk = el(V.mask, q);
     }
+    return in;
+}
 
+template <bool F64TRI, bool PRIOR>
+__device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const JointView& V, int xc, int yc,
+                                                    const JointIn& in) {
+    const bool hl = xc > 0, hu = yc > 0;
     // ---- data step + projection blend
     float AA, bu, bv;
     if (PRIOR) {
-        const float aFn = a.sparse ? a.aFn * msk : a.aFn;
+        const float aFn = a.sparse ? a.aFn * in.msk : a.aFn;
         AA = (a.aTv + a.aP) + aFn;
-        bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, duh, up0, uu, sku, ufn);
-        bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, dvh, vp0, vv, skv, vfn);
+        bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.duh, in.up0, in.uu, in.sku, in.ufn);
+        bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.dvh, in.vp0, in.vv, in.skv, in.vfn);
     } else {
         AA = a.aTv + a.aP;
-        bu = recon_blend(a.aTv, a.aP, AA, duh, up0, uu, sku);
-        bv = recon_blend(a.aTv, a.aP, AA, dvh, vp0, vv, skv);
+        bu = recon_blend(a.aTv, a.aP, AA, in.duh, in.up0, in.uu, in.sku);
+        bv = recon_blend(a.aTv, a.aP, AA, in.dvh, in.vp0, in.vv, in.skv);
     }
     float du, dv;
-    l1_threshold_div(ix_, iy_, iz_, bu, bv, a.lambda, AA, du, dv);
-    const float nduh = du + div_p(p1u, p1ul, p2u, p2ud, hl, hu) / AA;
-    const float ndvh = dv + div_p(p1v, p1vl, p2v, p2vd, hl, hu) / AA;
-    const float upn = uvproj_px(a.lf, a.aP, uc0, uu, nduh, sku);
-    const float vpn = uvproj_px(a.lf, a.aP, vc0, vv, ndvh, skv);
+    l1_threshold_div(in.ix, in.iy, in.iz, bu, bv, a.lambda, AA, du, dv);
+    const float nduh = du + div_p(in.p1u, in.p1ul, in.p2u, in.p2ud, hl, hu) / AA;
+    const float ndvh = dv + div_p(in.p1v, in.p1vl, in.p2v, in.p2vd, hl, hu) / AA;
+    const float upn = uvproj_px(a.lf, a.aP, in.uc0, in.uu, nduh, in.sku);
+    const float vpn = uvproj_px(a.lf, a.aP, in.vc0, in.vv, ndvh, in.skv);
 
     // ---- triangulation
     float oX, oY, oZ;
     if (F64TRI)
-        solve3d_px_f64(*V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl,
-                       Yl, Zl, Xu, Yu, Zu, hl, hu, oX, oY, oZ);
+        solve3d_px_f64(*V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr,
+                       in.Zr, in.Xd, in.Yd, in.Zd, in.Xl, in.Yl, in.Zl, in.Xu, in.Yu, in.Zu, hl, hu, oX, oY, oZ);
     else
-        solve3d_px<float>(*V.pq, a.lf, a.lms, xc, yc, upn, vpn, Xc, Yc, Zc, Xr, Yr, Zr, Xd, Yd, Zd, Xl, Yl, Zl, Xu,
-                          Yu, Zu, hl, hu, oX, oY, oZ);
-    return JointFront{p1u, p2u, p1v, p2v, uu, vv, sku, skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
+        solve3d_px<float>(*V.pq, a.lf, a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr, in.Zr, in.Xd,
+                          in.Yd, in.Zd, in.Xl, in.Yl, in.Zl, in.Xu, in.Yu, in.Zu, hl, hu, oX, oY, oZ);
+    return JointFront{in.p1u, in.p2u, in.p1v, in.p2v, in.uu, in.vv, in.sku, in.skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
 }
 
-// The back half of a joint iteration at the in-image pixel (x, y), element q, whose
-// extended-tile slot is e (EXW slots per row; the tile's first interior pixel is (x0, y0) in slot
-// (1, 1)): dual step from uhat_n in LDS, 3x3 XYZ median of X'_n in LDS, cleanup, reprojection,
-// Sk update, stores.
-template <int EXW>
-__device__ __forceinline__ void joint_back(const IterArgs& a, const JointView& V, const UcParams& ucp,
-                                           const JointFront& J, int x, int y, int x0, int y0, int e, unsigned q,
-                                           const float* s_du, const float* s_dv, const float* s_X, const float* s_Y,
-                                           const float* s_Z) {
-    const int w = V.w, h = V.h;
+template <bool F64TRI, bool PRIOR, typename TS>
+__device__ __forceinline__ JointFront joint_front(const IterArgs& a, const JointView& V, int xc, int yc) {
+    return joint_compute<F64TRI, PRIOR>(a, V, xc, yc, joint_load<PRIOR, TS>(a, V, xc, yc));
+}
+
+// The back half of a joint iteration at the in-image pixel (x, y) whose extended-tile slot is e
+// (EXW slots per row; the slot of pixel (x + i, y + j) is e + i + j * EXW): dual step from
+// uhat_n in LDS, 3x3 XYZ median of X'_n in LDS, cleanup, reprojection, Sk update. CLAMP keeps
+// every LDS index inside the NE slots (for slots whose results are not used).
+struct JointBack {
+    float p1u, p2u, p1v, p2v, X, Y, Z, uc, vc, sku, skv;
+};
+
+template <int EXW, int NE, bool CLAMP>
+__device__ __forceinline__ JointBack joint_back_calc(const IterArgs& a, const UcParams& ucp, const JointFront& J,
+                                                     int x, int y, int w, int h, int e, const float* s_du,
+                                                     const float* s_dv, const float* s_X, const float* s_Y,
+                                                     const float* s_Z) {
+    auto at = [&](int i) { return CLAMP ? min(max(i, 0), NE - 1) : i; };
+    JointBack B;
     // ---- dual step
     const float fac = a.tau / a.theta;
     float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
     if (x + 1 < w) {
-        dux = s_du[e + 1] - J.nduh;
-        dvx = s_dv[e + 1] - J.ndvh;
+        dux = s_du[at(e + 1)] - J.nduh;
+        dvx = s_dv[at(e + 1)] - J.ndvh;
     }
     if (y + 1 < h) {
-        duy = s_du[e + EXW] - J.nduh;
-        dvy = s_dv[e + EXW] - J.ndvh;
+        duy = s_du[at(e + EXW)] - J.nduh;
+        dvy = s_dv[at(e + EXW)] - J.ndvh;
     }
-    float o1, o2;
-    dual_px(dux, duy, J.p1u, J.p2u, fac, o1, o2);
-    el(V.p1u1, q) = o1;
-    el(V.p2u1, q) = o2;
-    dual_px(dvx, dvy, J.p1v, J.p2v, fac, o1, o2);
-    el(V.p1v1, q) = o1;
-    el(V.p2v1, q) = o2;
+    dual_px(dux, duy, J.p1u, J.p2u, fac, B.p1u, B.p2u);
+    dual_px(dvx, dvy, J.p1v, J.p2v, fac, B.p1v, B.p2v);
 
     // ---- XYZ median, cleanup, reprojection, Sk update. Tap rule of medianFilter.cu:193-214 for
     // an interior pixel (taps lie within one pixel of it): a tap in column w or row h, or left
@@ -619,28 +639,37 @@ __device__ __forceinline__ void joint_back(const IterArgs& a, const JointView& V
         for (int i = 0; i < 3; ++i) {
             const int col = x + i - 1, row = y + j - 1;
             const bool zero = col == w || row == h || (col < 0 && row < 0);
-            const int ee = (max(col, 0) - x0 + 1) + (max(row, 0) - y0 + 1) * EXW;
+            const int ee = at(e + (max(col, 0) - x) + (max(row, 0) - y) * EXW);
             const int k = j * 3 + i;
             mX[k] = zero ? 0.0f : s_X[ee];
             mY[k] = zero ? 0.0f : s_Y[ee];
             mZ[k] = zero ? 0.0f : s_Z[ee];
         }
     const float xm = median9_finite(mX), ym = median9_finite(mY), zm = median9_finite(mZ);
-    float Xn, Yn, Zn;
-    cleanup_px(J.oX, J.oY, J.oZ, xm, ym, zm, Xn, Yn, Zn);
-    float ucn, vcn;
-    uc_px(ucp, Xn, Yn, Zn, x, y, ucn, vcn);
-    el(V.X1, q) = Xn;
-    el(V.Y1, q) = Yn;
-    el(V.Z1, q) = Zn;
-    el(V.uc1, q) = ucn;
-    el(V.vc1, q) = vcn;
+    cleanup_px(J.oX, J.oY, J.oZ, xm, ym, zm, B.X, B.Y, B.Z);
+    uc_px(ucp, B.X, B.Y, B.Z, x, y, B.uc, B.vc);
+    B.sku = sk_px(J.sku, J.uu, J.nduh, J.upn);
+    B.skv = sk_px(J.skv, J.vv, J.ndvh, J.vpn);
+    return B;
+}
+
+// Stores state n of a pixel (element q).
+__device__ __forceinline__ void joint_store(const JointView& V, unsigned q, const JointFront& J, const JointBack& B) {
+    el(V.p1u1, q) = B.p1u;
+    el(V.p2u1, q) = B.p2u;
+    el(V.p1v1, q) = B.p1v;
+    el(V.p2v1, q) = B.p2v;
+    el(V.X1, q) = B.X;
+    el(V.Y1, q) = B.Y;
+    el(V.Z1, q) = B.Z;
+    el(V.uc1, q) = B.uc;
+    el(V.vc1, q) = B.vc;
     el(V.du1, q) = J.nduh;
     el(V.dv1, q) = J.ndvh;
     el(V.up1, q) = J.upn;
     el(V.vp1, q) = J.vpn;
-    el(V.sku1, q) = sk_px(J.sku, J.uu, J.nduh, J.upn);
-    el(V.skv1, q) = sk_px(J.skv, J.vv, J.ndvh, J.vpn);
+    el(V.sku1, q) = B.sku;
+    el(V.skv1, q) = B.skv;
 }
 
 // One joint iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
@@ -683,7 +712,85 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
     // offset
     const unsigned q = (unsigned)x + (unsigned)y * (unsigned)s;
-    joint_back<EXW>(a, V, ucp, J, x, y, x0, y0, e, q, s_du, s_dv, s_X, s_Y, s_Z);
+    joint_store(V, q, J, joint_back_calc<EXW, NE, false>(a, ucp, J, x, y, w, h, e, s_du, s_dv, s_X, s_Y, s_Z));
+}
+
+// K joint iterations per launch (temporal blocking) on an EXW x EXH extended tile indexed by flat
+// element: lane (lx, ly) holds element E = (x0 - R + lx) + (y0 - R + ly) * s with R = 2K - 1, so
+// the lane to the right holds E + 1 and the lane below E + s, the elements the reference reads as
+// X0[pos + 1] and X0[pos + s] (solve3dMinimalArea.cu:65-80). At column w - 1 those reach the
+// padding or, where the stride has none, the first pixel of the next row, which is then a lane of
+// this tile like any other. Lanes whose element is a pixel iterate; the others (padding, the guard
+// row) keep their X0, which no kernel writes during a level and which is the same in both X
+// buffers, and are read only as those X0 neighbours. The results of iteration k are right on
+// lanes at least 2k - 1 lanes inside the tile (the front half reads one lane around, the back
+// half one more), so the launch stores its (EXW - 2R) x (EXH - 2R) interior, pixels x0.. and
+// y0..: state n + K into the other parity, one flip per launch. Same per-pixel arithmetic as
+// kb_joint_px.
+template <int EXW, int EXH, int K, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
+__global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
+    constexpr int R = 2 * K - 1, TW = EXW - 2 * R, TH = EXH - 2 * R, NE = EXW * EXH;
+    static_assert(TW > 0 && TH > 0, "tile too small for K iterations");
+    __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];                              // uhat_n, X'_n
+    __shared__ float s_p1u[NE], s_p2u[NE], s_p1v[NE], s_p2v[NE], s_Xn[NE], s_Yn[NE], s_Zn[NE];  // p_n, X_n
+    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
+    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
+    const JointView V =
+        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
+    const int w = V.w, h = V.h, s = V.s;
+    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int x0 = txy.x * TW, y0 = txy.y * TH;
+    const int lx = threadIdx.x, ly = threadIdx.y, e = lx + ly * EXW;
+    const int cx = x0 - R + lx, ry = y0 - R + ly;  // flat coordinates: element cx + ry * s
+    // the element's pixel (one wrap at most: |cx| < s)
+    const int pxw = cx >= s ? cx - s : (cx < 0 ? cx + s : cx);
+    const int pyw = cx >= s ? ry + 1 : (cx < 0 ? ry - 1 : ry);
+    const bool pix = pxw < w && pyw >= 0 && pyw < h;
+    const int xc = min(pxw, w - 1), yc = min(max(pyw, 0), h - 1);  // loads of other lanes stay in bounds
+    // X0 of an element that is not a pixel: padding or guard row; elements past the guard row or
+    // before the field are never read as neighbours
+    const int Ec = min(max(cx + ry * s, 0), (h + 1) * s - 1);
+    const float Xk = el(V.X0, (unsigned)Ec), Yk = el(V.Y0, (unsigned)Ec), Zk = el(V.Z0, (unsigned)Ec);
+    const UcParams ucp = *V.ucp;  // before any store: scalar loads
+    JointIn in = joint_load<PRIOR, TS>(a, V, xc, yc);
+    const bool hl = xc > 0, hu = yc > 0;
+    const int e_l = max(e - 1, 0), e_u = max(e - EXW, 0), e_r = min(e + 1, NE - 1), e_d = min(e + EXW, NE - 1);
+#pragma unroll 1
+    for (int k = 0;; ++k) {
+        const JointFront J = joint_compute<F64TRI, PRIOR>(a, V, xc, yc, in);
+        s_du[e] = J.nduh;
+        s_dv[e] = J.ndvh;
+        s_X[e] = J.oX;
+        s_Y[e] = J.oY;
+        s_Z[e] = J.oZ;
+        __syncthreads();
+        const JointBack B = joint_back_calc<EXW, NE, true>(a, ucp, J, xc, yc, w, h, e, s_du, s_dv, s_X, s_Y, s_Z);
+        if (k == K - 1) {
+            if (lx >= R && lx < EXW - R && ly >= R && ly < EXH - R && cx < w && ry < h)
+                joint_store(V, (unsigned)cx + (unsigned)ry * (unsigned)s, J, B);
+            return;
+        }
+        const float Xn = pix ? B.X : Xk, Yn = pix ? B.Y : Yk, Zn = pix ? B.Z : Zk;
+        s_p1u[e] = B.p1u;
+        s_p2u[e] = B.p2u;
+        s_p1v[e] = B.p1v;
+        s_p2v[e] = B.p2v;
+        s_Xn[e] = Xn;
+        s_Yn[e] = Yn;
+        s_Zn[e] = Zn;
+        __syncthreads();
+        // state n as the next iteration's inputs (what joint_load reads from memory)
+        in.duh = J.nduh, in.dvh = J.ndvh;
+        in.p1u = B.p1u, in.p2u = B.p2u, in.p1v = B.p1v, in.p2v = B.p2v;
+        in.p1ul = hl ? s_p1u[e_l] : B.p1u, in.p1vl = hl ? s_p1v[e_l] : B.p1v;
+        in.p2ud = hu ? s_p2u[e_u] : B.p2u, in.p2vd = hu ? s_p2v[e_u] : B.p2v;
+        in.sku = B.sku, in.skv = B.skv, in.up0 = J.upn, in.vp0 = J.vpn, in.uc0 = B.uc, in.vc0 = B.vc;
+        in.Xc = Xn, in.Yc = Yn, in.Zc = Zn;
+        in.Xr = s_Xn[e_r], in.Yr = s_Yn[e_r], in.Zr = s_Zn[e_r];
+        in.Xd = s_Xn[e_d], in.Yd = s_Yn[e_d], in.Zd = s_Zn[e_d];
+        in.Xl = hl ? s_Xn[e_l] : Xn, in.Yl = hl ? s_Yn[e_l] : Yn, in.Zl = hl ? s_Zn[e_l] : Zn;
+        in.Xu = hu ? s_Xn[e_u] : Xn, in.Yu = hu ? s_Yn[e_u] : Yn, in.Zu = hu ? s_Zn[e_u] : Zn;
+    }
 }
 
 // One TV-L1 iteration: data step (tile + right column + bottom row), dual step (tile).
@@ -887,7 +994,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         set_error("rf_batch_create: unknown input type");
         return RF_ERR_ARG;
     }
-    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_OCC5 ||
+    if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_K3 ||
         p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x16) {
         set_error("rf_batch_create: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
@@ -1039,9 +1146,14 @@ TileSet& Batch::tiles(int level, int cls) { return sets_[level * TC_COUNT + cls]
 // Iteration tiling of a level for a warp (TV-L1 or joint): a fixed choice; during the tuning
 // solve the candidates in turn (the first min(3, warps per level)); afterwards the tuned choice
 // (the first candidate until then).
-constexpr int kNumCand = 3;
-static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_F32, TC_FLOW}, {TC_J16, TC_J16O, TC_J32}};
-static int num_cand(int warps) { return warps < kNumCand ? warps : kNumCand; }
+constexpr int kNumCand = 5;
+static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_F32, TC_FLOW, TC_F16, TC_F16},
+                                           {TC_J16, TC_J32, TC_JK2, TC_J16O, TC_JK3}};
+static const int kCandCount[2] = {3, 5};
+static int num_cand(bool recon, int warps) {
+    const int c = kCandCount[recon ? 1 : 0];
+    return warps < c ? warps : c;
+}
 
 int Batch::tile_class(bool recon, int level, int warp) const {
     if (recon) {
@@ -1050,6 +1162,8 @@ int Batch::tile_class(bool recon, int level, int warp) const {
             case RF_JOINT_TILE_32x16: return TC_J32;
             case RF_JOINT_TILE_16x16: return TC_J16;
             case RF_JOINT_TILE_16x16_OCC5: return TC_J16O;
+            case RF_JOINT_TILE_16x16_K2: return TC_JK2;
+            case RF_JOINT_TILE_16x16_K3: return TC_JK3;
             default: break;
         }
     } else {
@@ -1061,7 +1175,7 @@ int Batch::tile_class(bool recon, int level, int warp) const {
         }
     }
     const int k = recon ? 1 : 0;
-    if (tuning_) return kTuneCand[k][warp % num_cand(p_.n_warp_iters)];
+    if (tuning_) return kTuneCand[k][warp % num_cand(recon, p_.n_warp_iters)];
     return tuned_[k] ? cls_[k][level] : kTuneCand[k][0];
 }
 
@@ -1071,6 +1185,8 @@ int Batch::joint_tiling(int level) const {
     switch (cls_[1][level]) {
         case TC_J32: return RF_JOINT_TILE_32x16;
         case TC_J16O: return RF_JOINT_TILE_16x16_OCC5;
+        case TC_JK2: return RF_JOINT_TILE_16x16_K2;
+        case TC_JK3: return RF_JOINT_TILE_16x16_K3;
         default: return RF_JOINT_TILE_16x16;
     }
 }
@@ -1089,7 +1205,7 @@ int Batch::flow_tiling(int level) const {
 // per (level, warp) with events; each level keeps its faster candidate. Its results are those of
 // any other solve.
 int Batch::tune_tiling(bool recon) {
-    const int L = p_.n_levels, nw = p_.n_warp_iters, nc = num_cand(nw);
+    const int L = p_.n_levels, nw = p_.n_warp_iters, nc = num_cand(recon, nw);
     tev_.assign(2 * L * nw, nullptr);
     int rc = RF_OK;
     for (auto& ev : tev_)
@@ -1257,6 +1373,12 @@ static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
         case TC_J16O:
             hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
+        case TC_JK2:
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 2, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            break;
+        case TC_JK3:
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 3, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            break;
         default:
             hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
@@ -1338,8 +1460,9 @@ hipError_t Batch::enqueue_t(bool recon) {
             if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, TC_PIX), level, pr.d);
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi)], st_));
-            for (int it = 0; it < p_.n_solver_iters; ++it) {
-                const int cls = tile_class(recon, level, wi);
+            for (int it = 0; it < p_.n_solver_iters;) {
+                int cls = tile_class(recon, level, wi);
+                if (kIters[cls] > p_.n_solver_iters - it) cls = TC_J16;  // the remainder, one per launch
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
@@ -1354,11 +1477,12 @@ hipError_t Batch::enqueue_t(bool recon) {
                         launch_joint<TS>(cls, n() == 1, p_.tri_f64 != 0, fn, st_, a);
                     RF_TRY(hipGetLastError());
                 }
-                pr.d ^= 1;
+                pr.d ^= 1;  // one flip per launch, whatever its iteration count
                 if (recon) {
                     pr.f ^= 1;
                     pr.x ^= 1;
                 }
+                it += kIters[cls];
             }
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi) + 1], st_));
             prof_.end(st_, RF_KID_FUSED_ITER, level, p_.n_solver_iters);

@@ -35,7 +35,8 @@ struct PairFields {
 
 // Tile shapes (pixels per workgroup) of the batched launches: pointwise 64x4; joint iteration
 // 64x8 (two rows per lane), 30x14 and 14x14 (one pixel per lane of a 32x16 / 16x16 extended
-// tile; TC_J16O: the 14x14 tiling compiled for occupancy rather than latency); TV-L1 iteration
+// tile; TC_J16O: the 14x14 tiling compiled for occupancy rather than latency; TC_JK2, TC_JK3: 10x10
+// and 6x6 pixels of a 16x16 extended tile, two and three iterations per launch); TV-L1 iteration
 // 64x16 (four rows per lane), 15x15 and 31x15 (one pixel per lane of a 16x16 / 32x16 extended
 // tile).
 enum TileClass {
@@ -47,10 +48,14 @@ enum TileClass {
     TC_J16O = 5,
     TC_F16 = 6,
     TC_F32 = 7,
-    TC_COUNT = 8
+    TC_JK2 = 8,
+    TC_JK3 = 9,
+    TC_COUNT = 10
 };
-constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14, 15, 31};
-constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14, 15, 15};
+constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14, 15, 31, 10, 6};
+constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14, 15, 15, 10, 6};
+// solver iterations one launch of the class runs
+constexpr int kIters[TC_COUNT] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 3};
 
 // One launch's tile list: (pair, tile index within the pair).
 struct TileSet {

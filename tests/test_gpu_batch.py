@@ -45,7 +45,7 @@ def test_batch_flow(orc, graph, tile):
             assert np.array_equal(u[:, :w], ru[:, :w]) and np.array_equal(v[:, :w], rv[:, :w]), (rep, i)
 
 
-@pytest.mark.parametrize("tile", [0, 2], ids=["auto", "32x16"])
+@pytest.mark.parametrize("tile", [0, 2, 5], ids=["auto", "32x16", "16x16-k2"])
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
 def test_batch_recon(orc, graph, tile):
     print(f"case: batch joint {len(SIZES)} pairs graph={graph} tile={tile}", flush=True)

@@ -193,7 +193,8 @@ def test_recon_paths_equal_oracle(orc, path, graph, tri, fix):
         assert _same(got, ref, 176), rep
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4], ids=["64x8-rpt2", "32x16", "16x16", "16x16-occ5"])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6],
+                         ids=["64x8-rpt2", "32x16", "16x16", "16x16-occ5", "16x16-k2", "16x16-k3"])
 @pytest.mark.parametrize("tri", [0, 1], ids=["tri32", "tri64"])
 def test_recon_joint_tilings_equal_oracle(orc, tile, tri):
     # every tiling of the fused joint iteration gives the restatement's bits (uneven level
@@ -204,6 +205,20 @@ def test_recon_joint_tilings_equal_oracle(orc, tile, tri):
     r = _recon(200, 152, 3, path=rf.PATH_FUSED, graph=True, tri=tri, nw=2, ni=6, joint_tile=tile)
     for rep in range(2):
         assert _same(_run_recon(r, d), ref, 200), rep
+
+
+@pytest.mark.parametrize("tile", [5, 6], ids=["16x16-k2", "16x16-k3"])
+@pytest.mark.parametrize("w,h", [(200, 152), (256, 152)], ids=["padded-strides", "unpadded-strides"])
+def test_recon_multi_iteration_launches_equal_oracle(orc, tile, w, h):
+    # two / three iterations per launch: 7 iterations leave a remainder run one per launch; at
+    # 256 px the strides have no padding, so the triangulation's read right of column w - 1 is
+    # the next row's first pixel, computed inside the same launch
+    print(f"case: joint {w}x{h} fused graph tile={tile} 7 iterations", flush=True)
+    d = rf.make_recon_pair(w, h, 13)
+    ref = _oracle_recon(orc, d, w, h, 3, "f32", tri=1, nw=2, ni=7)
+    r = _recon(w, h, 3, path=rf.PATH_FUSED, graph=True, tri=1, nw=2, ni=7, joint_tile=tile)
+    for rep in range(2):
+        assert _same(_run_recon(r, d), ref, w), rep
 
 
 @pytest.mark.parametrize("path", [rf.PATH_PLAIN, rf.PATH_FUSED], ids=["plain-graph", "fused-graph"])
