@@ -52,7 +52,7 @@ KID = {"fused_iter": 0, "data": 1, "dual": 2, "uvproj": 3, "solve3d": 4, "median
        "solve_uc": 7, "update_sk": 8}
 TILING = {0: "auto", 1: "64x8-rpt2", 2: "32x16", 3: "16x16", 4: "16x16-occ5", 5: "16x16-k2", 6: "16x16-k3"}
 ITERS_PER_LAUNCH = {"16x16-k2": 2, "16x16-k3": 3}
-FLOW_TILING = {0: "auto", 1: "64x16-rpt4", 2: "16x16", 3: "32x16"}
+FLOW_TILING = {0: "auto", 1: "64x16-rpt4", 2: "16x16", 3: "32x16", 4: "32x16-k4", 5: "32x32-k8"}
 
 
 def make_solver(rf, path, graph, tri, joint_tile=0):

@@ -99,10 +99,10 @@ typedef struct rf_params {
     int flow_tile;         /* fused TV-L1 iteration tiling: RF_FLOW_TILE_* (results are identical) */
 } rf_params;
 
-/* Tilings of the fused joint iteration (all give the same results). AUTO: the first joint solve
- * times 16x16, 32x16, 16x16_K2, 16x16_OCC5 and 16x16_K3 on alternating warps of every pyramid
- * level and keeps the fastest per level (with fewer than 5 warps, the first ones of that list;
- * with one warp, 16x16). That first solve synchronises. A K2/K3 launch runs two/three solver
+/* Tilings of the fused joint iteration (all give the same results). AUTO: the first joint solves
+ * time 16x16, 32x16, 16x16_K2, 16x16_OCC5 and 16x16_K3 on alternating warps of every pyramid
+ * level (as many solves as it takes to time each at least once) and keep the fastest per level.
+ * Those solves synchronise. A K2/K3 launch runs two/three solver
  * iterations on an extended tile that recomputes a ring of 3/5 pixels (for latency-bound small
  * levels); an iteration count that K does not divide ends with one-iteration launches. */
 #define RF_JOINT_TILE_AUTO 0
@@ -113,13 +113,18 @@ typedef struct rf_params {
 #define RF_JOINT_TILE_16x16_K2 5   /* two iterations per launch, 10x10 pixels per 16x16 workgroup */
 #define RF_JOINT_TILE_16x16_K3 6   /* three iterations per launch, 6x6 pixels per 16x16 workgroup */
 
-/* Tilings of the fused TV-L1 iteration (all give the same results). AUTO: the first solve times
- * 16x16, 32x16 and 64x16_RPT4 on alternating warps of every level and keeps the fastest per level
- * (with fewer than 3 warps, the first ones of that list). That first solve synchronises. */
+/* Tilings of the fused TV-L1 iteration (all give the same results). AUTO: the first solves time
+ * 16x16, 32x16_K4, 32x16, 32x32_K8 and 64x16_RPT4 on alternating warps of every level (as many
+ * solves as it takes to time each at least once) and keep the fastest per level. Those solves
+ * synchronise. A K4/K8 launch runs four/eight solver iterations on an extended tile that
+ * recomputes a ring of pixels; an iteration count that K does not divide ends with
+ * one-iteration launches. */
 #define RF_FLOW_TILE_AUTO 0
 #define RF_FLOW_TILE_64x16_RPT4 1 /* 64x4 lanes, four rows per lane plus a ring pass */
 #define RF_FLOW_TILE_16x16 2      /* one pixel per lane, 15x15 pixels per workgroup */
 #define RF_FLOW_TILE_32x16 3      /* one pixel per lane, 31x15 pixels per workgroup */
+#define RF_FLOW_TILE_32x16_K4 4   /* four iterations per launch, 25x9 pixels per 32x16 workgroup */
+#define RF_FLOW_TILE_32x32_K8 5   /* eight iterations per launch, 17x17 pixels per 32x32 workgroup */
 
 /* Fills p with the reference defaults (Cudaflow.cu:99-101, Reconflow.cu:27-30) for
  * width x height: 1 warp, 100 iterations, lambda 100, theta 0.33, tau 0.25, alignment 32,

@@ -942,6 +942,82 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
     el(V.dv1, q) = ndvh;
 }
 
+// K TV-L1 iterations per launch (temporal blocking): the EXW x EXH extended tile starts K - 1
+// pixels left of and above its (EXW - 2K + 1) x (EXH - 2K + 1) interior. Iteration k is right on
+// lanes k - 1 or more inside the tile on the left and top (the data step reads the duals to the
+// left and above) and k or more on the right and bottom (the dual step reads uhat to the right and
+// below). State between the iterations stays in registers and LDS; the duals are rounded to the
+// storage type between iterations as a store and reload would round them. The launch stores state
+// n + K into the other parity, one flip per launch. Same per-pixel arithmetic as kb_flow_px.
+template <int EXW, int EXH, int K, bool SINGLE, typename TS>
+__global__ void __launch_bounds__(EXW* EXH) kb_flow_multi(IterArgs a) {
+    constexpr int TW = EXW - 2 * K + 1, TH = EXH - 2 * K + 1, NE = EXW * EXH;
+    static_assert(TW > 0 && TH > 0, "tile too small for K iterations");
+    __shared__ float s_du[NE], s_dv[NE], s_p1u[NE], s_p2u[NE], s_p1v[NE], s_p2v[NE];
+    const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
+    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
+    const JointView V =
+        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
+    const int w = V.w, h = V.h, s = V.s;
+    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int lx = threadIdx.x, ly = threadIdx.y, e = lx + ly * EXW;
+    const int x = txy.x * TW - (K - 1) + lx, y = txy.y * TH - (K - 1) + ly;
+    const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
+    const bool hl = xc > 0, hu = yc > 0;
+    const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
+    const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
+    float duh = el(V.du0, q), dvh = el(V.dv0, q);
+    float p1u = (float)el(p1u0, q), p2u = (float)el(p2u0, q), p1v = (float)el(p1v0, q), p2v = (float)el(p2v0, q);
+    float p1ul = (float)el(p1u0, ql), p1vl = (float)el(p1v0, ql), p2ud = (float)el(p2u0, qu),
+          p2vd = (float)el(p2v0, qu);
+    const float ix_ = (float)el(as<TS>(V.Ix), q), iy_ = (float)el(as<TS>(V.Iy), q), iz_ = (float)el(as<TS>(V.Iz), q);
+    const float fac = a.tau / a.theta;
+    const int e_l = max(e - 1, 0), e_u = max(e - EXW, 0), e_r = min(e + 1, NE - 1), e_d = min(e + EXW, NE - 1);
+#pragma unroll 1
+    for (int k = 0;; ++k) {
+        float du, dv;
+        l1_threshold(ix_, iy_, iz_, duh, dvh, a.lambda, a.theta, du, dv);
+        const float nduh = du + a.theta * div_p(p1u, p1ul, p2u, p2ud, hl, hu);
+        const float ndvh = dv + a.theta * div_p(p1v, p1vl, p2v, p2vd, hl, hu);
+        s_du[e] = nduh;
+        s_dv[e] = ndvh;
+        __syncthreads();
+        float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
+        if (xc + 1 < w) {
+            dux = s_du[e_r] - nduh;
+            dvx = s_dv[e_r] - ndvh;
+        }
+        if (yc + 1 < h) {
+            duy = s_du[e_d] - nduh;
+            dvy = s_dv[e_d] - ndvh;
+        }
+        float o1u, o2u, o1v, o2v;
+        dual_px(dux, duy, p1u, p2u, fac, o1u, o2u);
+        dual_px(dvx, dvy, p1v, p2v, fac, o1v, o2v);
+        if (k == K - 1) {
+            if (lx >= K - 1 && lx < K - 1 + TW && ly >= K - 1 && ly < K - 1 + TH && x < w && y < h) {
+                el(as<TS>(V.p1u1), q) = st<TS>(o1u);
+                el(as<TS>(V.p2u1), q) = st<TS>(o2u);
+                el(as<TS>(V.p1v1), q) = st<TS>(o1v);
+                el(as<TS>(V.p2v1), q) = st<TS>(o2v);
+                el(V.du1, q) = nduh;
+                el(V.dv1, q) = ndvh;
+            }
+            return;
+        }
+        p1u = (float)st<TS>(o1u), p2u = (float)st<TS>(o2u), p1v = (float)st<TS>(o1v), p2v = (float)st<TS>(o2v);
+        s_p1u[e] = p1u;
+        s_p2u[e] = p2u;
+        s_p1v[e] = p1v;
+        s_p2v[e] = p2v;
+        __syncthreads();
+        duh = nduh, dvh = ndvh;
+        p1ul = hl ? s_p1u[e_l] : p1u, p1vl = hl ? s_p1v[e_l] : p1v;
+        p2ud = hu ? s_p2u[e_u] : p2u, p2vd = hu ? s_p2v[e_u] : p2v;
+    }
+}
+
 // Zero uhat (a warp with no solver iterations leaves the zeroed uhat of the reference).
 __global__ void __launch_bounds__(256) kb_zero_uhat(Launch L, int l, int pd) {
     BT_PIX(L, l)
@@ -995,7 +1071,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         return RF_ERR_ARG;
     }
     if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_K3 ||
-        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x16) {
+        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x32_K8) {
         set_error("rf_batch_create: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
     }
@@ -1144,16 +1220,12 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
 TileSet& Batch::tiles(int level, int cls) { return sets_[level * TC_COUNT + cls]; }
 
 // Iteration tiling of a level for a warp (TV-L1 or joint): a fixed choice; during the tuning
-// solve the candidates in turn (the first min(3, warps per level)); afterwards the tuned choice
-// (the first candidate until then).
+// solves the candidates in turn over the warps of as many solves as it takes to time each one;
+// afterwards the tuned choice (the first candidate until then).
 constexpr int kNumCand = 5;
-static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_F32, TC_FLOW, TC_F16, TC_F16},
+static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_FK4, TC_F32, TC_FK8, TC_FLOW},
                                            {TC_J16, TC_J32, TC_JK2, TC_J16O, TC_JK3}};
-static const int kCandCount[2] = {3, 5};
-static int num_cand(bool recon, int warps) {
-    const int c = kCandCount[recon ? 1 : 0];
-    return warps < c ? warps : c;
-}
+static const int kCandCount[2] = {5, 5};
 
 int Batch::tile_class(bool recon, int level, int warp) const {
     if (recon) {
@@ -1171,11 +1243,13 @@ int Batch::tile_class(bool recon, int level, int warp) const {
             case RF_FLOW_TILE_64x16_RPT4: return TC_FLOW;
             case RF_FLOW_TILE_16x16: return TC_F16;
             case RF_FLOW_TILE_32x16: return TC_F32;
+            case RF_FLOW_TILE_32x16_K4: return TC_FK4;
+            case RF_FLOW_TILE_32x32_K8: return TC_FK8;
             default: break;
         }
     }
     const int k = recon ? 1 : 0;
-    if (tuning_) return kTuneCand[k][warp % num_cand(recon, p_.n_warp_iters)];
+    if (tuning_) return kTuneCand[k][(tune_round_ * p_.n_warp_iters + warp) % kCandCount[k]];
     return tuned_[k] ? cls_[k][level] : kTuneCand[k][0];
 }
 
@@ -1197,6 +1271,8 @@ int Batch::flow_tiling(int level) const {
     switch (cls_[0][level]) {
         case TC_FLOW: return RF_FLOW_TILE_64x16_RPT4;
         case TC_F32: return RF_FLOW_TILE_32x16;
+        case TC_FK4: return RF_FLOW_TILE_32x16_K4;
+        case TC_FK8: return RF_FLOW_TILE_32x32_K8;
         default: return RF_FLOW_TILE_16x16;
     }
 }
@@ -1205,39 +1281,46 @@ int Batch::flow_tiling(int level) const {
 // per (level, warp) with events; each level keeps its faster candidate. Its results are those of
 // any other solve.
 int Batch::tune_tiling(bool recon) {
-    const int L = p_.n_levels, nw = p_.n_warp_iters, nc = num_cand(recon, nw);
+    const int L = p_.n_levels, nw = p_.n_warp_iters, k = recon ? 1 : 0;
+    const int nc = kCandCount[k], rounds = (nc + nw - 1) / nw;  // every candidate timed at least once
     tev_.assign(2 * L * nw, nullptr);
     int rc = RF_OK;
     for (auto& ev : tev_)
         if ((rc = check(hipEventCreate(&ev), "hipEventCreate")) != RF_OK) break;
-    if (!rc) {
+    std::vector<double> t((size_t)L * nc, 0.0);
+    std::vector<int> cnt((size_t)L * nc, 0);
+    for (int r = 0; r < rounds && !rc; ++r) {
+        tune_round_ = r;
         tuning_ = true;
         hipError_t e = enqueue(recon);
         tuning_ = false;
         rc = check(e, "tuning solve");
+        if (!rc) rc = check(hipStreamSynchronize(st_), "tuning solve");
+        for (int l = 0; l < L && !rc; ++l)
+            for (int wi = 0; wi < nw; ++wi) {
+                float ms = 0;
+                if (check(hipEventElapsedTime(&ms, tev_[2 * (l * nw + wi)], tev_[2 * (l * nw + wi) + 1]),
+                          "hipEventElapsedTime")) {
+                    rc = RF_ERR_HIP;
+                    break;
+                }
+                const int c = (r * nw + wi) % nc;
+                t[(size_t)l * nc + c] += ms;
+                ++cnt[(size_t)l * nc + c];
+            }
     }
-    if (!rc) rc = check(hipStreamSynchronize(st_), "tuning solve");
-    const int k = recon ? 1 : 0;
     cls_[k].assign(L, kTuneCand[k][0]);
     for (int l = 0; l < L && !rc; ++l) {
-        double t[kNumCand] = {};
-        int n[kNumCand] = {};
-        for (int wi = 0; wi < nw; ++wi) {
-            float ms = 0;
-            if (check(hipEventElapsedTime(&ms, tev_[2 * (l * nw + wi)], tev_[2 * (l * nw + wi) + 1]),
-                      "hipEventElapsedTime")) {
-                rc = RF_ERR_HIP;
-                break;
-            }
-            t[wi % nc] += ms;
-            ++n[wi % nc];
-        }
         double best = 0;
-        for (int c = 0; c < nc && !rc; ++c)
-            if (n[c] && (c == 0 || t[c] / n[c] < best)) {
-                best = t[c] / n[c];
+        bool any = false;
+        for (int c = 0; c < nc; ++c) {
+            const int i = l * nc + c;
+            if (cnt[i] && (!any || t[i] / cnt[i] < best)) {
+                best = t[i] / cnt[i];
                 cls_[k][l] = kTuneCand[k][c];
+                any = true;
             }
+        }
     }
     for (auto& ev : tev_)
         if (ev) (void)hipEventDestroy(ev);
@@ -1397,6 +1480,14 @@ static void launch_flow(int cls, bool single, hipStream_t st, const IterArgs& a)
             if (single) hipLaunchKernelGGL((kb_flow_px<32, 16, true, TS>), grid, dim3(32, 16), 0, st, a);
             else hipLaunchKernelGGL((kb_flow_px<32, 16, false, TS>), grid, dim3(32, 16), 0, st, a);
             break;
+        case TC_FK4:
+            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 16, 4, true, TS>), grid, dim3(32, 16), 0, st, a);
+            else hipLaunchKernelGGL((kb_flow_multi<32, 16, 4, false, TS>), grid, dim3(32, 16), 0, st, a);
+            break;
+        case TC_FK8:
+            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 32, 8, true, TS>), grid, dim3(32, 32), 0, st, a);
+            else hipLaunchKernelGGL((kb_flow_multi<32, 32, 8, false, TS>), grid, dim3(32, 32), 0, st, a);
+            break;
         default: hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), grid, dim3(TXW, TRW), 0, st, a); break;
     }
 }
@@ -1462,7 +1553,7 @@ hipError_t Batch::enqueue_t(bool recon) {
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi)], st_));
             for (int it = 0; it < p_.n_solver_iters;) {
                 int cls = tile_class(recon, level, wi);
-                if (kIters[cls] > p_.n_solver_iters - it) cls = TC_J16;  // the remainder, one per launch
+                if (kIters[cls] > p_.n_solver_iters - it) cls = recon ? TC_J16 : TC_F16;  // the remainder, one per launch
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
@@ -1516,7 +1607,7 @@ int Batch::solve(bool recon, float flow_scale) {
     }
     flow_scale_ = flow_scale;
     const bool auto_tile = recon ? p_.joint_tile == RF_JOINT_TILE_AUTO : p_.flow_tile == RF_FLOW_TILE_AUTO;
-    if (auto_tile && !tuned_[kind] && p_.n_warp_iters >= 2 && p_.n_solver_iters > 0 && !prof_.armed)
+    if (auto_tile && !tuned_[kind] && p_.n_warp_iters > 0 && p_.n_solver_iters > 0 && !prof_.armed)
         return tune_tiling(recon);
     if (!p_.use_graph || prof_.armed) {
         if (check(enqueue(recon), "batch solve")) return RF_ERR_HIP;

@@ -38,7 +38,7 @@ struct PairFields {
 // tile; TC_J16O: the 14x14 tiling compiled for occupancy rather than latency; TC_JK2, TC_JK3: 10x10
 // and 6x6 pixels of a 16x16 extended tile, two and three iterations per launch); TV-L1 iteration
 // 64x16 (four rows per lane), 15x15 and 31x15 (one pixel per lane of a 16x16 / 32x16 extended
-// tile).
+// tile), 25x9 and 17x17 (four / eight iterations per launch on a 32x16 / 32x32 extended tile).
 enum TileClass {
     TC_PIX = 0,
     TC_J64 = 1,
@@ -50,12 +50,14 @@ enum TileClass {
     TC_F32 = 7,
     TC_JK2 = 8,
     TC_JK3 = 9,
-    TC_COUNT = 10
+    TC_FK4 = 10,
+    TC_FK8 = 11,
+    TC_COUNT = 12
 };
-constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14, 15, 31, 10, 6};
-constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14, 15, 15, 10, 6};
+constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14, 15, 31, 10, 6, 25, 17};
+constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14, 15, 15, 10, 6, 9, 17};
 // solver iterations one launch of the class runs
-constexpr int kIters[TC_COUNT] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 3};
+constexpr int kIters[TC_COUNT] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 3, 4, 8};
 
 // One launch's tile list: (pair, tile index within the pair).
 struct TileSet {
@@ -128,6 +130,7 @@ class Batch {
     // results)
     std::vector<int> cls_[2];
     bool tuned_[2] = {false, false}, tuning_ = false;
+    int tune_round_ = 0;  // tuning solve in progress: candidate of warp w is (round * warps + w) mod count
     std::vector<hipEvent_t> tev_;
     hipGraphExec_t gexec_[2] = {nullptr, nullptr};
     Parity gfinal_[2];

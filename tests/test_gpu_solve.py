@@ -68,15 +68,17 @@ def test_flow_320x240_paths_equal_plain(tvl1_320, path, graph):
         assert np.array_equal(u[:, :320], u32[:, :320]) and np.array_equal(v[:, :320], v32[:, :320]), rep
 
 
+@pytest.mark.parametrize("ni", [8, 11], ids=["8-iterations", "11-iterations"])
 @pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16], ids=["f32", "f16"])
-def test_flow_tilings_equal(orc, storage):
+def test_flow_tilings_equal(orc, storage, ni):
     # every tiling of the fused TV-L1 iteration gives the same bits (uneven level sizes:
-    # 200x152 -> 100x76 -> 50x38, partial tiles on every level); FP32 storage equals the oracle
-    print(f"case: TV-L1 200x152 fused graph tilings storage={storage}", flush=True)
+    # 200x152 -> 100x76 -> 50x38, partial tiles on every level; 11 iterations leave a remainder
+    # after the four- and eight-iteration launches); FP32 storage equals the oracle
+    print(f"case: TV-L1 200x152 fused graph tilings storage={storage} iterations={ni}", flush=True)
     i0, i1, _, _ = rf.make_flow_pair(200, 152, 5)
     res = {}
-    for tile in (1, 2, 3, 0):
-        f = _flow(path=rf.PATH_FUSED, graph=True, w=200, h=152, L=3, nw=3, ni=8, storage=storage, flow_tile=tile)
+    for tile in (1, 2, 3, 4, 5, 0):
+        f = _flow(path=rf.PATH_FUSED, graph=True, w=200, h=152, L=3, nw=3, ni=ni, storage=storage, flow_tile=tile)
         for rep in range(2):
             f.copyImagesToDevice(i0, i1)
             f.solveOpticalFlow(5.0)
@@ -84,12 +86,12 @@ def test_flow_tilings_equal(orc, storage):
         if tile == 0:
             tl = (C.c_int * 3)()
             assert f._lib.rf_flow_tiling(f._ctx, tl, 3) == 0
-            assert all(t in (1, 2, 3) for t in tl), list(tl)
+            assert all(t in (1, 2, 3, 4, 5) for t in tl), list(tl)
     base = res[(1, 0)]
     for k, (u, v) in res.items():
         assert np.array_equal(u[:, :200], base[0][:, :200]) and np.array_equal(v[:, :200], base[1][:, :200]), k
     if storage == rf.STORAGE_F32:
-        ru, rv = orc.solve_flow(orc.flow_params(200, 152, 3, n_warp=3, n_iter=8), i0, i1, "f32")
+        ru, rv = orc.solve_flow(orc.flow_params(200, 152, 3, n_warp=3, n_iter=ni), i0, i1, "f32")
         assert np.array_equal(base[0][:, :200], ru[:, :200]) and np.array_equal(base[1][:, :200], rv[:, :200])
 
 
