@@ -266,6 +266,9 @@ int rf_batch_synchronize(rf_batch* b);
 int rf_batch_get_result_device(const rf_batch* b, int i, const float** u, const float** v, const float** X,
                                const float** Y, const float** Z);
 void* rf_batch_get_stream(const rf_batch* b);
+/* The iteration tiling per level for levels 0..n-1: RF_JOINT_TILE_* for a joint batch,
+ * RF_FLOW_TILE_* for a TV-L1 batch (AUTO until the tuning solves ran). */
+int rf_batch_tiling(const rf_batch* b, int* tiling, int n);
 
 /* ---- per-kernel entry points on device pointers (the public kernel wrappers of
  * Cudaflow.h:101-193 and Reconflow.h:86-158), enqueued on `stream` ---- */

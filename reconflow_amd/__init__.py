@@ -342,6 +342,12 @@ class Batch:
               "rf_batch_get_result_device")
         return [p.value or 0 for p in ptrs]
 
+    def tiling(self, n_levels: int):
+        """Iteration tiling per level (RF_JOINT_TILE_* or RF_FLOW_TILE_* codes)."""
+        tl = (C.c_int * n_levels)()
+        check(self._lib.rf_batch_tiling(self._b, tl, n_levels), "rf_batch_tiling")
+        return list(tl)
+
 
 def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50.0, theta=0.3, tau=0.25,
                  alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1, lambdams=100.0, use_graph=True, tri_f64=0,

@@ -78,6 +78,7 @@ _PROTOS = {
     "rf_batch_get_result_device": (_I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                         C.POINTER(_P)]),
     "rf_batch_get_stream": (_P, [_P]),
+    "rf_batch_tiling": (_I, [_P, C.POINTER(_I), _I]),
     # per-kernel entry points (device pointers as ints, stream last)
     "rf_k_rgb_to_gray": (_I, [_P, _P, _I, _I, _I, _P]),
     "rf_k_cv8u_to_gray": (_I, [_P, _P, _I, _I, _I, _P]),

@@ -1762,4 +1762,11 @@ int rf_batch_get_result_device(const rf_batch* b, int i, const float** u, const 
 
 void* rf_batch_get_stream(const rf_batch* b) { return b ? (void*)b->b.stream() : nullptr; }
 
+int rf_batch_tiling(const rf_batch* b, int* tiling, int n) {
+    if (!b || !tiling || n < 0) return RF_ERR_ARG;
+    const bool recon = b->b.params().method >= RF_METHODR_TVL1_MS;
+    for (int l = 0; l < n; ++l) tiling[l] = recon ? b->b.joint_tiling(l) : b->b.flow_tiling(l);
+    return RF_OK;
+}
+
 }  // extern "C"
