@@ -223,6 +223,25 @@ def test_recon_multi_iteration_launches_equal_oracle(orc, tile, w, h):
         assert _same(_run_recon(r, d), ref, w), rep
 
 
+def test_tiny_images_every_tiling(orc):
+    # levels smaller than one tile (40x24 -> 20x12 -> 10x6): every joint and TV-L1 tiling, the
+    # multi-iteration ones with iteration counts they do not divide
+    print("case: joint and TV-L1 40x24 every tiling", flush=True)
+    d = rf.make_recon_pair(40, 24, 17)
+    ref = _oracle_recon(orc, d, 40, 24, 3, "f32", tri=1, nw=2, ni=5)
+    for tile in range(1, 7):
+        r = _recon(40, 24, 3, path=rf.PATH_FUSED, graph=False, tri=1, nw=2, ni=5, joint_tile=tile)
+        assert _same(_run_recon(r, d), ref, 40), tile
+    i0, i1, _, _ = rf.make_flow_pair(40, 24, 18)
+    ru, rv = orc.solve_flow(orc.flow_params(40, 24, 3, n_warp=2, n_iter=9), i0, i1, "f32")
+    for tile in range(1, 6):
+        f = _flow(path=rf.PATH_FUSED, graph=False, w=40, h=24, L=3, nw=2, ni=9, flow_tile=tile)
+        f.copyImagesToDevice(i0, i1)
+        f.solveOpticalFlow(5.0)
+        u, v = f.copyOpticalFlowToHost()
+        assert np.array_equal(u[:, :40], ru[:, :40]) and np.array_equal(v[:, :40], rv[:, :40]), tile
+
+
 @pytest.mark.parametrize("path", [rf.PATH_PLAIN, rf.PATH_FUSED], ids=["plain-graph", "fused-graph"])
 def test_recon_graph_replay_follows_pose_changes(orc, path):
     # the captured schedule reads the per-level camera matrices from device memory: a new pose
