@@ -528,11 +528,12 @@ struct JointIn {
     float ufn, vfn, msk;
 };
 
-template <bool PRIOR, typename TS>
+// INNER: the pixel has neighbours on every side (not on column 0 or row 0), known at compile time.
+template <bool PRIOR, typename TS, bool INNER = false>
 __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView& V, int xc, int yc) {
     const int s = V.s;
     const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
-    const bool hl = xc > 0, hu = yc > 0;
+    const bool hl = INNER || xc > 0, hu = INNER || yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
     // unconditional loads: on the first iteration of a warp the view's uhat and dual inputs are
     // the zero field
@@ -559,10 +560,10 @@ __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView
     return in;
 }
 
-template <bool F64TRI, bool PRIOR>
+template <bool F64TRI, bool PRIOR, bool INNER = false>
 __device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const JointView& V, int xc, int yc,
                                                     const JointIn& in) {
-    const bool hl = xc > 0, hu = yc > 0;
+    const bool hl = INNER || xc > 0, hu = INNER || yc > 0;
     // ---- data step + projection blend
     float AA, bu, bv;
     if (PRIOR) {
@@ -593,20 +594,21 @@ __device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const Joi
     return JointFront{in.p1u, in.p2u, in.p1v, in.p2v, in.uu, in.vv, in.sku, in.skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
 }
 
-template <bool F64TRI, bool PRIOR, typename TS>
+template <bool F64TRI, bool PRIOR, typename TS, bool INNER = false>
 __device__ __forceinline__ JointFront joint_front(const IterArgs& a, const JointView& V, int xc, int yc) {
-    return joint_compute<F64TRI, PRIOR>(a, V, xc, yc, joint_load<PRIOR, TS>(a, V, xc, yc));
+    return joint_compute<F64TRI, PRIOR, INNER>(a, V, xc, yc, joint_load<PRIOR, TS, INNER>(a, V, xc, yc));
 }
 
 // The back half of a joint iteration at the in-image pixel (x, y) whose extended-tile slot is e
 // (EXW slots per row; the slot of pixel (x + i, y + j) is e + i + j * EXW): dual step from
 // uhat_n in LDS, 3x3 XYZ median of X'_n in LDS, cleanup, reprojection, Sk update. CLAMP keeps
-// every LDS index inside the NE slots (for slots whose results are not used).
+// every LDS index inside the NE slots (for slots whose results are not used). INNER: the pixel
+// and its eight neighbours are inside the image (no border rule applies), known at compile time.
 struct JointBack {
     float p1u, p2u, p1v, p2v, X, Y, Z, uc, vc, sku, skv;
 };
 
-template <int EXW, int NE, bool CLAMP>
+template <int EXW, int NE, bool CLAMP, bool INNER = false>
 __device__ __forceinline__ JointBack joint_back_calc(const IterArgs& a, const UcParams& ucp, const JointFront& J,
                                                      int x, int y, int w, int h, int e, const float* s_du,
                                                      const float* s_dv, const float* s_X, const float* s_Y,
@@ -616,11 +618,11 @@ __device__ __forceinline__ JointBack joint_back_calc(const IterArgs& a, const Uc
     // ---- dual step
     const float fac = a.tau / a.theta;
     float dux = 0.0f, dvx = 0.0f, duy = 0.0f, dvy = 0.0f;
-    if (x + 1 < w) {
+    if (INNER || x + 1 < w) {
         dux = s_du[at(e + 1)] - J.nduh;
         dvx = s_dv[at(e + 1)] - J.ndvh;
     }
-    if (y + 1 < h) {
+    if (INNER || y + 1 < h) {
         duy = s_du[at(e + EXW)] - J.nduh;
         dvy = s_dv[at(e + EXW)] - J.ndvh;
     }
@@ -638,8 +640,8 @@ __device__ __forceinline__ JointBack joint_back_calc(const IterArgs& a, const Uc
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const int col = x + i - 1, row = y + j - 1;
-            const bool zero = col == w || row == h || (col < 0 && row < 0);
-            const int ee = at(e + (max(col, 0) - x) + (max(row, 0) - y) * EXW);
+            const bool zero = !INNER && (col == w || row == h || (col < 0 && row < 0));
+            const int ee = INNER ? e + (i - 1) + (j - 1) * EXW : at(e + (max(col, 0) - x) + (max(row, 0) - y) * EXW);
             const int k = j * 3 + i;
             mX[k] = zero ? 0.0f : s_X[ee];
             mY[k] = zero ? 0.0f : s_Y[ee];
@@ -686,6 +688,33 @@ __device__ __forceinline__ void joint_store(const JointView& V, unsigned q, cons
 // OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
 // schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
 // the FP64 triangulation, met without spilling, for occupancy on large levels).
+// The body of kb_joint_px for the tile at (x0, y0). INNER: every lane of the extended tile is a pixel
+// at least one pixel inside the image, so no coordinate is clamped and no border rule applies.
+template <int EXW, int EXH, bool INNER, bool F64TRI, bool PRIOR, typename TS>
+__device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView& V, int x0, int y0, float* s_du,
+                                              float* s_dv, float* s_X, float* s_Y, float* s_Z) {
+    constexpr int NE = EXW * EXH;
+    const int w = V.w, h = V.h, s = V.s;
+    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
+    const int x = x0 - 1 + ex, y = y0 - 1 + ey;
+    const int xc = INNER ? x : min(max(x, 0), w - 1), yc = INNER ? y : min(max(y, 0), h - 1);
+    const JointFront J = joint_front<F64TRI, PRIOR, TS, INNER>(a, V, xc, yc);
+    s_du[e] = J.nduh;
+    s_dv[e] = J.ndvh;
+    s_X[e] = J.oX;
+    s_Y[e] = J.oY;
+    s_Z[e] = J.oZ;
+    __syncthreads();
+    // read before any global store of the kernel, so that it compiles to scalar loads
+    const UcParams ucp = *V.ucp;
+    if (ex == 0 || ey == 0 || ex == EXW - 1 || ey == EXH - 1 || (!INNER && (x >= w || y >= h))) return;
+    // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
+    // offset
+    const unsigned q = (unsigned)x + (unsigned)y * (unsigned)s;
+    joint_store(V, q, J,
+                joint_back_calc<EXW, NE, false, INNER>(a, ucp, J, x, y, w, h, e, s_du, s_dv, s_X, s_Y, s_Z));
+}
+
 template <int EXW, int EXH, int OCC, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
 __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
     constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
@@ -694,25 +723,14 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
     const JointView V =
         SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
-    const int w = V.w, h = V.h, s = V.s;
     const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
     const int x0 = txy.x * TW, y0 = txy.y * TH;
-    const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
-    const int x = x0 - 1 + ex, y = y0 - 1 + ey;
-    // read before any store of the kernel, so that it compiles to scalar loads
-    const UcParams ucp = *V.ucp;
-    const JointFront J = joint_front<F64TRI, PRIOR, TS>(a, V, min(max(x, 0), w - 1), min(max(y, 0), h - 1));
-    s_du[e] = J.nduh;
-    s_dv[e] = J.ndvh;
-    s_X[e] = J.oX;
-    s_Y[e] = J.oY;
-    s_Z[e] = J.oZ;
-    __syncthreads();
-    if (ex == 0 || ey == 0 || ex == EXW - 1 || ey == EXH - 1 || x >= w || y >= h) return;
-    // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
-    // offset
-    const unsigned q = (unsigned)x + (unsigned)y * (unsigned)s;
-    joint_store(V, q, J, joint_back_calc<EXW, NE, false>(a, ucp, J, x, y, w, h, e, s_du, s_dv, s_X, s_Y, s_Z));
+    // lanes cover pixels x0 - 1 .. x0 + TW and y0 - 1 .. y0 + TH; the batched kernels keep one path
+    // (with the field table resolved in the kernel, a second path makes the registers spill)
+    if (SINGLE && x0 >= 2 && y0 >= 2 && x0 + TW < V.w - 1 && y0 + TH < V.h - 1)
+        joint_px_tile<EXW, EXH, true, F64TRI, PRIOR, TS>(a, V, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
+    else
+        joint_px_tile<EXW, EXH, false, F64TRI, PRIOR, TS>(a, V, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
 }
 
 // K joint iterations per launch (temporal blocking) on an EXW x EXH extended tile indexed by flat
