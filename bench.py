@@ -138,8 +138,9 @@ def run_joint_batched(args, rf, torch, n, storage):
     s = b.stride(0)
     u_ptr, v_ptr = b.result_ptrs(0)[:2]
     u, v = _fetch(u_ptr, H * s).reshape(H, s), _fetch(v_ptr, H * s).reshape(H, s)
+    tiling = [TILING.get(c, str(c)) for c in b.tiling(LEVELS)]
     del b, keep
-    return n * steps / dt, u, v
+    return n * steps / dt, u, v, tiling
 
 
 def run_joint(args, rf, torch):
@@ -221,8 +222,8 @@ def run_joint(args, rf, torch):
     if path == rf.PATH_FUSED and args.batch_pairs > 0:
         batched = {"pairs_per_launch": args.batch_pairs}
         for name, storage in (("f32", rf.STORAGE_F32), ("f16_derivatives", rf.STORAGE_F16_DERIV)):
-            bfps, bu, bv = run_joint_batched(args, rf, torch, args.batch_pairs, storage)
-            ent = {"frames_per_s": round(bfps, 2)}
+            bfps, bu, bv, btl = run_joint_batched(args, rf, torch, args.batch_pairs, storage)
+            ent = {"frames_per_s": round(bfps, 2), "joint_tiling_per_level": btl}
             if plain_fps:
                 ent["speedup_vs_plain_path"] = round(bfps / plain_fps, 3)
             if "joint" in REF64:

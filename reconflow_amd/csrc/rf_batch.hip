@@ -309,8 +309,47 @@ struct IterArgs {
     int level, pd, pf, px, pu, first;
     float lambda, theta, tau, aTv, aP, aFn, lf, lms;
     int sparse;
-    JointView v;  // one pair's fields on this level, by value (one-pixel kernels, SINGLE)
+    JointView v;  // pair 0's fields on this level, by value (one-pixel kernels, SINGLE)
+    // SINGLE over several pairs of one size: tiles per pair (0: one pair), its reciprocal, and the
+    // elements between consecutive pairs' fields in the slab (the fields of pair i are pair 0's
+    // fields moved by i * pair_elems elements; the whole slab stays below 2^30 elements)
+    int tpp;
+    float inv_tpp;
+    unsigned pair_elems;
 };
+
+// Where a tile's pair lives relative to the view: element q of the pair's fields is element
+// qb + q of the view's, and its camera data are cb bytes after the view's.
+struct PairOff {
+    unsigned qb;
+    unsigned cb;
+};
+template <typename T>
+__device__ __forceinline__ const T& cam(const T* p, const PairOff& o) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + o.cb);
+}
+
+// How a launch finds its tiles' fields (SM): 0, from the tile list and the pair descriptors; 1, one
+// pair whose field table is in the kernel arguments; 2, pairs of one size: pair 0's table in the
+// kernel arguments, the pair = tile / tiles per pair (none when tpp is 0), located by `o`.
+template <int SM>
+__device__ __forceinline__ JointView tile_view(const IterArgs& a, int tile, int tw, int& t, PairOff& o) {
+    o = PairOff{0u, 0u};
+    if (SM > 0) {
+        t = tile;
+        if (SM == 2 && a.tpp > 0) {
+            int pair = (int)((float)tile * a.inv_tpp);
+            if (pair * a.tpp > tile) --pair;
+            else if ((pair + 1) * a.tpp <= tile) ++pair;
+            t = tile - pair * a.tpp;
+            o = PairOff{(unsigned)pair * a.pair_elems, (unsigned)pair * (unsigned)sizeof(PairFields)};
+        }
+        return a.v;
+    }
+    const int2 pt = a.L.map[tile];
+    t = pt.y;
+    return joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, tw, a.first);
+}
 
 // Tile index -> (column, row) of tiles for small indices (< 2^24) with a float reciprocal of the
 // row length and one correction each way.
@@ -530,9 +569,9 @@ struct JointIn {
 
 // INNER: the pixel has neighbours on every side (not on column 0 or row 0), known at compile time.
 template <bool PRIOR, typename TS, bool INNER = false>
-__device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView& V, int xc, int yc) {
+__device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView& V, int xc, int yc, unsigned qb) {
     const int s = V.s;
-    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
+    const unsigned q = qb + (unsigned)xc + (unsigned)yc * (unsigned)s;
     const bool hl = INNER || xc > 0, hu = INNER || yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
     // unconditional loads: on the first iteration of a warp the view's uhat and dual inputs are
@@ -561,8 +600,8 @@ __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView
 }
 
 template <bool F64TRI, bool PRIOR, bool INNER = false>
-__device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const JointView& V, int xc, int yc,
-                                                    const JointIn& in) {
+__device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const JointView& V, const PairOff& po, int xc,
+                                                    int yc, const JointIn& in) {
     const bool hl = INNER || xc > 0, hu = INNER || yc > 0;
     // ---- data step + projection blend
     float AA, bu, bv;
@@ -586,17 +625,18 @@ __device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const Joi
     // ---- triangulation
     float oX, oY, oZ;
     if (F64TRI)
-        solve3d_px_f64(*V.pqd, (double)a.lf, (double)a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr,
+        solve3d_px_f64(cam(V.pqd, po), (double)a.lf, (double)a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr,
                        in.Zr, in.Xd, in.Yd, in.Zd, in.Xl, in.Yl, in.Zl, in.Xu, in.Yu, in.Zu, hl, hu, oX, oY, oZ);
     else
-        solve3d_px<float>(*V.pq, a.lf, a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr, in.Zr, in.Xd,
+        solve3d_px<float>(cam(V.pq, po), a.lf, a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr, in.Zr, in.Xd,
                           in.Yd, in.Zd, in.Xl, in.Yl, in.Zl, in.Xu, in.Yu, in.Zu, hl, hu, oX, oY, oZ);
     return JointFront{in.p1u, in.p2u, in.p1v, in.p2v, in.uu, in.vv, in.sku, in.skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
 }
 
 template <bool F64TRI, bool PRIOR, typename TS, bool INNER = false>
-__device__ __forceinline__ JointFront joint_front(const IterArgs& a, const JointView& V, int xc, int yc) {
-    return joint_compute<F64TRI, PRIOR, INNER>(a, V, xc, yc, joint_load<PRIOR, TS, INNER>(a, V, xc, yc));
+__device__ __forceinline__ JointFront joint_front(const IterArgs& a, const JointView& V, const PairOff& po, int xc,
+                                                  int yc) {
+    return joint_compute<F64TRI, PRIOR, INNER>(a, V, po, xc, yc, joint_load<PRIOR, TS, INNER>(a, V, xc, yc, po.qb));
 }
 
 // The back half of a joint iteration at the in-image pixel (x, y) whose extended-tile slot is e
@@ -691,14 +731,14 @@ __device__ __forceinline__ void joint_store(const JointView& V, unsigned q, cons
 // The body of kb_joint_px for the tile at (x0, y0). INNER: every lane of the extended tile is a pixel
 // at least one pixel inside the image, so no coordinate is clamped and no border rule applies.
 template <int EXW, int EXH, bool INNER, bool F64TRI, bool PRIOR, typename TS>
-__device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView& V, int x0, int y0, float* s_du,
-                                              float* s_dv, float* s_X, float* s_Y, float* s_Z) {
+__device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView& V, const PairOff& po, int x0,
+                                              int y0, float* s_du, float* s_dv, float* s_X, float* s_Y, float* s_Z) {
     constexpr int NE = EXW * EXH;
     const int w = V.w, h = V.h, s = V.s;
     const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
     const int x = x0 - 1 + ex, y = y0 - 1 + ey;
     const int xc = INNER ? x : min(max(x, 0), w - 1), yc = INNER ? y : min(max(y, 0), h - 1);
-    const JointFront J = joint_front<F64TRI, PRIOR, TS, INNER>(a, V, xc, yc);
+    const JointFront J = joint_front<F64TRI, PRIOR, TS, INNER>(a, V, po, xc, yc);
     s_du[e] = J.nduh;
     s_dv[e] = J.ndvh;
     s_X[e] = J.oX;
@@ -706,31 +746,32 @@ __device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView
     s_Z[e] = J.oZ;
     __syncthreads();
     // read before any global store of the kernel, so that it compiles to scalar loads
-    const UcParams ucp = *V.ucp;
+    const UcParams ucp = cam(V.ucp, po);
     if (ex == 0 || ey == 0 || ex == EXW - 1 || ey == EXH - 1 || (!INNER && (x >= w || y >= h))) return;
     // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
     // offset
-    const unsigned q = (unsigned)x + (unsigned)y * (unsigned)s;
+    const unsigned q = po.qb + (unsigned)x + (unsigned)y * (unsigned)s;
     joint_store(V, q, J,
                 joint_back_calc<EXW, NE, false, INNER>(a, ucp, J, x, y, w, h, e, s_du, s_dv, s_X, s_Y, s_Z));
 }
 
-template <int EXW, int EXH, int OCC, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
+template <int EXW, int EXH, int OCC, int SM, bool F64TRI, bool PRIOR, typename TS>
 __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
     constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
     __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
     const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
-    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const JointView V =
-        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
-    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    int tp;
+    PairOff po;
+    const JointView V = tile_view<SM>(a, tile, TW, tp, po);
+    const unsigned qb = po.qb;
+    const int2 txy = tile_xy(tp, V.tiles_x, V.inv_tiles_x);
     const int x0 = txy.x * TW, y0 = txy.y * TH;
     // lanes cover pixels x0 - 1 .. x0 + TW and y0 - 1 .. y0 + TH; the batched kernels keep one path
-    // (with the field table resolved in the kernel, a second path makes the registers spill)
-    if (SINGLE && x0 >= 2 && y0 >= 2 && x0 + TW < V.w - 1 && y0 + TH < V.h - 1)
-        joint_px_tile<EXW, EXH, true, F64TRI, PRIOR, TS>(a, V, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
+    // (with more scalars live, a second path makes the registers spill)
+    if (SM == 1 && x0 >= 2 && y0 >= 2 && x0 + TW < V.w - 1 && y0 + TH < V.h - 1)
+        joint_px_tile<EXW, EXH, true, F64TRI, PRIOR, TS>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
     else
-        joint_px_tile<EXW, EXH, false, F64TRI, PRIOR, TS>(a, V, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
+        joint_px_tile<EXW, EXH, false, F64TRI, PRIOR, TS>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
 }
 
 // K joint iterations per launch (temporal blocking) on an EXW x EXH extended tile indexed by flat
@@ -745,18 +786,19 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
 // half one more), so the launch stores its (EXW - 2R) x (EXH - 2R) interior, pixels x0.. and
 // y0..: state n + K into the other parity, one flip per launch. Same per-pixel arithmetic as
 // kb_joint_px.
-template <int EXW, int EXH, int K, bool SINGLE, bool F64TRI, bool PRIOR, typename TS>
+template <int EXW, int EXH, int K, int SM, bool F64TRI, bool PRIOR, typename TS>
 __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
     constexpr int R = 2 * K - 1, TW = EXW - 2 * R, TH = EXH - 2 * R, NE = EXW * EXH;
     static_assert(TW > 0 && TH > 0, "tile too small for K iterations");
     __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];                              // uhat_n, X'_n
     __shared__ float s_p1u[NE], s_p2u[NE], s_p1v[NE], s_p2v[NE], s_Xn[NE], s_Yn[NE], s_Zn[NE];  // p_n, X_n
     const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
-    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const JointView V =
-        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
+    int tp;
+    PairOff po;
+    const JointView V = tile_view<SM>(a, tile, TW, tp, po);
+    const unsigned qb = po.qb;
     const int w = V.w, h = V.h, s = V.s;
-    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int2 txy = tile_xy(tp, V.tiles_x, V.inv_tiles_x);
     const int x0 = txy.x * TW, y0 = txy.y * TH;
     const int lx = threadIdx.x, ly = threadIdx.y, e = lx + ly * EXW;
     const int cx = x0 - R + lx, ry = y0 - R + ly;  // flat coordinates: element cx + ry * s
@@ -767,15 +809,15 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
     const int xc = min(pxw, w - 1), yc = min(max(pyw, 0), h - 1);  // loads of other lanes stay in bounds
     // X0 of an element that is not a pixel: padding or guard row; elements past the guard row or
     // before the field are never read as neighbours
-    const int Ec = min(max(cx + ry * s, 0), (h + 1) * s - 1);
-    const float Xk = el(V.X0, (unsigned)Ec), Yk = el(V.Y0, (unsigned)Ec), Zk = el(V.Z0, (unsigned)Ec);
-    const UcParams ucp = *V.ucp;  // before any store: scalar loads
-    JointIn in = joint_load<PRIOR, TS>(a, V, xc, yc);
+    const unsigned Ec = qb + (unsigned)min(max(cx + ry * s, 0), (h + 1) * s - 1);
+    const float Xk = el(V.X0, Ec), Yk = el(V.Y0, Ec), Zk = el(V.Z0, Ec);
+    const UcParams ucp = cam(V.ucp, po);  // before any store: scalar loads
+    JointIn in = joint_load<PRIOR, TS>(a, V, xc, yc, qb);
     const bool hl = xc > 0, hu = yc > 0;
     const int e_l = max(e - 1, 0), e_u = max(e - EXW, 0), e_r = min(e + 1, NE - 1), e_d = min(e + EXW, NE - 1);
 #pragma unroll 1
     for (int k = 0;; ++k) {
-        const JointFront J = joint_compute<F64TRI, PRIOR>(a, V, xc, yc, in);
+        const JointFront J = joint_compute<F64TRI, PRIOR>(a, V, po, xc, yc, in);
         s_du[e] = J.nduh;
         s_dv[e] = J.ndvh;
         s_X[e] = J.oX;
@@ -785,7 +827,7 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
         const JointBack B = joint_back_calc<EXW, NE, true>(a, ucp, J, xc, yc, w, h, e, s_du, s_dv, s_X, s_Y, s_Z);
         if (k == K - 1) {
             if (lx >= R && lx < EXW - R && ly >= R && ly < EXH - R && cx < w && ry < h)
-                joint_store(V, (unsigned)cx + (unsigned)ry * (unsigned)s, J, B);
+                joint_store(V, qb + (unsigned)cx + (unsigned)ry * (unsigned)s, J, B);
             return;
         }
         const float Xn = pix ? B.X : Xk, Yn = pix ? B.Y : Yk, Zn = pix ? B.Z : Zk;
@@ -908,18 +950,20 @@ __global__ void __launch_bounds__(256) kb_flow_iter(IterArgs a) {
 // tiles are the dispatch indices and whose field table is in the kernel arguments.
 template <int EXW, int EXH, bool SINGLE, typename TS>
 __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
+    constexpr int SM = SINGLE ? 2 : 0;
     constexpr int TW = EXW - 1, TH = EXH - 1, NE = EXW * EXH;
     __shared__ float s_du[NE], s_dv[NE];
     const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
-    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const JointView V =
-        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
+    int tp;
+    PairOff po;
+    const JointView V = tile_view<SM>(a, tile, TW, tp, po);
+    const unsigned qb = po.qb;
     const int w = V.w, h = V.h, s = V.s;
-    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int2 txy = tile_xy(tp, V.tiles_x, V.inv_tiles_x);
     const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
     const int x = txy.x * TW + ex, y = txy.y * TH + ey;
     const int xc = min(x, w - 1), yc = min(y, h - 1);
-    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
+    const unsigned q = qb + (unsigned)xc + (unsigned)yc * (unsigned)s;
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
     const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
@@ -969,19 +1013,21 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
 // n + K into the other parity, one flip per launch. Same per-pixel arithmetic as kb_flow_px.
 template <int EXW, int EXH, int K, bool SINGLE, typename TS>
 __global__ void __launch_bounds__(EXW* EXH) kb_flow_multi(IterArgs a) {
+    constexpr int SM = SINGLE ? 2 : 0;
     constexpr int TW = EXW - 2 * K + 1, TH = EXH - 2 * K + 1, NE = EXW * EXH;
     static_assert(TW > 0 && TH > 0, "tile too small for K iterations");
     __shared__ float s_du[NE], s_dv[NE], s_p1u[NE], s_p2u[NE], s_p1v[NE], s_p2v[NE];
     const int tile = xcd_tile(blockIdx.x, a.L.ntiles);
-    const int2 pt = SINGLE ? make_int2(0, tile) : a.L.map[tile];
-    const JointView V =
-        SINGLE ? a.v : joint_view(a.L.P[pt.x], a.L.P + pt.x, a.level, a.pd, a.pf, a.px, a.pu, TW, a.first);
+    int tp;
+    PairOff po;
+    const JointView V = tile_view<SM>(a, tile, TW, tp, po);
+    const unsigned qb = po.qb;
     const int w = V.w, h = V.h, s = V.s;
-    const int2 txy = tile_xy(pt.y, V.tiles_x, V.inv_tiles_x);
+    const int2 txy = tile_xy(tp, V.tiles_x, V.inv_tiles_x);
     const int lx = threadIdx.x, ly = threadIdx.y, e = lx + ly * EXW;
     const int x = txy.x * TW - (K - 1) + lx, y = txy.y * TH - (K - 1) + ly;
     const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
-    const unsigned q = (unsigned)xc + (unsigned)yc * (unsigned)s;
+    const unsigned q = qb + (unsigned)xc + (unsigned)yc * (unsigned)s;
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
     const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
@@ -1132,6 +1178,12 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
     const long per_fields = 2L * L + 51;  // pyramids, inputs, derivatives, state double buffers, prior, uvrgb(3), scratch, zero
     size_t total = 0;
     for (auto& g : geo_) total += (size_t)per_fields * g.bufn * sizeof(float);
+    // pairs of one size: the same field sits per_fields * bufn floats further on in each pair; the
+    // kernels reach it through 32-bit byte offsets, so the slab must stay below 2^30 floats
+    pair_elems_ = 0;
+    bool same = n > 1 && (long long)per_fields * geo_[0].bufn * n < (1LL << 30);
+    for (int i = 1; i < n && same; ++i) same = geo_[i].W == geo_[0].W && geo_[i].H == geo_[0].H && geo_[i].S == geo_[0].S;
+    if (same) pair_elems_ = (unsigned)(per_fields * geo_[0].bufn);
     if (hipMalloc(&slab_, total) != hipSuccess) {
         slab_ = nullptr;
         set_error("rf_batch_create: device allocation of " + std::to_string(total) + " bytes failed");
@@ -1461,7 +1513,7 @@ int Batch::set_prior(int i, const float* u, const float* v, const float* mask, b
     return check(hipStreamSynchronize(st_), "copy prior");
 }
 
-template <typename TS, bool SINGLE, bool F64, bool PRIOR>
+template <typename TS, int SM, bool F64, bool PRIOR>
 static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
     const dim3 grid(a.L.ntiles);
     switch (cls) {
@@ -1469,19 +1521,19 @@ static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
             hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, F64, PRIOR, TS>), grid, dim3(TXW, TRW), 0, st, a);
             break;
         case TC_J32:
-            hipLaunchKernelGGL((kb_joint_px<32, 16, 5, SINGLE, F64, PRIOR, TS>), grid, dim3(32, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<32, 16, 5, SM, F64, PRIOR, TS>), grid, dim3(32, 16), 0, st, a);
             break;
         case TC_J16O:
-            hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
         case TC_JK2:
-            hipLaunchKernelGGL((kb_joint_multi<16, 16, 2, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 2, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
         case TC_JK3:
-            hipLaunchKernelGGL((kb_joint_multi<16, 16, 3, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 3, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
         default:
-            hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SINGLE, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
             break;
     }
 }
@@ -1510,19 +1562,25 @@ static void launch_flow(int cls, bool single, hipStream_t st, const IterArgs& a)
     }
 }
 
+// sm: how the kernels find the fields (see tile_view)
 template <typename TS>
-static void launch_joint(int cls, bool single, bool f64, bool prior, hipStream_t st, const IterArgs& a) {
-    const int k = (single ? 4 : 0) | (f64 ? 2 : 0) | (prior ? 1 : 0);
-    switch (k) {
-        case 0: launch_joint_t<TS, false, false, false>(cls, st, a); break;
-        case 1: launch_joint_t<TS, false, false, true>(cls, st, a); break;
-        case 2: launch_joint_t<TS, false, true, false>(cls, st, a); break;
-        case 3: launch_joint_t<TS, false, true, true>(cls, st, a); break;
-        case 4: launch_joint_t<TS, true, false, false>(cls, st, a); break;
-        case 5: launch_joint_t<TS, true, false, true>(cls, st, a); break;
-        case 6: launch_joint_t<TS, true, true, false>(cls, st, a); break;
-        default: launch_joint_t<TS, true, true, true>(cls, st, a); break;
+static void launch_joint(int cls, int sm, bool f64, bool prior, hipStream_t st, const IterArgs& a) {
+    const int k = (f64 ? 2 : 0) | (prior ? 1 : 0);
+#define RF_J(SM)                                                            \
+    switch (k) {                                                            \
+        case 0: launch_joint_t<TS, SM, false, false>(cls, st, a); break;    \
+        case 1: launch_joint_t<TS, SM, false, true>(cls, st, a); break;     \
+        case 2: launch_joint_t<TS, SM, true, false>(cls, st, a); break;     \
+        default: launch_joint_t<TS, SM, true, true>(cls, st, a); break;     \
     }
+    if (sm == 1) {
+        RF_J(1)
+    } else if (sm == 2) {
+        RF_J(2)
+    } else {
+        RF_J(0)
+    }
+#undef RF_J
 }
 
 #define RF_TRY(x)                          \
@@ -1575,15 +1633,23 @@ hipError_t Batch::enqueue_t(bool recon) {
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
-                           (fn && sparse_) ? 1 : 0, JointView{}};
-                // one pair: its fields travel in the kernel arguments (no dependent descriptor load)
-                if (n() == 1 && cls != TC_J64 && cls != TC_FLOW)
+                           (fn && sparse_) ? 1 : 0, JointView{}, 0, 0.0f, 0u};
+                // one pair, or pairs of one size: the fields travel in the kernel arguments (no
+                // dependent descriptor load)
+                const bool single = (n() == 1 || pair_elems_ > 0) && cls != TC_J64 && cls != TC_FLOW;
+                if (single) {
                     a.v = joint_view(hp_[0], dp_, level, pr.d, pr.f, pr.x, pr.u, kTileW[cls], it == 0);
+                    if (n() > 1) {
+                        a.tpp = ts.ntiles / n();
+                        a.inv_tpp = 1.0f / (float)a.tpp;
+                        a.pair_elems = pair_elems_;
+                    }
+                }
                 if (ts.ntiles > 0) {
                     if (!recon)
-                        launch_flow<TS>(cls, n() == 1, st_, a);
+                        launch_flow<TS>(cls, single, st_, a);
                     else
-                        launch_joint<TS>(cls, n() == 1, p_.tri_f64 != 0, fn, st_, a);
+                        launch_joint<TS>(cls, single ? (n() == 1 ? 1 : 2) : 0, p_.tri_f64 != 0, fn, st_, a);
                     RF_TRY(hipGetLastError());
                 }
                 pr.d ^= 1;  // one flip per launch, whatever its iteration count

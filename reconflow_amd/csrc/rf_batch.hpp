@@ -130,7 +130,8 @@ class Batch {
     // results)
     std::vector<int> cls_[2];
     bool tuned_[2] = {false, false}, tuning_ = false;
-    int tune_round_ = 0;  // tuning solve in progress: candidate of warp w is (round * warps + w) mod count
+    int tune_round_ = 0;
+    unsigned pair_elems_ = 0;  // pairs of one size: floats between their fields (0 otherwise)  // tuning solve in progress: candidate of warp w is (round * warps + w) mod count
     std::vector<hipEvent_t> tev_;
     hipGraphExec_t gexec_[2] = {nullptr, nullptr};
     Parity gfinal_[2];

@@ -70,3 +70,57 @@ def test_batch_recon(orc, graph, tile):
             ref = orc.solve_recon(pr, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f32")
             for a, r in zip(got, ref):
                 assert np.array_equal(a[:, :w], r[:, :w], equal_nan=True), (rep, i)
+
+
+@pytest.mark.parametrize("tile", [0, 5], ids=["auto", "16x16-k2"])
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_batch_recon_same_size(orc, graph, tile):
+    # pairs of one size run the one-pair kernels with the pair taken from the dispatch index
+    w, h, n = 144, 104, 3
+    print(f"case: batch joint {n} pairs of {w}x{h} graph={graph} tile={tile}", flush=True)
+    p = rf.batch_params(14, 3, 2, 7, tau=0.125, use_graph=graph, joint_tile=tile, tri_f64=1)
+    b = rf.Batch(p, [w] * n, [h] * n)
+    data, keep = [], []
+    for i in range(n):
+        d = rf.make_recon_pair(w, h, 300 + i)
+        d0, d1 = _dev_img(d["i0"]), _dev_img(d["i1"])
+        keep += [d0, d1]
+        b.set_images_device(i, d0.data_ptr(), d1.data_ptr())
+        b.set_camera(i, d["K"], d["K"], d["R"], d["t"])
+        data.append(d)
+    s = rf.align_up(w)
+    for rep in range(2):
+        b.solve()
+        b.synchronize()
+        for i in range(n):
+            got = [_fetch(ptr, h * s).reshape(h, s) for ptr in b.result_ptrs(i)]
+            d = data[i]
+            pr = orc.recon_params(w, h, 3, n_warp=2, n_iter=7, tri_f64=1)
+            ref = orc.solve_recon(pr, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f32")
+            for a, r in zip(got, ref):
+                assert np.array_equal(a[:, :w], r[:, :w], equal_nan=True), (rep, i)
+
+
+@pytest.mark.parametrize("tile", [0, 4], ids=["auto", "32x16-k4"])
+def test_batch_flow_same_size(orc, tile):
+    w, h, n = 144, 104, 3
+    print(f"case: batch TV-L1 {n} pairs of {w}x{h} tile={tile}", flush=True)
+    p = rf.batch_params(0, 3, 2, 9, use_graph=True, flow_tile=tile)
+    b = rf.Batch(p, [w] * n, [h] * n)
+    pairs, keep = [], []
+    for i in range(n):
+        i0, i1, _, _ = rf.make_flow_pair(w, h, 400 + i)
+        d0, d1 = _dev_img(i0), _dev_img(i1)
+        keep += [d0, d1]
+        b.set_images_device(i, d0.data_ptr(), d1.data_ptr())
+        pairs.append((i0, i1))
+    s = rf.align_up(w)
+    for rep in range(2):
+        b.solve()
+        b.synchronize()
+        for i in range(n):
+            u_ptr, v_ptr = b.result_ptrs(i)[:2]
+            u = _fetch(u_ptr, h * s).reshape(h, s)
+            v = _fetch(v_ptr, h * s).reshape(h, s)
+            ru, rv = orc.solve_flow(orc.flow_params(w, h, 3, n_warp=2, n_iter=9), *pairs[i], "f32")
+            assert np.array_equal(u[:, :w], ru[:, :w]) and np.array_equal(v[:, :w], rv[:, :w]), (rep, i)
