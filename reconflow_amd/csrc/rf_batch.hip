@@ -319,7 +319,8 @@ struct IterArgs {
 };
 
 // Where a tile's pair lives relative to the view: element q of the pair's fields is element
-// qb + q of the view's, and its camera data are cb bytes after the view's.
+// qb + q of the view's (qb counted in floats: a field of 16-bit elements is qb * 2 elements
+// further on), and its camera data are cb bytes after the view's.
 struct PairOff {
     unsigned qb;
     unsigned cb;
@@ -583,7 +584,8 @@ __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView
     in.uu = el(V.U, q), in.vv = el(V.V, q);
     in.sku = el(V.sku0, q), in.skv = el(V.skv0, q), in.up0 = el(V.up0, q), in.vp0 = el(V.vp0, q);
     in.uc0 = el(V.uc0, q), in.vc0 = el(V.vc0, q);
-    in.ix = (float)el(as<TS>(V.Ix), q), in.iy = (float)el(as<TS>(V.Iy), q), in.iz = (float)el(as<TS>(V.Iz), q);
+    const unsigned qt = q + qb * (unsigned)(sizeof(float) / sizeof(TS) - 1);  // the same element in TS fields
+    in.ix = (float)el(as<TS>(V.Ix), qt), in.iy = (float)el(as<TS>(V.Iy), qt), in.iz = (float)el(as<TS>(V.Iz), qt);
     const unsigned qr = q + 1u, qd = q + (unsigned)s;
     in.Xc = el(V.X0, q), in.Yc = el(V.Y0, q), in.Zc = el(V.Z0, q);
     in.Xr = el(V.X0, qr), in.Yr = el(V.Y0, qr), in.Zr = el(V.Z0, qr);
@@ -967,14 +969,16 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
     const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
+    // the same elements in the TS fields (the pair offset counts floats)
+    const unsigned dt = qb * (unsigned)(sizeof(float) / sizeof(TS) - 1), qt = q + dt, qlt = ql + dt, qut = qu + dt;
     // unconditional loads: on the first iteration of a warp the view's uhat and dual inputs are
     // the zero field
     const float duh = el(V.du0, q), dvh = el(V.dv0, q);
-    const float p1u = (float)el(p1u0, q), p2u = (float)el(p2u0, q), p1v = (float)el(p1v0, q),
-                p2v = (float)el(p2v0, q);
-    const float p1ul = (float)el(p1u0, ql), p1vl = (float)el(p1v0, ql), p2ud = (float)el(p2u0, qu),
-                p2vd = (float)el(p2v0, qu);
-    const float ix_ = (float)el(as<TS>(V.Ix), q), iy_ = (float)el(as<TS>(V.Iy), q), iz_ = (float)el(as<TS>(V.Iz), q);
+    const float p1u = (float)el(p1u0, qt), p2u = (float)el(p2u0, qt), p1v = (float)el(p1v0, qt),
+                p2v = (float)el(p2v0, qt);
+    const float p1ul = (float)el(p1u0, qlt), p1vl = (float)el(p1v0, qlt), p2ud = (float)el(p2u0, qut),
+                p2vd = (float)el(p2v0, qut);
+    const float ix_ = (float)el(as<TS>(V.Ix), qt), iy_ = (float)el(as<TS>(V.Iy), qt), iz_ = (float)el(as<TS>(V.Iz), qt);
     float du, dv;
     l1_threshold(ix_, iy_, iz_, duh, dvh, a.lambda, a.theta, du, dv);
     const float nduh = du + a.theta * div_p(p1u, p1ul, p2u, p2ud, hl, hu);
@@ -995,11 +999,11 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
     }
     float o1, o2;
     dual_px(dux, duy, p1u, p2u, fac, o1, o2);
-    el(as<TS>(V.p1u1), q) = st<TS>(o1);
-    el(as<TS>(V.p2u1), q) = st<TS>(o2);
+    el(as<TS>(V.p1u1), qt) = st<TS>(o1);
+    el(as<TS>(V.p2u1), qt) = st<TS>(o2);
     dual_px(dvx, dvy, p1v, p2v, fac, o1, o2);
-    el(as<TS>(V.p1v1), q) = st<TS>(o1);
-    el(as<TS>(V.p2v1), q) = st<TS>(o2);
+    el(as<TS>(V.p1v1), qt) = st<TS>(o1);
+    el(as<TS>(V.p2v1), qt) = st<TS>(o2);
     el(V.du1, q) = nduh;
     el(V.dv1, q) = ndvh;
 }
@@ -1031,11 +1035,13 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_multi(IterArgs a) {
     const bool hl = xc > 0, hu = yc > 0;
     const unsigned ql = hl ? q - 1u : q, qu = hu ? q - (unsigned)s : q;
     const TS *p1u0 = as<TS>(V.p1u0), *p2u0 = as<TS>(V.p2u0), *p1v0 = as<TS>(V.p1v0), *p2v0 = as<TS>(V.p2v0);
+    // the same elements in the TS fields (the pair offset counts floats)
+    const unsigned dt = qb * (unsigned)(sizeof(float) / sizeof(TS) - 1), qt = q + dt, qlt = ql + dt, qut = qu + dt;
     float duh = el(V.du0, q), dvh = el(V.dv0, q);
-    float p1u = (float)el(p1u0, q), p2u = (float)el(p2u0, q), p1v = (float)el(p1v0, q), p2v = (float)el(p2v0, q);
-    float p1ul = (float)el(p1u0, ql), p1vl = (float)el(p1v0, ql), p2ud = (float)el(p2u0, qu),
-          p2vd = (float)el(p2v0, qu);
-    const float ix_ = (float)el(as<TS>(V.Ix), q), iy_ = (float)el(as<TS>(V.Iy), q), iz_ = (float)el(as<TS>(V.Iz), q);
+    float p1u = (float)el(p1u0, qt), p2u = (float)el(p2u0, qt), p1v = (float)el(p1v0, qt), p2v = (float)el(p2v0, qt);
+    float p1ul = (float)el(p1u0, qlt), p1vl = (float)el(p1v0, qlt), p2ud = (float)el(p2u0, qut),
+          p2vd = (float)el(p2v0, qut);
+    const float ix_ = (float)el(as<TS>(V.Ix), qt), iy_ = (float)el(as<TS>(V.Iy), qt), iz_ = (float)el(as<TS>(V.Iz), qt);
     const float fac = a.tau / a.theta;
     const int e_l = max(e - 1, 0), e_u = max(e - EXW, 0), e_r = min(e + 1, NE - 1), e_d = min(e + EXW, NE - 1);
 #pragma unroll 1
@@ -1061,10 +1067,10 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_multi(IterArgs a) {
         dual_px(dvx, dvy, p1v, p2v, fac, o1v, o2v);
         if (k == K - 1) {
             if (lx >= K - 1 && lx < K - 1 + TW && ly >= K - 1 && ly < K - 1 + TH && x < w && y < h) {
-                el(as<TS>(V.p1u1), q) = st<TS>(o1u);
-                el(as<TS>(V.p2u1), q) = st<TS>(o2u);
-                el(as<TS>(V.p1v1), q) = st<TS>(o1v);
-                el(as<TS>(V.p2v1), q) = st<TS>(o2v);
+                el(as<TS>(V.p1u1), qt) = st<TS>(o1u);
+                el(as<TS>(V.p2u1), qt) = st<TS>(o2u);
+                el(as<TS>(V.p1v1), qt) = st<TS>(o1v);
+                el(as<TS>(V.p2v1), qt) = st<TS>(o2v);
                 el(V.du1, q) = nduh;
                 el(V.dv1, q) = ndvh;
             }

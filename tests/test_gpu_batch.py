@@ -124,3 +124,53 @@ def test_batch_flow_same_size(orc, tile):
             v = _fetch(v_ptr, h * s).reshape(h, s)
             ru, rv = orc.solve_flow(orc.flow_params(w, h, 3, n_warp=2, n_iter=9), *pairs[i], "f32")
             assert np.array_equal(u[:, :w], ru[:, :w]) and np.array_equal(v[:, :w], rv[:, :w]), (rep, i)
+
+
+def _solve_single(p, w, h, setup):
+    """One pair alone in a batch of one: the reference result of a pair of a batch."""
+    b = rf.Batch(p, [w], [h])
+    keep = setup(b, 0)
+    b.solve()
+    b.synchronize()
+    s = rf.align_up(w)
+    out = [_fetch(ptr, h * s).reshape(h, s) for ptr in b.result_ptrs(0) if ptr]
+    del keep
+    return out
+
+
+@pytest.mark.parametrize("kind", ["joint-f16-derivatives", "tvl1-f16"])
+def test_batch_same_size_fp16_pairs_equal_single(kind):
+    # 16-bit fields of later pairs sit at the pair offset counted in 16-bit elements: every pair of
+    # a same-size FP16 batch equals the same pair solved alone, bit for bit
+    w, h, n = 144, 104, 3
+    print(f"case: batch {kind} {n} pairs of {w}x{h}", flush=True)
+    joint = kind.startswith("joint")
+    if joint:
+        p = rf.batch_params(14, 3, 2, 7, tau=0.125, use_graph=True, tri_f64=1, storage=rf.STORAGE_F16_DERIV)
+        data = [rf.make_recon_pair(w, h, 500 + i) for i in range(n)]
+    else:
+        p = rf.batch_params(0, 3, 2, 9, use_graph=True, storage=rf.STORAGE_F16)
+        data = [dict(zip(("i0", "i1"), rf.make_flow_pair(w, h, 600 + i)[:2])) for i in range(n)]
+
+    def setup_for(i):
+        def setup(b, j):
+            d = data[i]
+            d0, d1 = _dev_img(d["i0"]), _dev_img(d["i1"])
+            b.set_images_device(j, d0.data_ptr(), d1.data_ptr())
+            if joint:
+                b.set_camera(j, d["K"], d["K"], d["R"], d["t"])
+            return d0, d1
+        return setup
+
+    b = rf.Batch(p, [w] * n, [h] * n)
+    keep = [setup_for(i)(b, i) for i in range(n)]
+    b.solve()
+    b.synchronize()
+    s = rf.align_up(w)
+    for i in range(n):
+        got = [_fetch(ptr, h * s).reshape(h, s) for ptr in b.result_ptrs(i) if ptr]
+        ref = _solve_single(p, w, h, setup_for(i))
+        assert len(got) == len(ref)
+        for a, r in zip(got, ref):
+            assert np.array_equal(a[:, :w], r[:, :w], equal_nan=True), i
+    del keep
