@@ -765,7 +765,6 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     int tp;
     PairOff po;
     const JointView V = tile_view<SM>(a, tile, TW, tp, po);
-    const unsigned qb = po.qb;
     const int2 txy = tile_xy(tp, V.tiles_x, V.inv_tiles_x);
     const int x0 = txy.x * TW, y0 = txy.y * TH;
     // lanes cover pixels x0 - 1 .. x0 + TW and y0 - 1 .. y0 + TH; the batched kernels keep one path
@@ -1102,18 +1101,19 @@ __global__ void __launch_bounds__(256) kb_zero_uhat(Launch L, int l, int pd) {
 
 Batch::~Batch() { release(); }
 
+// Teardown: a failing destroy or free leaves nothing to recover, so its status is not checked.
 void Batch::release() {
-    for (auto& e : prof_.ev) hipEventDestroy(e);
+    for (auto& e : prof_.ev) (void)hipEventDestroy(e);
     prof_.ev.clear();
     for (auto& g : gexec_)
         if (g) {
-            hipGraphExecDestroy(g);
+            (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
-    if (maps_) hipFree(maps_);
-    if (dp_) hipFree(dp_);
-    if (slab_) hipFree(slab_);
-    if (st_) hipStreamDestroy(st_);
+    if (maps_) (void)hipFree(maps_);
+    if (dp_) (void)hipFree(dp_);
+    if (slab_) (void)hipFree(slab_);
+    if (st_) (void)hipStreamDestroy(st_);
     maps_ = nullptr;
     dp_ = nullptr;
     slab_ = nullptr;
@@ -1692,7 +1692,7 @@ int Batch::solve(bool recon, float flow_scale) {
     if (dirty_ && upload_pairs()) return RF_ERR_HIP;
     const int kind = recon ? 1 : 0;
     if (p_.with_visualization && flow_scale != flow_scale_ && gexec_[kind]) {
-        hipGraphExecDestroy(gexec_[kind]);
+        (void)hipGraphExecDestroy(gexec_[kind]);  // recaptured below with the new scale
         gexec_[kind] = nullptr;
     }
     flow_scale_ = flow_scale;
@@ -1710,11 +1710,11 @@ int Batch::solve(bool recon, float flow_scale) {
         hipError_t e = enqueue(recon);
         hipError_t e2 = hipStreamEndCapture(st_, &graph);
         if (check(e, "batch solve (capture)") || check(e2, "hipStreamEndCapture")) {
-            if (graph) hipGraphDestroy(graph);
+            if (graph) (void)hipGraphDestroy(graph);
             return RF_ERR_HIP;
         }
         hipError_t e3 = hipGraphInstantiate(&gexec_[kind], graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (check(e3, "hipGraphInstantiate")) {
             gexec_[kind] = nullptr;
             return RF_ERR_HIP;

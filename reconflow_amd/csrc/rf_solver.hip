@@ -38,26 +38,27 @@ int Solver::check(hipError_t e, const char* what) {
     return RF_ERR_HIP;
 }
 
+// Teardown: a failing destroy or free leaves nothing to recover, so its status is not checked.
 int Solver::release() {
     batch_.reset();
-    for (auto& e : prof_.ev) hipEventDestroy(e);
+    for (auto& e : prof_.ev) (void)hipEventDestroy(e);
     prof_.ev.clear();
     prof_.armed = false;
     for (auto& g : gexec_)
         if (g) {
-            hipGraphExecDestroy(g);
+            (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
     if (slab_) {
-        hipFree(slab_);
+        (void)hipFree(slab_);
         slab_ = nullptr;
     }
     if (dparams_) {
-        hipFree(dparams_);
+        (void)hipFree(dparams_);
         dparams_ = nullptr;
     }
     if (st_) {
-        hipStreamDestroy(st_);
+        (void)hipStreamDestroy(st_);
         st_ = nullptr;
     }
     inited_ = false;
@@ -651,11 +652,11 @@ int Solver::run_schedule(int kind) {
         hipError_t e = enqueue(f);
         hipError_t e2 = hipStreamEndCapture(st_, &graph);
         if (check(e, "solve (capture)") || check(e2, "hipStreamEndCapture")) {
-            if (graph) hipGraphDestroy(graph);
+            if (graph) (void)hipGraphDestroy(graph);
             return RF_ERR_HIP;
         }
         hipError_t e3 = hipGraphInstantiate(&gexec_[kind], graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (check(e3, "hipGraphInstantiate")) {
             gexec_[kind] = nullptr;
             return RF_ERR_HIP;
@@ -678,7 +679,7 @@ int Solver::solve_flow(float flow_scale) {
     }
     if (batch_) return batch_->solve(false, flow_scale);
     if (p_.use_graph && p_.with_visualization && flow_scale != flow_scale_ && gexec_[0]) {
-        hipGraphExecDestroy(gexec_[0]);
+        (void)hipGraphExecDestroy(gexec_[0]);  // recaptured with the new scale
         gexec_[0] = nullptr;
     }
     flow_scale_ = flow_scale;
@@ -744,7 +745,7 @@ int Solver::solve_recon(const double* R0, const double* t0, const double* R1, co
     // the stream orders the upload before the solve; make sure the host copy is consumed
     if (check(hipStreamSynchronize(st_), "upload pose")) return RF_ERR_HIP;
     if (p_.use_graph && p_.with_visualization && flow_scale != flow_scale_ && gexec_[1]) {
-        hipGraphExecDestroy(gexec_[1]);
+        (void)hipGraphExecDestroy(gexec_[1]);  // recaptured with the new scale
         gexec_[1] = nullptr;
     }
     flow_scale_ = flow_scale;
@@ -818,7 +819,7 @@ int Solver::solve_inpainting(float flow_scale) {
         return RF_ERR_STATE;
     }
     if (p_.use_graph && p_.with_visualization && flow_scale != flow_scale_ && gexec_[2]) {
-        hipGraphExecDestroy(gexec_[2]);
+        (void)hipGraphExecDestroy(gexec_[2]);  // recaptured with the new scale
         gexec_[2] = nullptr;
     }
     flow_scale_ = flow_scale;

@@ -65,12 +65,12 @@ struct Profiler {
     void begin(hipStream_t st, int k, int l) {
         if (!armed || k != kid || l != level || open) return;
         if (used + 2 > (int)ev.size()) return;
-        hipEventRecord(ev[used], st);
+        (void)hipEventRecord(ev[used], st);  // a failed record shows when the times are read
         open = true;
     }
     void end(hipStream_t st, int k, int l, int n) {
         if (!armed || k != kid || l != level || !open) return;
-        hipEventRecord(ev[used + 1], st);
+        (void)hipEventRecord(ev[used + 1], st);
         used += 2;
         launches += n;
         open = false;
@@ -126,7 +126,10 @@ class Solver {
     hipError_t enqueue_gray_pyramid(Fields& f);
     int check(hipError_t e, const char* what);
 
-    int bw_, bh_, align_;
+    // the reference constructor's launch block shape (Cudaflow.cu:10-14), kept for its interface:
+    // the kernels here choose their own tiles
+    [[maybe_unused]] int bw_, bh_;
+    int align_;
     rf_params p_{};
     bool inited_ = false, prior_ = false, sparse_ = false, camera_ = false;
     bool masks_ = false, depth_ = false, scene_ = false, masked_run_ = false;
