@@ -17,6 +17,8 @@ GPU flow against that FP64 result, and the per-step (plain) GPU path's frames/s 
 
 --workload tvl1-320: the 320x240 TV-L1 case (fused, fused FP16, plain; EPE and time of the FP64
 oracle). --workload methods: each CudaFlow data term at 320x240 on the per-step path.
+--workload stream720: consecutive pairs of a 1280x720 synthetic camera trajectory, one and eight
+pairs per batched launch: frames/s and p50/p99 latency.
 --workload batch64: 64 seeded pairs of uneven sizes (W in 160..1280, H in 120..720, multiples of
 16), even seeds TV-L1 (4 levels, 3 warps, 30 iterations; FP16 images, derivatives and duals),
 odd seeds joint (as above; FP16 derivatives, FP32 images); the two batches run on their own
@@ -399,6 +401,70 @@ def run_tvl1(args, rf, torch):
     }
 
 
+def run_stream(args, rf, torch):
+    """Streaming 1280x720 joint flow+depth: consecutive pairs of a synthetic camera trajectory
+    (per-frame rotation <= 0.5 deg, camera displacement <= 0.02, 16 planes at depths 2-20), each
+    solved cold (5 levels x 5 warps x 50 iterations), B consecutive pairs per batched launch. A
+    frame's result is ready when its batch completes: its latency is the batch's time from
+    submission to completion (waiting for the batch's later frames to arrive is not counted)."""
+    w, h, n = 1280, 720, args.stream_frames
+    t_gen = time.perf_counter()
+    seq = rf.make_sequence(w, h, n, args.seed)
+    t_gen = time.perf_counter() - t_gen
+    frames = torch.from_numpy(seq["frames"]).cuda()
+    K, R, t = seq["K"], seq["R"], seq["t"]
+    runs, pair0 = {}, None
+    for name, B, storage in (("1_pair_f32", 1, rf.STORAGE_F32), ("8_pairs_f32", 8, rf.STORAGE_F32),
+                             ("8_pairs_f16_derivatives", 8, rf.STORAGE_F16_DERIV)):
+        p = rf.batch_params(14, LEVELS, WARPS, ITERS, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0,
+                            lambdaf=0.1, lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage)
+        b = rf.Batch(p, [w] * B, [h] * B)
+
+        def submit(k0):
+            for i in range(B):
+                b.set_images_device(i, frames[k0 + i].data_ptr(), frames[k0 + i + 1].data_ptr())
+                b.set_camera(i, K, K, R[k0 + i], t[k0 + i])
+            b.solve()
+
+        submit(0)  # the first solve of a batch includes the tiling tuning solves
+        b.synchronize()
+        if pair0 is None:
+            st = b.stride(0)
+            u_ptr, v_ptr = b.result_ptrs(0)[:2]
+            pair0 = (_fetch(u_ptr, h * st).reshape(h, st), _fetch(v_ptr, h * st).reshape(h, st))
+        npairs = ((n - 1) // B) * B
+        lat = []
+        t0 = time.perf_counter()
+        for k0 in range(0, npairs, B):
+            ts = time.perf_counter()
+            submit(k0)
+            b.synchronize()
+            lat.append(time.perf_counter() - ts)
+        dt = time.perf_counter() - t0
+        runs[name] = {"frames_per_s": round(npairs / dt, 2), "pairs": npairs,
+                      "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                      "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                      "joint_tiling_per_level": [TILING.get(c, str(c)) for c in b.tiling(LEVELS)]}
+        del b
+    epe = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        f = seq["frames"]
+        ref = orc.solve_recon(orc.recon_params(w, h, LEVELS), K, K, np.eye(3), np.zeros(3), R[0], t[0], f[0], f[1],
+                              "f64")
+        epe = float(np.hypot(pair0[0][:, :w] - ref[0][:, :w], pair0[1][:, :w] - ref[1][:, :w]).mean())
+    best = runs["8_pairs_f16_derivatives"]
+    return {
+        "metric": "1280x720 streaming joint flow+depth frames/s (consecutive pairs, 8 per batched launch)",
+        "value": best["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(1e3 / best["frames_per_s"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 updates, f16 derivatives" + (", f64 triangulation" if args.tri_f64 else ""),
+        "data": f"synthetic {n}-frame camera trajectory over a piecewise-planar scene (generated in {t_gen:.1f} s)",
+        "config": {"workload": "stream 1280x720, 5 levels x 5 warps x 50 iterations, cold start per pair"},
+        "runs": runs, "epe_pair0_vs_fp64_oracle": None if epe is None else round(epe, 5), "cpu_baseline": None,
+    }
+
+
 def run_methods(args, rf, torch):
     """Each CudaFlow data term at 320x240 (4 levels, 3 warps, 30 iterations) on the per-step
     path, graph-replayed: frames/s per method."""
@@ -438,7 +504,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["joint640", "batch64", "tvl1-320", "methods"], default="joint640")
+    ap.add_argument("--workload", choices=["joint640", "batch64", "tvl1-320", "methods", "stream720"],
+                    default="joint640")
+    ap.add_argument("--stream-frames", type=int, default=49, help="stream720: frames of the sequence")
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--batch-pairs", type=int, default=8,
                     help="joint640: also time this many 640x480 pairs per batched launch (0: skip)")
@@ -472,7 +540,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "launches": 5}))
         return 0
-    run = {"batch64": run_batch, "tvl1-320": run_tvl1, "methods": run_methods}.get(args.workload, run_joint)
+    run = {"batch64": run_batch, "tvl1-320": run_tvl1, "methods": run_methods, "stream720": run_stream}.get(
+        args.workload, run_joint)
     line = run(args, rf, torch)
     print(json.dumps(line))
     return 0

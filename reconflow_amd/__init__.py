@@ -386,3 +386,15 @@ def make_recon_pair(w: int, h: int, seed: int, stride: int | None = None):
                                               i0.ctypes.data, i1.ctypes.data, gu.ctypes.data, gv.ctypes.data,
                                               gz.ctypes.data), "rfs_make_recon_pair")
     return dict(i0=i0, i1=i1, K=K.reshape(3, 3), R=R.reshape(3, 3), t=t, gt_u=gu, gt_v=gv, gt_Z=gz)
+
+
+def make_sequence(w: int, h: int, n: int, seed: int, stride: int | None = None):
+    """Synthetic camera trajectory over a piecewise-planar scene: returns dict with frames
+    (n, h, stride) float32, K (3x3) and the n - 1 relative poses R (n-1, 3, 3), t (n-1, 3) that map
+    frame k's camera points into frame k + 1's."""
+    s = stride or align_up(w)
+    frames = np.zeros((n, h, s), np.float32)
+    K, R, t = np.zeros(9), np.zeros((n - 1) * 9), np.zeros((n - 1) * 3)
+    check(_native.synth().rfs_make_sequence(w, h, s, seed, n, K.ctypes.data, R.ctypes.data, t.ctypes.data,
+                                            frames.ctypes.data), "rfs_make_sequence")
+    return dict(frames=frames, K=K.reshape(3, 3), R=R.reshape(n - 1, 3, 3), t=t.reshape(n - 1, 3))

@@ -115,6 +115,7 @@ _PROTOS = {
 _SYNTH_PROTOS = {
     "rfs_make_flow_pair": (_I, [_I, _I, _I, C.c_uint64, _P, _P, _P, _P]),
     "rfs_make_recon_pair": (_I, [_I, _I, _I, C.c_uint64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "rfs_make_sequence": (_I, [_I, _I, _I, C.c_uint64, _I, _P, _P, _P, _P]),
 }
 
 _lib = None

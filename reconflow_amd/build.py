@@ -102,7 +102,8 @@ def build_synth() -> Path:
     OUT.mkdir(exist_ok=True)
     lib = OUT / "librfsynth.so"
     tmp = OUT / "librfsynth.so.tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", str(CSRC / "synth.cpp"), "-o", str(tmp)])
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", str(CSRC / "synth.cpp"), "-o",
+          str(tmp)])
     tmp.replace(lib)
     return lib
 

@@ -96,3 +96,36 @@ def test_synthetic_pairs_are_deterministic(synth_lib):
     assert abs(np.linalg.norm(d["t"]) - 0.1) < 1e-12
     z = d["gt_Z"][:, :160]
     assert z.min() >= 2.0 - 1e-6 and z.max() <= 10.0 + 1e-6
+
+
+def test_synthetic_sequence_motion_and_poses(synth_lib):
+    # the streaming workload's trajectory: deterministic frames, per-frame rotation <= 0.5 deg,
+    # camera displacement <= 0.02, proper rotations; chaining the relative poses from frame 0 (the
+    # reference pose) gives frame k's absolute rotation, 4.5 deg * |sin(2 pi k / 60)| about one
+    # fixed axis
+    import reconflow_amd as rf
+    n = 40
+    a = rf.make_sequence(96, 64, n, 5)
+    b = rf.make_sequence(96, 64, n, 5)
+    assert np.array_equal(a["frames"], b["frames"])
+    assert a["frames"].shape == (n, 64, 96)
+    R, t = a["R"], a["t"]
+    Rc = np.eye(3)
+    axis0 = None
+    for k, (Rk, tk) in enumerate(zip(R, t), start=1):
+        np.testing.assert_allclose(Rk @ Rk.T, np.eye(3), atol=1e-12)
+        assert abs(np.linalg.det(Rk) - 1.0) < 1e-12
+        ang = np.degrees(np.arccos(np.clip((np.trace(Rk) - 1) / 2, -1, 1)))
+        assert ang <= 0.5 + 1e-9
+        assert np.linalg.norm(Rk.T @ tk) <= 0.02 + 1e-9  # camera displacement between frames
+        Rc = Rk @ Rc
+        want = 4.5 * abs(np.sin(2 * np.pi * k / 60.0))
+        ax = np.array([Rc[2, 1] - Rc[1, 2], Rc[0, 2] - Rc[2, 0], Rc[1, 0] - Rc[0, 1]])  # 2 sin(angle) axis
+        got = np.degrees(np.arctan2(np.linalg.norm(ax) / 2, (np.trace(Rc) - 1) / 2))
+        assert abs(got - want) < 1e-9, (k, got, want)
+        if want > 0.01:
+            ax /= np.linalg.norm(ax)
+            axis0 = ax if axis0 is None else axis0
+            assert abs(abs(ax @ axis0) - 1.0) < 1e-9, k
+    # consecutive frames differ (the camera moves) but stay the same scene
+    assert 0.0 < np.abs(a["frames"][1] - a["frames"][0]).mean() < 0.3
