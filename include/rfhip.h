@@ -299,6 +299,10 @@ int rf_k_median_filter(const float* u, const float* v, int w, int h, int s, floa
 int rf_k_median_filter_3d(const float* X, const float* Y, const float* Z, int w, int h, int s, float* X1,
                           float* Y1, float* Z1, int kernel_size, void* stream);              /* medianFilter.cu:246-258 */
 int rf_k_add(const float* a, const float* b, int count, float* sum, void* stream);           /* addKernel.cu:13-18 */
+/* No counterpart in the reference: q[i] = n[i] / d by the sequence the fused joint kernels use for
+ * their launch-uniform divisors (alpha_tv + alpha_proj, lambdaf + alpha_proj), for tests that
+ * compare it with IEEE FP32 division. d must be finite and positive. */
+int rf_k_uniform_divide(const float* n, int count, float d, float* q, void* stream);
 int rf_k_solve_recon_data_l1(const float* u, const float* v, const float* duhat0, const float* dvhat0,
                              const float* uproj, const float* vproj, const float* sku, const float* skv,
                              const float* pu1, const float* pu2, const float* pv1, const float* pv2,

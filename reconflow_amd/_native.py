@@ -94,6 +94,7 @@ _PROTOS = {
     "rf_k_median_filter": (_I, [_P, _P, _I, _I, _I, _P, _P, _I, _P]),
     "rf_k_median_filter_3d": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _P]),
     "rf_k_add": (_I, [_P, _P, _I, _P, _P]),
+    "rf_k_uniform_divide": (_I, [_P, _I, _F, _P, _P]),
     "rf_k_solve_recon_data_l1": (_I, [_P] * 15 + [_I, _I, _I, _F, _F, _F, _P, _P, _P]),
     "rf_k_solve_recon_data_l1_fn": (_I, [_P] * 17 + [_I, _I, _I, _F, _F, _F, _F, _P, _P, _P, _P]),
     "rf_k_solve_uvproj": (_I, [_P] * 8 + [_I, _I, _I, _F, _F, _P, _P, _P]),

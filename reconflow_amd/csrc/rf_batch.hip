@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 
 #include "rf_batch.hpp"
@@ -316,6 +317,9 @@ struct IterArgs {
     int tpp;
     float inv_tpp;
     unsigned pair_elems;
+    // the launch-uniform divisors aTv + aP and lf + aP (every joint kernel without a prior flow
+    // divides by the first, every joint kernel by the second)
+    UDiv uA, uP;
 };
 
 // Where a tile's pair lives relative to the view: element q of the pair's fields is element
@@ -412,26 +416,27 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
             if (hu) p2ud = p2u0[q - s], p2vd = p2v0[q - s];
         }
         const float uu = U[q], vv = V[q], sku = sku0[q], skv = skv0[q];
-        float AA, bu, bv;
+        float du, dv, nduh, ndvh;
         if (PRIOR) {
             const float aFn = mask ? a.aFn * mask[q] : a.aFn;
-            AA = (a.aTv + a.aP) + aFn;
-            bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, duh, up0[q], uu, sku, ufn[q]);
-            bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, dvh, vp0[q], vv, skv, vfn[q]);
+            const float AA = (a.aTv + a.aP) + aFn;
+            const float bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, duh, up0[q], uu, sku, ufn[q]);
+            const float bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, dvh, vp0[q], vv, skv, vfn[q]);
+            l1_threshold_div(Ix[q], Iy[q], Iz[q], bu, bv, a.lambda, AA, du, dv);
+            nduh = du + div_p(p1u, p1ul, p2u, p2ud, hl, hu) / AA;
+            ndvh = dv + div_p(p1v, p1vl, p2v, p2vd, hl, hu) / AA;
         } else {
-            AA = a.aTv + a.aP;
-            bu = recon_blend(a.aTv, a.aP, AA, duh, up0[q], uu, sku);
-            bv = recon_blend(a.aTv, a.aP, AA, dvh, vp0[q], vv, skv);
+            const float bu = recon_blend(a.aTv, a.aP, a.uA, duh, up0[q], uu, sku);
+            const float bv = recon_blend(a.aTv, a.aP, a.uA, dvh, vp0[q], vv, skv);
+            l1_threshold_div(Ix[q], Iy[q], Iz[q], bu, bv, a.lambda, a.uA, du, dv);
+            nduh = du + udiv(div_p(p1u, p1ul, p2u, p2ud, hl, hu), a.uA);
+            ndvh = dv + udiv(div_p(p1v, p1vl, p2v, p2vd, hl, hu), a.uA);
         }
-        float du, dv;
-        l1_threshold_div(Ix[q], Iy[q], Iz[q], bu, bv, a.lambda, AA, du, dv);
-        const float nduh = du + div_p(p1u, p1ul, p2u, p2ud, hl, hu) / AA;
-        const float ndvh = dv + div_p(p1v, p1vl, p2v, p2vd, hl, hu) / AA;
         const int e = ex + ey * EX;
         s_du[e] = nduh;
         s_dv[e] = ndvh;
-        s_up[e] = uvproj_px(a.lf, a.aP, uc0[q], uu, nduh, sku);
-        s_vp[e] = uvproj_px(a.lf, a.aP, vc0[q], vv, ndvh, skv);
+        s_up[e] = uvproj_px(a.lf, a.aP, a.uP, uc0[q], uu, nduh, sku);
+        s_vp[e] = uvproj_px(a.lf, a.aP, a.uP, vc0[q], vv, ndvh, skv);
         if (keep) {
             keep[0] = p1u;
             keep[1] = p2u;
@@ -605,24 +610,26 @@ template <bool F64TRI, bool PRIOR, bool INNER = false>
 __device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const JointView& V, const PairOff& po, int xc,
                                                     int yc, const JointIn& in) {
     const bool hl = INNER || xc > 0, hu = INNER || yc > 0;
-    // ---- data step + projection blend
-    float AA, bu, bv;
+    // ---- data step + projection blend (divisions by the launch-uniform A and lf + aP through
+    // UDiv: the IEEE quotients)
+    float du, dv, nduh, ndvh;
     if (PRIOR) {
         const float aFn = a.sparse ? a.aFn * in.msk : a.aFn;
-        AA = (a.aTv + a.aP) + aFn;
-        bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.duh, in.up0, in.uu, in.sku, in.ufn);
-        bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.dvh, in.vp0, in.vv, in.skv, in.vfn);
+        const float AA = (a.aTv + a.aP) + aFn;
+        const float bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.duh, in.up0, in.uu, in.sku, in.ufn);
+        const float bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.dvh, in.vp0, in.vv, in.skv, in.vfn);
+        l1_threshold_div(in.ix, in.iy, in.iz, bu, bv, a.lambda, AA, du, dv);
+        nduh = du + div_p(in.p1u, in.p1ul, in.p2u, in.p2ud, hl, hu) / AA;
+        ndvh = dv + div_p(in.p1v, in.p1vl, in.p2v, in.p2vd, hl, hu) / AA;
     } else {
-        AA = a.aTv + a.aP;
-        bu = recon_blend(a.aTv, a.aP, AA, in.duh, in.up0, in.uu, in.sku);
-        bv = recon_blend(a.aTv, a.aP, AA, in.dvh, in.vp0, in.vv, in.skv);
+        const float bu = recon_blend(a.aTv, a.aP, a.uA, in.duh, in.up0, in.uu, in.sku);
+        const float bv = recon_blend(a.aTv, a.aP, a.uA, in.dvh, in.vp0, in.vv, in.skv);
+        l1_threshold_div(in.ix, in.iy, in.iz, bu, bv, a.lambda, a.uA, du, dv);
+        nduh = du + udiv(div_p(in.p1u, in.p1ul, in.p2u, in.p2ud, hl, hu), a.uA);
+        ndvh = dv + udiv(div_p(in.p1v, in.p1vl, in.p2v, in.p2vd, hl, hu), a.uA);
     }
-    float du, dv;
-    l1_threshold_div(in.ix, in.iy, in.iz, bu, bv, a.lambda, AA, du, dv);
-    const float nduh = du + div_p(in.p1u, in.p1ul, in.p2u, in.p2ud, hl, hu) / AA;
-    const float ndvh = dv + div_p(in.p1v, in.p1vl, in.p2v, in.p2vd, hl, hu) / AA;
-    const float upn = uvproj_px(a.lf, a.aP, in.uc0, in.uu, nduh, in.sku);
-    const float vpn = uvproj_px(a.lf, a.aP, in.vc0, in.vv, ndvh, in.skv);
+    const float upn = uvproj_px(a.lf, a.aP, a.uP, in.uc0, in.uu, nduh, in.sku);
+    const float vpn = uvproj_px(a.lf, a.aP, a.uP, in.vc0, in.vv, ndvh, in.skv);
 
     // ---- triangulation
     float oX, oY, oZ;
@@ -1639,7 +1646,8 @@ hipError_t Batch::enqueue_t(bool recon) {
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
-                           (fn && sparse_) ? 1 : 0, JointView{}, 0, 0.0f, 0u};
+                           (fn && sparse_) ? 1 : 0, JointView{}, 0, 0.0f, 0u,
+                           make_udiv(p_.alpha_tv + p_.alpha_proj), make_udiv(p_.lambdaf + p_.alpha_proj)};
                 // one pair, or pairs of one size: the fields travel in the kernel arguments (no
                 // dependent descriptor load)
                 const bool single = (n() == 1 || pair_elems_ > 0) && cls != TC_J64 && cls != TC_FLOW;
@@ -1688,6 +1696,16 @@ int Batch::solve(bool recon, float flow_scale) {
     if (!st_) {
         set_error("rf_batch_solve: batch not created");
         return RF_ERR_STATE;
+    }
+    if (recon) {
+        // the joint kernels divide by these two launch-uniform sums through UDiv, which needs them
+        // finite and positive (the plain path takes any value)
+        const float A = p_.alpha_tv + p_.alpha_proj, P = p_.lambdaf + p_.alpha_proj;
+        if (!(A > 0.0f) || !std::isfinite(A) || !(P > 0.0f) || !std::isfinite(P)) {
+            set_error("fused joint solve: alpha_tv + alpha_proj and lambdaf + alpha_proj must be finite and "
+                      "positive; use the plain path");
+            return RF_ERR_UNSUPPORTED;
+        }
     }
     if (dirty_ && upload_pairs()) return RF_ERR_HIP;
     const int kind = recon ? 1 : 0;

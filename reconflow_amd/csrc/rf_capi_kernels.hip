@@ -1,6 +1,7 @@
 // Per-kernel C entry points (rfhip.h rf_k_*): the public kernel wrappers of the reference
 // (Cudaflow.h:101-193, Reconflow.h:86-158) on device pointers, enqueued on a caller stream.
 
+#include <cmath>
 #include <string>
 
 #include "../../include/rfhip.h"
@@ -106,6 +107,13 @@ int rf_k_median_filter_3d(const float* X, const float* Y, const float* Z, int w,
 int rf_k_add(const float* a, const float* b, int count, float* sum, void* stream) {
     if (count < 0) return RF_ERR_ARG;
     return done(launch_add(a, b, count, sum, ST), "rf_k_add");
+}
+int rf_k_uniform_divide(const float* n, int count, float d, float* q, void* stream) {
+    if (count < 0 || !std::isfinite(d) || !(d > 0.0f)) {
+        set_error("rf_k_uniform_divide: count must be >= 0 and d finite and positive");
+        return RF_ERR_ARG;
+    }
+    return done(launch_udiv(n, count, d, q, ST), "rf_k_uniform_divide");
 }
 int rf_k_solve_recon_data_l1(const float* u, const float* v, const float* duhat0, const float* dvhat0,
                              const float* uproj, const float* vproj, const float* sku, const float* skv,

@@ -81,6 +81,35 @@ __device__ __forceinline__ float d5(float m2, float m1, float p1, float p2) {
     return t;
 }
 
+// Division by a launch-uniform divisor d (finite, d > 0) that returns the IEEE FP32 quotient n / d
+// for every FP32 n, without a per-division reciprocal. With rd = 1/d rounded to nearest in binary64
+// (formed once by the host) and C = 1.5 * 2^-124:
+//   s = RN64(n rd + C) (one FMA),  q = RN64(s - C),  result RN32(q) with the sign bit of n or-ed in.
+// n rd = (n/d)(1 + e), |e| <= 2^-53. Below 2^-125 in magnitude, s lies in [2^-124, 2^-123), so q is
+// n rd rounded to the 2^-176 grid (the subtraction is exact); above, q is within 2^-51 |n/d|.
+// For FP32 n = N 2^a, d = D 2^b (N, D odd, below 2^24), n/d either lies exactly on an FP32
+// rounding boundary, which needs |n/d| < 2^-126 and puts n/d on the 2^-176 grid, at most
+// 2^-179 from n rd, so q = n/d and RN32 applies its tie rule to the exact value; or lies at
+// least min(2^-174, 2^-24 |n/d|) (below 2^-125) or 2^-49 |n/d| (above, overflow threshold
+// included) from every boundary, farther than q. Either way RN32(q) = RN32(n/d). Infinities and
+// NaNs pass through; a zero or underflowing quotient comes out +0, and or-ing the sign of n (the
+// quotient's sign, as d > 0) restores -0.
+struct UDiv {
+    double rd;
+};
+
+__device__ __forceinline__ float udiv(float n, const UDiv& u) {
+    const double C = 0x1.8p-124;
+    const double q = fma((double)n, u.rd, C) - C;
+    return __uint_as_float(__float_as_uint((float)q) | (__float_as_uint(n) & 0x80000000u));
+}
+
+// The UDiv of d (finite, d > 0), formed on the host once per launch.
+__host__ __device__ inline UDiv make_udiv(float d) { return UDiv{1.0 / (double)d}; }
+
+__device__ __forceinline__ float divide(float n, float d) { return n / d; }
+__device__ __forceinline__ float divide(float n, const UDiv& d) { return udiv(n, d); }
+
 // Divergence with the reference's left/top border rule (solveDataL1.cu:64-88). pl = p1 at
 // x-1, pd = p2 at y-1 (ignored on the first column / row).
 __device__ __forceinline__ float div_p(float p1, float p1l, float p2, float p2d, bool has_left, bool has_up) {
@@ -123,12 +152,13 @@ __device__ __forceinline__ void l1_threshold(float dix, float diy, float dit, fl
 
 // lower = ((-lambda) * g2) / A is -upper bit for bit (negation commutes with correctly rounded
 // multiplication and division), and bu - t = bu + (-t) exactly, so the two outer cases share
-// one pair of divisions.
+// one pair of divisions. DV: float, or UDiv for a launch-uniform A (same quotients).
+template <typename DV>
 __device__ __forceinline__ void l1_threshold_div(float dix, float diy, float dit, float bu, float bv, float lambda,
-                                                 float A, float& du, float& dv) {
+                                                 const DV& A, float& du, float& dv) {
     float rho = (dix * bu + diy * bv) + dit;
     float g2 = dix * dix + diy * diy;
-    float upper = (lambda * g2) / A;
+    float upper = divide(lambda * g2, A);
     float lower = -upper;
     if ((rho <= upper) && (rho >= lower)) {
         if (g2 != 0.0f) {
@@ -139,7 +169,7 @@ __device__ __forceinline__ void l1_threshold_div(float dix, float diy, float dit
             dv = bv;
         }
     } else {
-        const float tx = (lambda * dix) / A, ty = (lambda * diy) / A;
+        const float tx = divide(lambda * dix, A), ty = divide(lambda * diy, A);
         const bool below = rho < lower;
         du = below ? bu + tx : bu - tx;
         dv = below ? bv + ty : bv - ty;
@@ -191,8 +221,10 @@ __device__ __forceinline__ void charb_px(float du, float dv, float dix, float di
 }
 
 // Joint data blend (solveReconDataL1.cu:35-36): (aTv*uhat + aP*((uproj - u) - sk)) / A.
-__device__ __forceinline__ float recon_blend(float aTv, float aP, float A, float uhat, float uproj, float u, float sk) {
-    return (aTv * uhat + aP * ((uproj - u) - sk)) / A;
+template <typename DV>
+__device__ __forceinline__ float recon_blend(float aTv, float aP, const DV& A, float uhat, float uproj, float u,
+                                             float sk) {
+    return divide(aTv * uhat + aP * ((uproj - u) - sk), A);
 }
 
 // With a prior flow (solveReconDataL1.cu:172-173).
@@ -211,6 +243,11 @@ __device__ __forceinline__ void dual_px(float ux, float uy, float p1, float p2, 
 // UVProj blend (solveUVProj.cu:19-20).
 __device__ __forceinline__ float uvproj_px(float lf, float aP, float uc, float u, float du, float sk) {
     return (lf * uc + aP * ((u + du) + sk)) / (lf + aP);
+}
+
+// The same with the launch-uniform divisor lf + aP as a UDiv.
+__device__ __forceinline__ float uvproj_px(float lf, float aP, const UDiv& P, float uc, float u, float du, float sk) {
+    return udiv(lf * uc + aP * ((u + du) + sk), P);
 }
 
 // Sk update (solveUcSk.cu:93-94).

@@ -599,6 +599,17 @@ hipError_t launch_median3d(const float* X, const float* Y, const float* Z, int w
                            float* Y1, float* Z1, hipStream_t st) {
     RF_L2(k_median3d, X, Y, Z, w, h, s, X1, Y1, Z1);
 }
+__global__ void __launch_bounds__(256) k_udiv(const float* n, long count, UDiv d, float* q) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < count) q[i] = udiv(n[i], d);
+}
+
+hipError_t launch_udiv(const float* n, long count, float d, float* q, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_udiv, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, n, count, make_udiv(d), q);
+    return hipGetLastError();
+}
+
 hipError_t launch_add(const float* a, const float* b, long count, float* out, hipStream_t st) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_add, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, a, b, count, out);

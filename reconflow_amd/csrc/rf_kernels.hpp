@@ -36,6 +36,7 @@ hipError_t launch_median5(const float* u, const float* v, int w, int h, int s, f
 hipError_t launch_median3d(const float* X, const float* Y, const float* Z, int w, int h, int s, float* X1,
                            float* Y1, float* Z1, hipStream_t st);
 hipError_t launch_add(const float* a, const float* b, long count, float* out, hipStream_t st);
+hipError_t launch_udiv(const float* n, long count, float d, float* q, hipStream_t st);
 hipError_t launch_recon_data_l1(const float* u, const float* v, const float* du, const float* dv, const float* up,
                                 const float* vp, const float* sku, const float* skv, const float* pu1,
                                 const float* pu2, const float* pv1, const float* pv2, const float* Ix,

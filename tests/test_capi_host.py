@@ -77,6 +77,9 @@ def test_pyramid_level_count_matches_reference_demo(lib):
 def test_kernel_entry_points_reject_bad_geometry(lib):
     # stride smaller than width is rejected before any launch
     assert lib.rf_k_add(None, None, -1, None, None) == -1
+    # the uniform division needs a finite positive divisor
+    for d in (0.0, -2.0, float("inf"), float("nan")):
+        assert lib.rf_k_uniform_divide(None, 4, d, None, None) == -1
     assert lib.rf_k_warp_image(None, 64, 48, 32, None, None, None, None) == -1
     assert lib.rf_k_median_filter(None, None, 8, 8, 8, None, None, 3, None) == -5
 

@@ -294,3 +294,26 @@ def test_gray_conversions(lib, orc):
         g = [dzeros((h + 1) * s)]
         assert call(lib, cname, dev(arr), g[0], w, h, s) == 0
         check(w, h, s, [host(x) for x in g], res["f32"], res["f64"])
+
+
+def test_uniform_division_equals_ieee(lib):
+    """rf_k_uniform_divide (the fused joint kernels' division by a launch-uniform divisor) against
+    numpy's IEEE FP32 division, bit for bit: every subnormal and every significand at three
+    exponents, both signs, plus infinities and NaN, for the workloads' divisors, divisors with
+    exact subnormal ties (98, 6), a power of two, an odd integer and the extremes."""
+    sub = np.arange(1 << 23, dtype=np.uint32)
+    sets = [sub] + [sub | np.uint32((e + 127) << 23) for e in (-126, 0, 127)]
+    special = np.array([np.inf, -np.inf, np.nan, 0.0, -0.0], np.float32).view(np.uint32)
+    nums = np.concatenate([np.concatenate([s, s | np.uint32(0x80000000)]) for s in sets] + [special]).view(np.float32)
+    g_n = dev(nums)
+    g_q = dzeros(nums.size)
+    for d in (np.float32(33.3) + np.float32(60.0), np.float32(0.1) + np.float32(60.0), 98.0, 6.0, 1.0, 3.0, 1e-40,
+              3.4028235e38):
+        d32 = np.float32(d)
+        assert call(lib, "uniform_divide", g_n, nums.size, float(d32), g_q) == 0
+        got = host(g_q)
+        with np.errstate(all="ignore"):
+            ref = nums / d32
+        both_nan = np.isnan(ref) & np.isnan(got)
+        bad = (ref.view(np.uint32) != got.view(np.uint32)) & ~both_nan
+        assert not bad.any(), f"d={d32}: {int(bad.sum())} quotients differ, first n={nums[bad][:4]}"
