@@ -459,11 +459,11 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
         const int e = ex + ey * EX;
         float oX, oY, oZ;
         if (F64TRI)
-            solve3d_px_f64(F.pqd[l], (double)a.lf, (double)a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q],
+            solve3d_px_f64(const_view(F.pqd[l]), (double)a.lf, (double)a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q],
                            X0[q + 1], Y0[q + 1], Z0[q + 1], X0[q + s], Y0[q + s], Z0[q + s], X0[ql], Y0[ql], Z0[ql],
                            X0[qu], Y0[qu], Z0[qu], hl, hu, oX, oY, oZ);
         else
-            solve3d_px<float>(F.pq[l], a.lf, a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q], X0[q + 1],
+            solve3d_px<float>(const_view(F.pq[l]), a.lf, a.lms, x, y, s_up[e], s_vp[e], X0[q], Y0[q], Z0[q], X0[q + 1],
                               Y0[q + 1], Z0[q + 1], X0[q + s], Y0[q + s], Z0[q + s], X0[ql], Y0[ql], Z0[ql], X0[qu],
                               Y0[qu], Z0[qu], hl, hu, oX, oY, oZ);
         s_X[e] = oX;
@@ -634,10 +634,10 @@ __device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const Joi
     // ---- triangulation
     float oX, oY, oZ;
     if (F64TRI)
-        solve3d_px_f64(cam(V.pqd, po), (double)a.lf, (double)a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr,
+        solve3d_px_f64(const_view(cam(V.pqd, po)), (double)a.lf, (double)a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr,
                        in.Zr, in.Xd, in.Yd, in.Zd, in.Xl, in.Yl, in.Zl, in.Xu, in.Yu, in.Zu, hl, hu, oX, oY, oZ);
     else
-        solve3d_px<float>(cam(V.pq, po), a.lf, a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr, in.Zr, in.Xd,
+        solve3d_px<float>(const_view(cam(V.pq, po)), a.lf, a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr, in.Zr, in.Xd,
                           in.Yd, in.Zd, in.Xl, in.Yl, in.Zl, in.Xu, in.Yu, in.Zu, hl, hu, oX, oY, oZ);
     return JointFront{in.p1u, in.p2u, in.p1v, in.p2v, in.uu, in.vv, in.sku, in.skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
 }

@@ -279,18 +279,40 @@ struct PQT {
 using PQ = PQT<float>;
 using PQd = PQT<double>;
 
+// Element reads through the constant address space: scalar loads wherever the address is uniform,
+// also after barriers and global stores, where a plain read of global memory becomes a vector load
+// into registers of every lane. Only for memory no kernel writes (the per-level camera matrices,
+// written by the host between launches).
+template <typename T>
+struct ConstPtr {
+    const T* p;
+    __device__ __forceinline__ T operator[](int i) const {
+        return *(const __attribute__((address_space(4))) T*)(p + i);
+    }
+};
+
+// The camera matrices of a PQT in device memory, read through ConstPtr.
+template <typename R>
+struct PQConst {
+    ConstPtr<R> P, Q;
+};
+template <typename R>
+__device__ __forceinline__ PQConst<R> const_view(const PQT<R>& m) {
+    return PQConst<R>{{m.P}, {m.Q}};
+}
+
 // Solve3dMinimalAreaKernel for one pixel (solve3dMinimalArea.cu:21-185), evaluated in R (float
 // as the reference, or double). The neighbour values are passed in: r = X0 at ix+1, d = X0 at
 // iy+1, l = X0 at ix-1 (used when has_left), u = X0 at iy-1 (used when has_up). Column w and
 // row h are legitimately read by the caller (the reference's border guards never fire).
-template <typename R>
-__device__ __forceinline__ void solve3d_px(const PQT<R>& m, R lambdaf, R lambdams, int ix, int iy, float uproj_f,
+template <typename R, typename M = PQT<R>>
+__device__ __forceinline__ void solve3d_px(const M& m, R lambdaf, R lambdams, int ix, int iy, float uproj_f,
                                            float vproj_f, float X0f, float Y0f, float Z0f, float Xrf, float Yrf,
                                            float Zrf, float Xdf, float Ydf, float Zdf, float Xlf, float Ylf, float Zlf,
                                            float Xuf, float Yuf, float Zuf, bool has_left, bool has_up, float& oX,
                                            float& oY, float& oZ) {
-    const R* P = m.P;
-    const R* Q = m.Q;
+    const auto P = m.P;  // a PQT's arrays, or a PQConst's readers
+    const auto Q = m.Q;
     const R X0 = X0f, Y0 = Y0f, Z0 = Z0f, Xr = Xrf, Yr = Yrf, Zr = Zrf, Xd = Xdf, Yd = Ydf, Zd = Zdf;
     const R Xl = Xlf, Yl = Ylf, Zl = Zlf, Xu = Xuf, Yu = Yuf, Zu = Zuf;
     R x1 = (R)ix, y1 = (R)iy;
@@ -379,13 +401,14 @@ __device__ __forceinline__ void solve3d_px(const PQT<R>& m, R lambdaf, R lambdam
 // matrix and its adjugate are formed once, the determinant is the cofactor expansion along the
 // first row and the solution is adjugate times b times one reciprocal of the determinant.
 // Neighbours and the fallback as in solve3d_px.
-__device__ __forceinline__ void solve3d_px_f64(const PQd& m, double lambdaf, double lambdams, int ix, int iy,
+template <typename M>
+__device__ __forceinline__ void solve3d_px_f64(const M& m, double lambdaf, double lambdams, int ix, int iy,
                                                float uproj_f, float vproj_f, float X0f, float Y0f, float Z0f,
                                                float Xrf, float Yrf, float Zrf, float Xdf, float Ydf, float Zdf,
                                                float Xlf, float Ylf, float Zlf, float Xuf, float Yuf, float Zuf,
                                                bool hl, bool hu, float& oX, float& oY, float& oZ) {
-    const double* P = m.P;
-    const double* Q = m.Q;
+    const auto P = m.P;  // a PQd's arrays, or a PQConst<double>'s readers
+    const auto Q = m.Q;
     const double x1 = (double)ix, y1 = (double)iy, x2 = (double)ix + (double)uproj_f,
                  y2 = (double)iy + (double)vproj_f;
     double pp[4], qq[4], rr[4], ss[4];

@@ -242,6 +242,29 @@ def test_tiny_images_every_tiling(orc):
         assert np.array_equal(u[:, :40], ru[:, :40]) and np.array_equal(v[:, :40], rv[:, :40]), tile
 
 
+def test_fused_joint_solve_rejects_nonpositive_uniform_divisors(orc):
+    # the fused joint kernels divide by alpha_tv + alpha_proj through the binary64 reciprocal,
+    # which needs the sum finite and positive: a zero sum is an error on the fused path and an
+    # ordinary (all-NaN) solve on the plain path, which equals the oracle
+    print("case: joint 40x24 alpha_tv + alpha_proj = 0, fused and plain", flush=True)
+    d = rf.make_recon_pair(40, 24, 5)
+    for path in (rf.PATH_FUSED, rf.PATH_PLAIN):
+        r = rf.ReconFlow()
+        r._initializeR(40, 24, 1, 2, 2.0, 14, 50.0, 0.0, 0.1, 100.0, 33.3, -33.3, 0.125, 1, 2,
+                       cvType=rf.INPUT_32F, path=path, use_graph=False)
+        r.setCameraMatrices(d["K"], d["K"])
+        r.copyImagesToDevice(d["i0"], d["i1"])
+        if path == rf.PATH_FUSED:
+            with pytest.raises(RuntimeError, match="finite and positive"):
+                r.solveR(d["R"], d["t"], 50.0)
+        else:
+            r.solveR(d["R"], d["t"], 50.0)
+            got = (*r.copyOpticalFlowToHost(), *r.copy3dToHost())
+            p = orc.recon_params(40, 24, 2, n_warp=1, n_iter=2, alpha_proj=-33.3)
+            ref = orc.solve_recon(p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f32")
+            assert _same(got, ref, 40)
+
+
 @pytest.mark.parametrize("path", [rf.PATH_PLAIN, rf.PATH_FUSED], ids=["plain-graph", "fused-graph"])
 def test_recon_graph_replay_follows_pose_changes(orc, path):
     # the captured schedule reads the per-level camera matrices from device memory: a new pose
