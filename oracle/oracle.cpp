@@ -582,16 +582,19 @@ static inline void tri_fma(const double* P, const double* Q, double lambdaf, dou
         rr[k] = fma(x2, Q[8 + k], -Q[k]);
         ss[k] = fma(y2, Q[8 + k], -Q[4 + k]);
     }
+    const double Xx = Xr - X0, Yx = Yr - Y0, Zx = Zr - Z0;
+    const double Xy = Xd - X0, Yy = Yd - Y0, Zy = Zd - Z0;
+    const double c1 = fma(Yx, Zy, -(Zx * Yy)), c2 = fma(Zx, Xy, -(Xx * Zy)), c3 = fma(Xx, Yy, -(Yx * Xy));
+    // the system scaled by sq = sqrt(|Xx x Xy|^2 + 0.001): data terms weighted lambdaf * sq, the
+    // regulariser lambdams (the same solution, no division)
+    const double sq = std::sqrt(fma(c3, c3, fma(c2, c2, c1 * c1)) + (double)0.001f);
+    const double ls = lambdaf * sq, lm = lambdams;
     auto dot = [&](int i, int j) {
-        return lambdaf * fma(ss[i], ss[j], fma(rr[i], rr[j], fma(qq[i], qq[j], pp[i] * pp[j])));
+        return ls * fma(ss[i], ss[j], fma(rr[i], rr[j], fma(qq[i], qq[j], pp[i] * pp[j])));
     };
     const double ax = dot(0, 0), bx = dot(0, 1), cx = dot(0, 2), dx = dot(0, 3);
     const double by = dot(1, 1), cy = dot(1, 2), dy = dot(1, 3);
     const double cz = dot(2, 2), dz = dot(2, 3);
-    const double Xx = Xr - X0, Yx = Yr - Y0, Zx = Zr - Z0;
-    const double Xy = Xd - X0, Yy = Yd - Y0, Zy = Zd - Z0;
-    const double c1 = fma(Yx, Zy, -(Zx * Yy)), c2 = fma(Zx, Xy, -(Xx * Zy)), c3 = fma(Xx, Yy, -(Yx * Xy));
-    const double lm = lambdams / std::sqrt(fma(c3, c3, fma(c2, c2, c1 * c1)) + (double)0.001f);
     const double Xib = hl ? fma(0.5, Xl, 0.5 * Xr) : Xr, Yib = hl ? fma(0.5, Yl, 0.5 * Yr) : Yr,
                  Zib = hl ? fma(0.5, Zl, 0.5 * Zr) : Zr;
     const double Xjb = hu ? fma(0.5, Xu, 0.5 * Xd) : Xd, Yjb = hu ? fma(0.5, Yu, 0.5 * Yd) : Yd,
@@ -649,8 +652,10 @@ static inline void tri_fma(const double* P, const double* Q, double lambdaf, dou
 // The FP64 triangulation mode: the normal equations of Solve3dMinimalAreaKernel
 // (solve3dMinimalArea.cu:21-185) built and solved in FP64 with fused multiply-adds, on T-typed
 // fields (inputs widened, outputs rounded). Same terms as solve3d; the evaluation differs: the
-// 4x3 products and all sums of products are FMA chains, the symmetric normal matrix and its
-// adjugate are formed once (a21 = a12, a31 = a13, a32 = a23 hold bit for bit in solve3d too),
+// 4x3 products and all sums of products are FMA chains, the system is scaled by
+// sq = sqrt(|Xx x Xy|^2 + 0.001) (data terms weighted lambdaf * sq, the regulariser lambdams, no
+// division), the symmetric normal matrix and its adjugate are formed once (a21 = a12, a31 = a13,
+// a32 = a23 hold bit for bit in solve3d too),
 // the determinant is the cofactor expansion along the first row, and the solution is the
 // adjugate times b scaled by one reciprocal of the determinant. Differs from solve3d<double> by
 // FP64 rounding only (pinned against it in the tests).

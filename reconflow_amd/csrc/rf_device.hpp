@@ -397,10 +397,12 @@ __device__ __forceinline__ void solve3d_px(const M& m, R lambdaf, R lambdams, in
 
 // The FP64 triangulation mode for one pixel: the normal equations of Solve3dMinimalAreaKernel
 // (solve3dMinimalArea.cu:21-185) built and solved in FP64 with fused multiply-adds. Same terms
-// as solve3d_px; the 4x3 products and every sum of products are FMA chains, the symmetric normal
-// matrix and its adjugate are formed once, the determinant is the cofactor expansion along the
-// first row and the solution is adjugate times b times one reciprocal of the determinant.
-// Neighbours and the fallback as in solve3d_px.
+// as solve3d_px; the 4x3 products and every sum of products are FMA chains, the system is scaled
+// by sq = sqrt(|Xx x Xy|^2 + 0.001) (data terms weighted lambdaf * sq, the regulariser lambdams,
+// so no division forms lambdams / sq), the symmetric normal matrix and its adjugate are formed
+// once, the determinant is the cofactor expansion along the first row and the solution is
+// adjugate times b times one reciprocal of the determinant. Neighbours and the fallback as in
+// solve3d_px.
 template <typename M>
 __device__ __forceinline__ void solve3d_px_f64(const M& m, double lambdaf, double lambdams, int ix, int iy,
                                                float uproj_f, float vproj_f, float X0f, float Y0f, float Z0f,
@@ -419,17 +421,20 @@ __device__ __forceinline__ void solve3d_px_f64(const M& m, double lambdaf, doubl
         rr[k] = fma(x2, Q[8 + k], -Q[k]);
         ss[k] = fma(y2, Q[8 + k], -Q[4 + k]);
     }
-    auto dot = [&](int i, int j) {
-        return lambdaf * fma(ss[i], ss[j], fma(rr[i], rr[j], fma(qq[i], qq[j], pp[i] * pp[j])));
-    };
-    const double ax = dot(0, 0), bx = dot(0, 1), cx = dot(0, 2), dx = dot(0, 3);
-    const double by = dot(1, 1), cy = dot(1, 2), dy = dot(1, 3);
-    const double cz = dot(2, 2), dz = dot(2, 3);
     const double X0 = X0f, Y0 = Y0f, Z0 = Z0f, Xr = Xrf, Yr = Yrf, Zr = Zrf, Xd = Xdf, Yd = Ydf, Zd = Zdf;
     const double Xx = Xr - X0, Yx = Yr - Y0, Zx = Zr - Z0;
     const double Xy = Xd - X0, Yy = Yd - Y0, Zy = Zd - Z0;
     const double c1 = fma(Yx, Zy, -(Zx * Yy)), c2 = fma(Zx, Xy, -(Xx * Zy)), c3 = fma(Xx, Yy, -(Yx * Xy));
-    const double lm = lambdams / sqrt(fma(c3, c3, fma(c2, c2, c1 * c1)) + (double)0.001f);
+    // the system scaled by sq = sqrt(|Xx x Xy|^2 + 0.001): data terms weighted lambdaf * sq, the
+    // regulariser lambdams (the same solution, no division)
+    const double sq = sqrt(fma(c3, c3, fma(c2, c2, c1 * c1)) + (double)0.001f);
+    const double ls = lambdaf * sq, lm = lambdams;
+    auto dot = [&](int i, int j) {
+        return ls * fma(ss[i], ss[j], fma(rr[i], rr[j], fma(qq[i], qq[j], pp[i] * pp[j])));
+    };
+    const double ax = dot(0, 0), bx = dot(0, 1), cx = dot(0, 2), dx = dot(0, 3);
+    const double by = dot(1, 1), cy = dot(1, 2), dy = dot(1, 3);
+    const double cz = dot(2, 2), dz = dot(2, 3);
     const double Xib = hl ? fma(0.5, (double)Xlf, 0.5 * Xr) : Xr, Yib = hl ? fma(0.5, (double)Ylf, 0.5 * Yr) : Yr,
                  Zib = hl ? fma(0.5, (double)Zlf, 0.5 * Zr) : Zr;
     const double Xjb = hu ? fma(0.5, (double)Xuf, 0.5 * Xd) : Xd, Yjb = hu ? fma(0.5, (double)Yuf, 0.5 * Yd) : Yd,
