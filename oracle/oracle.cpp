@@ -3,7 +3,7 @@
 // CPU restatement of the Reconflow coarse-to-fine TV-L1 optical-flow path and its
 // epipolar-constrained joint flow/depth path, written from the text of the reference
 // kernels. Every routine is a template over the arithmetic type T:
-//   * T = double  -> the FP64 oracle that judges the GPU path (tolerances);
+//   * T = double  -> the FP64 oracle the GPU path is measured against (tolerances);
 //   * T = float   -> an FP32 restatement with the reference's exact operation order,
 //                    built with -ffp-contract=off, used to show bit-level agreement.
 // Only tests, the smoke check and the bench's CPU-baseline leg load this library.

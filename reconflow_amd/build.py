@@ -80,7 +80,7 @@ def build_hip(jobs: int = 8, csrc: Path | None = None, out: Path | None = None) 
 
 
 def build_demo() -> Path:
-    """Link the C++ command-line driver over include/rfhip_cudaflow.hpp (needs both libraries)."""
+    """Link the C++ command-line program over include/rfhip_cudaflow.hpp (needs both libraries)."""
     exe = OUT / "rf_demo"
     tmp = OUT / "rf_demo.tmp"
     _run(["g++", "-O2", "-std=c++17", str(CSRC / "rf_demo.cpp"), "-I", str(ROOT / "include"), "-L", str(OUT),

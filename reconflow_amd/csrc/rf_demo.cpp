@@ -1,4 +1,4 @@
-// Command-line driver over the C++ drop-in classes (include/rfhip_cudaflow.hpp), in the shape of
+// Command-line program over the C++ drop-in classes (include/rfhip_cudaflow.hpp), in the shape of
 // the reference demo (test_reconflow/main.cpp:16-141): build the solver, copy a pair in, solve,
 // copy the results out. The pair comes from the seeded generator; results are written as raw
 // float32 planes (u, v[, X, Y, Z]) of stride x height each.
