@@ -169,6 +169,15 @@ int rf_set_camera(rf_context* ctx, const double K0[9], const double K1[9]);
  * copySparseOpticalFlowToDevice (Cudaflow.cu:332-343); mask may be NULL (all ones). */
 int rf_copy_initial_flow(rf_context* ctx, const float* u, const float* v);
 int rf_copy_sparse_flow(rf_context* ctx, const float* u, const float* v, const float* mask);
+/* Warm-start input, no counterpart in the reference (whose solves start from zero flow,
+ * Cudaflow.cu:467-468, Reconflow.cu:209-220): stride x height host flow (u, v) that replaces the
+ * zero initial flow at the coarsest level of the following solves, downscaled there as the prior
+ * flow is (DownscaleScalingKernel from level 0, downscale.cu:28-48). A warm start with a
+ * non-finite component, or one larger than max(width, height), over the width x height pixels
+ * is not used: that solve starts from zero. NULL u or v clears it. Synchronous. */
+int rf_copy_warm_start(rf_context* ctx, const float* u, const float* v);
+/* 1 when a warm start is set and the last solve used it, 0 otherwise; synchronises the stream. */
+int rf_warm_start_in_use(rf_context* ctx);
 /* CudaFlow::copyMaskToDevice (Inpainting.cu:3-7): stride x height float masks of
  * METHOD_TVL1_INPAINTING (0 = no data term at the pixel); mask1 is kept and, as in the
  * reference, not used by the data step. Synchronous. */
@@ -266,6 +275,12 @@ int rf_batch_synchronize(rf_batch* b);
 int rf_batch_get_result_device(const rf_batch* b, int i, const float** u, const float** v, const float** X,
                                const float** Y, const float** Z);
 void* rf_batch_get_stream(const rf_batch* b);
+/* Warm-start input of pair i (no counterpart in the reference; the rule of rf_copy_warm_start):
+ * device flow (stride_i x height_i floats each), read by every solve until replaced, for example
+ * the previous frame's result of a stream; NULL clears. rf_batch_warm_start_in_use: 1 when the
+ * last solve used it, 0 otherwise (synchronises the batch stream). */
+int rf_batch_set_warm_start_device(rf_batch* b, int i, const float* d_u, const float* d_v);
+int rf_batch_warm_start_in_use(rf_batch* b, int i);
 /* The iteration tiling per level for levels 0..n-1: RF_JOINT_TILE_* for a joint batch,
  * RF_FLOW_TILE_* for a TV-L1 batch (AUTO until the tuning solves ran). */
 int rf_batch_tiling(const rf_batch* b, int* tiling, int n);

@@ -1104,14 +1104,48 @@ static void gray_input(const Params& P, const void* in, T* out, int s0) {
     else gray_f32((const float*)in, out, P.width, P.height, s0);
 }
 
+// Warm start (this implementation's addition; the reference always starts from zero,
+// Cudaflow.cu:467-468, Reconflow.cu:209-220): level-0 flow ws (stride s0) replaces the zero
+// initial u, v at the coarsest level, downscaled there by the prior flow's operator
+// (DownscaleScalingKernel straight from level 0, as Reconflow.cu:232-238 does for the prior). A
+// warm start with a non-finite component, or one larger than max(w0, h0), over the w0 x h0
+// pixels is not used.
+inline bool warm_start_usable(const float* wsu, const float* wsv, int w0, int h0, int s0) {
+    if (!wsu || !wsv) return false;
+    const float bound = (float)std::max(w0, h0);
+    for (int y = 0; y < h0; ++y)
+        for (int x = 0; x < w0; ++x) {
+            const long i = x + (long)y * s0;
+            if (!(std::fabs(wsu[i]) <= bound) || !(std::fabs(wsv[i]) <= bound)) return false;
+        }
+    return true;
+}
+
+template <typename T>
+void apply_warm_start(const float* wsu, const float* wsv, const std::vector<int>& pW, const std::vector<int>& pH,
+                      const std::vector<int>& pS, Pool<T>& pool, T* u, T* v) {
+    const int L = (int)pW.size() - 1;
+    if (!warm_start_usable(wsu, wsv, pW[0], pH[0], pS[0])) return;
+    const long n = (long)pS[0] * pH[0];
+    T *a = pool.make(), *b = pool.make();
+    for (long i = 0; i < n; ++i) {
+        a[i] = (T)wsu[i];
+        b[i] = (T)wsv[i];
+    }
+    const float sc = (float)pW[L] / (float)pW[0];
+    downscale(a, pW[0], pH[0], pS[0], pW[L], pH[L], pS[L], (T)sc, true, u);
+    downscale(b, pW[0], pH[0], pS[0], pW[L], pH[L], pS[L], (T)sc, true, v);
+}
+
 // A11: CudaFlow::_solveOpticalFlow for METHOD_TVL1 (Cudaflow.cu:403-621), with the
 // pointer swaps mirrored. Images are stride x height, outputs u, v likewise.
-template <typename T>
 // Method branches follow Cudaflow.cu:426-578: Charbonnier-gradient for methods 2 and 7 (with
 // gradient pyramids, :426-461, and three warps / derivative sets per warp, :489-504),
 // Charbonnier for method 1, patch derivatives for method 3, the L1 step otherwise. With mask0_in
 // it is CudaFlow::_solveInpainting (Inpainting.cu:13-152): the masked L1 step, plain derivatives.
-int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, float* v_out, const float* mask0_in = nullptr) {
+template <typename T>
+int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, float* v_out, const float* mask0_in = nullptr,
+               const float* wsu = nullptr, const float* wsv = nullptr) {
     std::vector<int> pW, pH, pS;
     pyramid(P, pW, pH, pS);
     const int w0 = pW[0], h0 = pH[0], s0 = pS[0];
@@ -1176,6 +1210,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
     }
     zero(u, n);
     zero(v, n);
+    apply_warm_start(wsu, wsv, pW, pH, pS, pool, u, v);
     const T lambda = P.lambda, lambdagrad = P.lambdagrad, theta = P.theta, tau = P.tau;
     for (int level = P.n_levels - 1; level >= 0; --level) {
         const int w = pW[level], h = pH[level], s = pS[level];
@@ -1272,7 +1307,7 @@ template <typename T>
 int solve_recon(const Params& P, const double* K0, const double* K1, const double* R0, const double* t0,
                 const double* R1, const double* t1, const void* i0, const void* i1, const float* ufn_in,
                 const float* vfn_in, const float* mask_in, float* u_out, float* v_out, float* X_out,
-                float* Y_out, float* Z_out) {
+                float* Y_out, float* Z_out, const float* wsu = nullptr, const float* wsv = nullptr) {
     std::vector<int> pW, pH, pS;
     pyramid(P, pW, pH, pS);
     const int h0 = pH[0], s0 = pS[0];
@@ -1320,6 +1355,7 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
         downscale(pI1[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI1[l]);
     }
     for (T* b : {u, v, uproj, vproj, uc, vc, X, Y, Z, sku, skv}) zero(b, n);
+    apply_warm_start(wsu, wsv, pW, pH, pS, pool, u, v);
 
     const T lambda = P.lambda, tau = P.tau, alphaTv = P.alpha_tv, alphaProj = P.alpha_proj;
     const T thetaTv = (T)(1.0f / P.alpha_tv);  // Reconflow.cu:32
@@ -1498,18 +1534,25 @@ void orc_gray_u16_f32(const uint16_t* in, float* out, int w, int h, int s) { orc
                              const T* up, const T* vp, int w, int h, int s, T* sku1, T* skv1) {              \
         orc::update_sk<T>(sku, skv, u, v, du, dv, up, vp, w, h, s, sku1, skv1);                              \
     }                                                                                                        \
-    int orc_solve_flow_##SUF(const orc::Params* P, const void* i0, const void* i1, float* u, float* v) {     \
-        return orc::solve_flow<T>(*P, i0, i1, u, v);                                                         \
+    int orc_solve_flow_##SUF(const orc::Params* P, const void* i0, const void* i1, const float* wsu,        \
+                             const float* wsv, float* u, float* v) {                                         \
+        return orc::solve_flow<T>(*P, i0, i1, u, v, nullptr, wsu, wsv);                                      \
     }                                                                                                        \
     int orc_solve_recon_##SUF(const orc::Params* P, const double* K0, const double* K1, const double* R0,    \
                               const double* t0, const double* R1, const double* t1, const void* i0,          \
                               const void* i1, const float* ufn, const float* vfn, const float* mask,         \
-                              float* u, float* v, float* X, float* Y, float* Z) {                            \
-        return orc::solve_recon<T>(*P, K0, K1, R0, t0, R1, t1, i0, i1, ufn, vfn, mask, u, v, X, Y, Z);      \
+                              const float* wsu, const float* wsv, float* u, float* v, float* X, float* Y,    \
+                              float* Z) {                                                                    \
+        return orc::solve_recon<T>(*P, K0, K1, R0, t0, R1, t1, i0, i1, ufn, vfn, mask, u, v, X, Y, Z, wsu,  \
+                                   wsv);                                                                     \
     }
 
 DEF2(double, f64)
 DEF2(float, f32)
+
+int orc_warm_start_usable(const float* wsu, const float* wsv, int w0, int h0, int s0) {
+    return orc::warm_start_usable(wsu, wsv, w0, h0, s0) ? 1 : 0;
+}
 
 #define DEF3(T, SUF)                                                                                          \
     void orc_scene_flow_to_hsv_##SUF(const T* sf, int w, int h, int s, T* rgb, float fs) {                     \

@@ -113,15 +113,32 @@ def pyramid(p: Params):
     return list(W), list(H), list(S)
 
 
-def solve_flow(p: Params, i0: np.ndarray, i1: np.ndarray, kind: str = "f64"):
-    """Whole CudaFlow TV-L1 solve. Images are (h, stride) float32 (or uint8 rgb)."""
+def _ws(ws):
+    """A warm start (u, v), level-0 (h, stride) float32 planes, as two pointers (or two NULLs)."""
+    if ws is None:
+        return [None, None], []
+    keep = [np.ascontiguousarray(a, np.float32) for a in ws]
+    return [_p(a) for a in keep], keep
+
+
+def warm_start_usable(wsu, wsv, w, h) -> bool:
+    """The warm-start rule: every component over the w x h pixels finite and at most max(w, h)."""
+    a, b = (np.ascontiguousarray(x, np.float32) for x in (wsu, wsv))
+    return bool(lib().orc_warm_start_usable(_p(a), _p(b), int(w), int(h), int(a.shape[1])))
+
+
+def solve_flow(p: Params, i0: np.ndarray, i1: np.ndarray, kind: str = "f64", ws=None):
+    """Whole CudaFlow TV-L1 solve. Images are (h, stride) float32 (or uint8 rgb). ws: an optional
+    warm start (u, v) that replaces the zero initial flow at the coarsest level."""
     s0 = pyramid(p)[2][0]
     u = np.zeros((p.height, s0), np.float32)
     v = np.zeros_like(u)
     f = getattr(lib(), f"orc_solve_flow_{kind}")
     f.restype = C.c_int
-    rc = f(C.byref(p), _p(np.ascontiguousarray(i0)), _p(np.ascontiguousarray(i1)), _p(u), _p(v))
+    wsp, keep = _ws(ws)
+    rc = f(C.byref(p), _p(np.ascontiguousarray(i0)), _p(np.ascontiguousarray(i1)), *wsp, _p(u), _p(v))
     assert rc == 0
+    del keep
     return u, v
 
 
@@ -138,16 +155,19 @@ def solve_flow_masked(p: Params, i0: np.ndarray, i1: np.ndarray, mask0: np.ndarr
     return u, v
 
 
-def solve_recon(p: Params, K0, K1, R0, t0, R1, t1, i0, i1, kind="f64", ufn=None, vfn=None, mask=None):
+def solve_recon(p: Params, K0, K1, R0, t0, R1, t1, i0, i1, kind="f64", ufn=None, vfn=None, mask=None, ws=None):
     s0 = pyramid(p)[2][0]
     outs = [np.zeros((p.height, s0), np.float32) for _ in range(5)]
     dd = [np.ascontiguousarray(np.asarray(m, np.float64).ravel()) for m in (K0, K1, R0, t0, R1, t1)]
     f = getattr(lib(), f"orc_solve_recon_{kind}")
     f.restype = C.c_int
-    opt = [None if a is None else _p(np.ascontiguousarray(a, np.float32)) for a in (ufn, vfn, mask)]
+    optk = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (ufn, vfn, mask)]
+    opt = [None if a is None else _p(a) for a in optk]
+    wsp, keep = _ws(ws)
     rc = f(C.byref(p), *[_p(d) for d in dd], _p(np.ascontiguousarray(i0)), _p(np.ascontiguousarray(i1)),
-           *opt, *[_p(o) for o in outs])
+           *opt, *wsp, *[_p(o) for o in outs])
     assert rc == 0
+    del keep, optk
     return tuple(outs)
 
 

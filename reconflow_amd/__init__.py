@@ -109,6 +109,24 @@ class CudaFlow:
             raise ValueError(f"image must be {shape} (padded to the stride), got {a.shape}")
         return a
 
+    def copyWarmStartToDevice(self, u=None, v=None) -> int:
+        """Warm-start input (not in the reference): (h, stride) float32 flow that replaces the zero
+        initial flow at the coarsest level of the following solves; a non-finite component or one
+        larger than max(width, height) makes a solve start from zero; None clears."""
+        if u is None or v is None:
+            check(self._lib.rf_copy_warm_start(self._ctx, None, None), "rf_copy_warm_start")
+            return 0
+        u, v = (np.ascontiguousarray(a, np.float32) for a in (u, v))
+        check(self._lib.rf_copy_warm_start(self._ctx, u.ctypes.data, v.ctypes.data), "rf_copy_warm_start")
+        return 0
+
+    def warmStartInUse(self) -> bool:
+        """True when a warm start is set and the last solve used it."""
+        rc = self._lib.rf_warm_start_in_use(self._ctx)
+        if rc < 0:
+            check(rc, "rf_warm_start_in_use")
+        return rc == 1
+
     def copyImagesToDevice(self, i0, i1) -> int:
         """CudaFlow::copyImagesToDevice (Cudaflow.cu:288-311)."""
         a0, a1 = self._img(i0), self._img(i1)
@@ -326,6 +344,17 @@ class Batch:
     def set_camera(self, i, K0, K1, R1, t1) -> None:
         ps = [_dptr(a) for a in (K0, K1, R1, t1)]
         check(self._lib.rf_batch_set_camera(self._b, i, *[p for p, _ in ps]), "rf_batch_set_camera")
+
+    def set_warm_start_device(self, i: int, d_u: int, d_v: int) -> None:
+        """Warm start of pair i from device flow (stride_i x height_i floats each); 0 clears."""
+        check(self._lib.rf_batch_set_warm_start_device(self._b, i, d_u or None, d_v or None),
+              "rf_batch_set_warm_start_device")
+
+    def warm_start_in_use(self, i: int) -> bool:
+        rc = self._lib.rf_batch_warm_start_in_use(self._b, i)
+        if rc < 0:
+            check(rc, "rf_batch_warm_start_in_use")
+        return rc == 1
 
     def solve(self) -> None:
         check(self._lib.rf_batch_solve(self._b), "rf_batch_solve")

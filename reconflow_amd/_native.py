@@ -49,6 +49,8 @@ _PROTOS = {
     "rf_set_camera": (_I, [_P, _D, _D]),
     "rf_copy_initial_flow": (_I, [_P, _P, _P]),
     "rf_copy_sparse_flow": (_I, [_P, _P, _P, _P]),
+    "rf_copy_warm_start": (_I, [_P, _P, _P]),
+    "rf_warm_start_in_use": (_I, [_P]),
     "rf_solve_flow": (_I, [_P, _F]),
     "rf_copy_masks": (_I, [_P, _P, _P]),
     "rf_copy_depth": (_I, [_P, _P, _P]),
@@ -78,6 +80,8 @@ _PROTOS = {
     "rf_batch_get_result_device": (_I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                         C.POINTER(_P)]),
     "rf_batch_get_stream": (_P, [_P]),
+    "rf_batch_set_warm_start_device": (_I, [_P, _I, _P, _P]),
+    "rf_batch_warm_start_in_use": (_I, [_P, _I]),
     "rf_batch_tiling": (_I, [_P, C.POINTER(_I), _I]),
     # per-kernel entry points (device pointers as ints, stream last)
     "rf_k_rgb_to_gray": (_I, [_P, _P, _I, _I, _I, _P]),

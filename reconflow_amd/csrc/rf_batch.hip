@@ -1109,14 +1109,20 @@ __global__ void __launch_bounds__(256) kb_zero_uhat(Launch L, int l, int pd) {
 Batch::~Batch() { release(); }
 
 // Teardown: a failing destroy or free leaves nothing to recover, so its status is not checked.
-void Batch::release() {
-    for (auto& e : prof_.ev) (void)hipEventDestroy(e);
-    prof_.ev.clear();
+void Batch::drop_graphs() {
     for (auto& g : gexec_)
         if (g) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
+}
+
+void Batch::release() {
+    for (auto& e : prof_.ev) (void)hipEventDestroy(e);
+    prof_.ev.clear();
+    drop_graphs();
+    if (ws_bad_) (void)hipFree(ws_bad_);
+    ws_bad_ = nullptr;
     if (maps_) (void)hipFree(maps_);
     if (dp_) (void)hipFree(dp_);
     if (slab_) (void)hipFree(slab_);
@@ -1204,6 +1210,14 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         return RF_ERR_ALLOC;
     }
     if (check(hipMemset(slab_, 0, total), "hipMemset")) return RF_ERR_HIP;
+    wsu_.assign(n, nullptr);
+    wsv_.assign(n, nullptr);
+    if (hipMalloc(&ws_bad_, sizeof(int) * n) != hipSuccess) {
+        ws_bad_ = nullptr;
+        set_error("rf_batch_create: device allocation of the warm-start flags failed");
+        release();
+        return RF_ERR_ALLOC;
+    }
     hp_.assign(n, PairFields{});
     float* cur = (float*)slab_;
     for (int i = 0; i < n; ++i) {
@@ -1526,6 +1540,33 @@ int Batch::set_prior(int i, const float* u, const float* v, const float* mask, b
     return check(hipStreamSynchronize(st_), "copy prior");
 }
 
+// Warm start (no counterpart in the reference): the schedule gains three launches per pair with
+// one, so a captured schedule is captured again when a pair gains or loses its warm start.
+int Batch::set_warm_start(int i, const float* u, const float* v) {
+    if (i < 0 || i >= n()) {
+        set_error("rf_batch_set_warm_start_device: bad pair index");
+        return RF_ERR_ARG;
+    }
+    if (!u || !v) u = v = nullptr;
+    if ((u != nullptr) != (wsu_[i] != nullptr) || u != wsu_[i] || v != wsv_[i]) drop_graphs();
+    wsu_[i] = u;
+    wsv_[i] = v;
+    return RF_OK;
+}
+
+int Batch::warm_start_in_use(int i) {
+    if (i < 0 || i >= n()) {
+        set_error("rf_batch_warm_start_in_use: bad pair index");
+        return RF_ERR_ARG;
+    }
+    if (!wsu_[i]) return 0;
+    int bad = 1;
+    if (check(hipStreamSynchronize(st_), "warm start flag") ||
+        check(hipMemcpy(&bad, ws_bad_ + i, sizeof bad, hipMemcpyDeviceToHost), "warm start flag"))
+        return RF_ERR_HIP;
+    return bad ? 0 : 1;
+}
+
 template <typename TS, int SM, bool F64, bool PRIOR>
 static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
     const dim3 grid(a.L.ntiles);
@@ -1627,6 +1668,12 @@ hipError_t Batch::enqueue_t(bool recon) {
     RF_LAUNCH(kb_gray<TI>, tiles(0, TC_PIX), p_.input_type);
     for (int l = 1; l < L; ++l) RF_LAUNCH(kb_pyramid<TI>, tiles(l, TC_PIX), l);
     RF_LAUNCH(kb_zero, tiles(FLAT, TC_PIX), recon ? 1 : 0);
+    for (int i = 0; i < n(); ++i)
+        if (wsu_[i]) {
+            const Geo& g = geo_[i];
+            RF_TRY(launch_warm_start(wsu_[i], wsv_[i], g.W[0], g.H[0], g.S[0], g.W[L - 1], g.H[L - 1], g.S[L - 1],
+                                     ws_bad_ + i, hp_[i].u[0], hp_[i].v[0], st_));
+        }
     const bool fn = recon && prior_ && (p_.method == RF_METHODR_TVL1_MS_FN || p_.method == RF_METHODR_TVL1_MS_FNSPARSE);
     const float theta = recon ? 1.0f / p_.alpha_tv : p_.theta;  // Reconflow.cu:32
     for (int level = L - 1; level >= 0; --level) {
@@ -1869,6 +1916,16 @@ int rf_batch_get_result_device(const rf_batch* b, int i, const float** u, const 
 }
 
 void* rf_batch_get_stream(const rf_batch* b) { return b ? (void*)b->b.stream() : nullptr; }
+
+int rf_batch_set_warm_start_device(rf_batch* b, int i, const float* d_u, const float* d_v) {
+    if (!b) return RF_ERR_ARG;
+    return b->b.set_warm_start(i, d_u, d_v);
+}
+
+int rf_batch_warm_start_in_use(rf_batch* b, int i) {
+    if (!b) return RF_ERR_ARG;
+    return b->b.warm_start_in_use(i);
+}
 
 int rf_batch_tiling(const rf_batch* b, int* tiling, int n) {
     if (!b || !tiling || n < 0) return RF_ERR_ARG;

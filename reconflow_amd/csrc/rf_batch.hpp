@@ -86,6 +86,10 @@ class Batch {
                    const double* t1);
     int set_prior(int i, const float* u, const float* v, const float* mask, bool sparse);
     void enable_prior() { prior_ = true; }
+    // warm start of pair i: device flow in the pair's level-0 layout, read at every solve until
+    // replaced (null clears); warm_start_in_use: 1 when the last solve found it usable
+    int set_warm_start(int i, const float* u, const float* v);
+    int warm_start_in_use(int i);
     int solve(bool recon, float flow_scale);
     int sync();
     void result(int i, const float** u, const float** v, const float** X, const float** Y, const float** Z) const;
@@ -131,9 +135,12 @@ class Batch {
     std::vector<int> cls_[2];
     bool tuned_[2] = {false, false}, tuning_ = false;
     int tune_round_ = 0;
-    unsigned pair_elems_ = 0;  // pairs of one size: floats between their fields (0 otherwise)  // tuning solve in progress: candidate of warp w is (round * warps + w) mod count
+    unsigned pair_elems_ = 0;  // pairs of one size: floats between their fields (0 otherwise)
     std::vector<hipEvent_t> tev_;
     hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    std::vector<const float*> wsu_, wsv_;  // warm starts (null: none)
+    int* ws_bad_ = nullptr;                // per pair: the warm-start rule failed in the last solve
+    void drop_graphs();
     Parity gfinal_[2];
     Profiler prof_;
 };

@@ -70,6 +70,39 @@ __global__ void __launch_bounds__(256) k_downscale(const float* src, int W, int 
     out[pos] = scaled ? (scale * 0.25f) * sum : 0.25f * sum;
 }
 
+// Warm start (no counterpart in the reference, which starts every solve from zero): the rule
+// over the w x h level-0 pixels is that every component is finite and at most `bound` in
+// magnitude; a violation sets *bad, which k_set_int clears first.
+__global__ void __launch_bounds__(256) k_ws_check(const float* u, const float* v, int w, int h, int s, float bound,
+                                                   int* bad) {
+    RF_PIX
+    if (!(fabsf(u[pos]) <= bound) || !(fabsf(v[pos]) <= bound)) atomicOr(bad, 1);
+}
+
+__global__ void k_set_int(int* p, int value) {
+    if (threadIdx.x == 0) *p = value;
+}
+
+// The warm start at the coarsest level (w, h, s) from level 0 (W, H, S), by the prior flow's
+// DownscaleScalingKernel (downscale.cu:28-48, as k_downscale with scaling): u, v keep their zeros
+// when the rule failed.
+__global__ void __launch_bounds__(256) k_ws_apply(const float* wu, const float* wv, int W, int H, int S, int w, int h,
+                                                   int s, float scale, const int* bad, float* u, float* v) {
+    RF_PIX
+    if (*bad) return;
+    const int dxn = 4 * w, dyn = 4 * h;
+    const int xc = (4 * ix + 2) * W - 2 * w, yc = (4 * iy + 2) * H - 2 * h;
+    auto down = [&](const float* src) {
+        const float a = sample_ratio(src, W, H, S, xc - W, dxn, yc, dyn);
+        const float b = sample_ratio(src, W, H, S, xc + W, dxn, yc, dyn);
+        const float c = sample_ratio(src, W, H, S, xc, dxn, yc - H, dyn);
+        const float d = sample_ratio(src, W, H, S, xc, dxn, yc + H, dyn);
+        return (scale * 0.25f) * (((a + b) + c) + d);
+    };
+    u[pos] = down(wu);
+    v[pos] = down(wv);
+}
+
 // upscale.cu:14-27. (w, h, s) is the fine output; (cw, ch, cs) the coarse source.
 __global__ void __launch_bounds__(256) k_upscale(const float* src, int cw, int ch, int cs, int w, int h, int s,
                                                   float scale, float* out) {
@@ -541,6 +574,15 @@ UcParams make_uc(const double* R, const double* t, const double* K1) {
         hipLaunchKernelGGL(K, grid2(w, h), dim3(BX, BY), 0, st, __VA_ARGS__); \
         return hipGetLastError();                                           \
     } while (0)
+
+hipError_t launch_warm_start(const float* wu, const float* wv, int W, int H, int S, int w, int h, int s, int* bad,
+                             float* u, float* v, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, bad, 0);
+    if (W > 0 && H > 0)
+        hipLaunchKernelGGL(k_ws_check, grid2(W, H), dim3(BX, BY), 0, st, wu, wv, W, H, S, (float)std::max(W, H), bad);
+    const float scale = (float)w / (float)W;
+    RF_L2(k_ws_apply, wu, wv, W, H, S, w, h, s, scale, bad, u, v);
+}
 
 hipError_t launch_fill_zero(float* const* bufs, int nbuf, long n, hipStream_t st) {
     FillArgs a{};

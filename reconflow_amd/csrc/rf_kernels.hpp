@@ -12,6 +12,10 @@ namespace rf {
 // Zero up to 16 device buffers of n floats each in one launch (fill kernel instead of
 // hipMemsetAsync so the schedule can be captured in a graph).
 hipError_t launch_fill_zero(float* const* bufs, int nbuf, long n, hipStream_t st);
+// Warm start: clears *bad, checks the level-0 flow (W, H, S) against the rule (finite, at most
+// max(W, H)) and, when it holds, writes it downscaled to level (w, h, s) into u, v.
+hipError_t launch_warm_start(const float* wu, const float* wv, int W, int H, int S, int w, int h, int s, int* bad,
+                             float* u, float* v, hipStream_t st);
 
 hipError_t launch_gray_rgb(const uint8_t* rgb, float* out, int w, int h, int s, hipStream_t st);
 hipError_t launch_gray_u8(const uint8_t* in, float* out, int w, int h, int s, hipStream_t st);

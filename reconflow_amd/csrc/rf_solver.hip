@@ -39,16 +39,26 @@ int Solver::check(hipError_t e, const char* what) {
 }
 
 // Teardown: a failing destroy or free leaves nothing to recover, so its status is not checked.
-int Solver::release() {
-    batch_.reset();
-    for (auto& e : prof_.ev) (void)hipEventDestroy(e);
-    prof_.ev.clear();
-    prof_.armed = false;
+void Solver::drop_graphs() {
     for (auto& g : gexec_)
         if (g) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
+}
+
+int Solver::release() {
+    batch_.reset();
+    for (auto& e : prof_.ev) (void)hipEventDestroy(e);
+    prof_.ev.clear();
+    prof_.armed = false;
+    drop_graphs();
+    if (ws_) {
+        (void)hipFree(ws_);
+        ws_ = nullptr;
+        ws_bad_ = nullptr;
+    }
+    ws_set_ = false;
     if (slab_) {
         (void)hipFree(slab_);
         slab_ = nullptr;
@@ -333,6 +343,43 @@ int Solver::copy_prior(const float* u, const float* v, const float* mask, bool s
     return check(hipStreamSynchronize(st_), "copy prior");
 }
 
+// Warm start (no counterpart in the reference): the flow that replaces the zero initial u, v at
+// the coarsest level of the next solves (see launch_warm_start). The schedule gains three launches
+// when one is set, so a captured schedule is captured again.
+int Solver::copy_warm_start(const float* u, const float* v) {
+    if (!inited_) {
+        set_error("rf_copy_warm_start: not initialized");
+        return RF_ERR_STATE;
+    }
+    const bool set = u && v;
+    if (set != ws_set_) drop_graphs();
+    if (!set) {
+        ws_set_ = false;
+        return batch_ ? batch_->set_warm_start(0, nullptr, nullptr) : RF_OK;
+    }
+    const size_t bytes = (size_t)g_.n * 4;
+    if (!ws_) {
+        if (check(hipMalloc(&ws_, 2 * bytes + 256), "rf_copy_warm_start: allocation")) return RF_ERR_HIP;
+        ws_bad_ = reinterpret_cast<int*>(reinterpret_cast<char*>(ws_) + 2 * bytes);
+    }
+    if (check(hipMemcpyAsync(ws_, u, bytes, hipMemcpyHostToDevice, st_), "rf_copy_warm_start u")) return RF_ERR_HIP;
+    if (check(hipMemcpyAsync(ws_ + g_.n, v, bytes, hipMemcpyHostToDevice, st_), "rf_copy_warm_start v"))
+        return RF_ERR_HIP;
+    if (check(hipStreamSynchronize(st_), "rf_copy_warm_start")) return RF_ERR_HIP;
+    ws_set_ = true;
+    return batch_ ? batch_->set_warm_start(0, ws_, ws_ + g_.n) : RF_OK;
+}
+
+// 1 when a warm start is set and the last solve found it usable, else 0.
+int Solver::warm_start_in_use() {
+    if (!inited_ || !ws_set_) return 0;
+    if (batch_) return batch_->warm_start_in_use(0);
+    int bad = 1;
+    if (check(hipStreamSynchronize(st_), "rf_warm_start_in_use")) return RF_ERR_HIP;
+    if (check(hipMemcpy(&bad, ws_bad_, sizeof bad, hipMemcpyDeviceToHost), "rf_warm_start_in_use")) return RF_ERR_HIP;
+    return bad ? 0 : 1;
+}
+
 #define RF_TRY(x)                         \
     do {                                  \
         hipError_t e__ = (x);             \
@@ -404,6 +451,11 @@ hipError_t Solver::enqueue_flow_plain(Fields& f) {
     {
         float* z[] = {f.u, f.v};
         RF_TRY(launch_fill_zero(z, 2, g_.n, st_));
+    }
+    if (ws_set_) {
+        const int c = g_.L - 1;
+        RF_TRY(launch_warm_start(ws_, ws_ + g_.n, g_.W[0], g_.H[0], g_.S[0], g_.W[c], g_.H[c], g_.S[c], ws_bad_, f.u,
+                                 f.v, st_));
     }
     const int m = p_.method;
     const bool cg = (m == RF_METHOD_TVCHARBGRAD || m == RF_METHOD_TVCHARBGRAD_FN);
@@ -479,6 +531,11 @@ hipError_t Solver::enqueue_recon_plain(Fields& f) {
     {
         float* z[] = {f.u, f.v, f.uproj, f.vproj, f.uc, f.vc, f.X, f.Y, f.Z, f.sku, f.skv};
         RF_TRY(launch_fill_zero(z, 11, g_.n, st_));
+    }
+    if (ws_set_) {
+        const int c = g_.L - 1;
+        RF_TRY(launch_warm_start(ws_, ws_ + g_.n, g_.W[0], g_.H[0], g_.S[0], g_.W[c], g_.H[c], g_.S[c], ws_bad_, f.u,
+                                 f.v, st_));
     }
     const bool fn = prior_ && (p_.method == RF_METHODR_TVL1_MS_FN || p_.method == RF_METHODR_TVL1_MS_FNSPARSE);
     const float thetaTv = 1.0f / p_.alpha_tv;  // Reconflow.cu:32
@@ -1045,6 +1102,16 @@ int rf_copy_initial_flow(rf_context* ctx, const float* u, const float* v) {
 int rf_copy_sparse_flow(rf_context* ctx, const float* u, const float* v, const float* mask) {
     RF_CTX(ctx);
     return ctx->s->copy_prior(u, v, mask, true);
+}
+
+int rf_copy_warm_start(rf_context* ctx, const float* u, const float* v) {
+    RF_CTX(ctx);
+    return ctx->s->copy_warm_start(u, v);
+}
+
+int rf_warm_start_in_use(rf_context* ctx) {
+    RF_CTX(ctx);
+    return ctx->s->warm_start_in_use();
 }
 
 int rf_solve_flow(rf_context* ctx, float flow_scale) {

@@ -97,6 +97,8 @@ class Solver {
     int copy_images(const void* i0, const void* i1);
     int copy_images_device(const float* i0, const float* i1);
     int copy_prior(const float* u, const float* v, const float* mask, bool sparse);
+    int copy_warm_start(const float* u, const float* v);  // host, level-0 layout; null clears
+    int warm_start_in_use();
     int copy_masks(const float* m0, const float* m1);
     int copy_depth(const uint16_t* d0, const uint16_t* d1);
     int solve_flow(float flow_scale);
@@ -149,6 +151,12 @@ class Solver {
     Profiler prof_;
     std::unique_ptr<Batch> batch_;  // the fused path
     double K0raw_[9], K1raw_[9];
+    // warm start: level-0 flow (device, allocated on first use) and the rule's flag of the last
+    // solve that applied it
+    float* ws_ = nullptr;
+    int* ws_bad_ = nullptr;
+    bool ws_set_ = false;
+    void drop_graphs();
 };
 
 }  // namespace rf
