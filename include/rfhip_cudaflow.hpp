@@ -104,6 +104,13 @@ class CudaFlow {
     int copySparseOpticalFlowToDevice(const Mat& u, const Mat& v, const Mat& mask) {
         return rf_copy_sparse_flow(ctx_, u.ptr<float>(), v.ptr<float>(), mask.ptr<float>());
     }
+    // Warm-start input (not in the reference): the flow that replaces the zero initial flow at
+    // the coarsest level of the following solves; clearWarmStart goes back to zero.
+    int copyWarmStartToDevice(const Mat& u, const Mat& v) {
+        return rf_copy_warm_start(ctx_, u.ptr<float>(), v.ptr<float>());
+    }
+    int clearWarmStart() { return rf_copy_warm_start(ctx_, nullptr, nullptr); }
+    bool warmStartInUse() { return rf_warm_start_in_use(ctx_) == 1; }
 
     int solveOpticalFlow() { return solveOpticalFlow(5.0f); }
     int solveOpticalFlow(float flowScale) {
