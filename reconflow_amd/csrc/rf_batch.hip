@@ -238,6 +238,34 @@ __global__ void __launch_bounds__(256) kb_median5(Launch L, int l, int pu) {
     F.v[pu ^ 1][pos] = median25(mv);
 }
 
+// The last kernel of a warp followed by the next warp's WarpingKernel at the same pixel: the warp
+// reads the flow at its own pixel only, so it needs no other pixel's result and one launch per
+// warp transition is saved. kb_med5_add + kb_warp<TI> (TV-L1):
+template <typename TI>
+__global__ void __launch_bounds__(256) kb_med5_add_warp(Launch L, int l, int pd, int pu) {
+    BT_PIX(L, l)
+    const float *du = F.du[pd], *dv = F.dv[pd];
+    float mu[25], mv[25];
+    gather5x5(du, dv, ix, iy, w, h, s, mu, mv);
+    const float un = F.u[pu][pos] + median25(mu), vn = F.v[pu][pos] + median25(mv);
+    F.u[pu][pos] = un;
+    F.v[pu][pos] = vn;
+    as<TI>(F.i1warp)[pos] = st<TI>(warp_sample(as<TI>(F.I1[l]), w, h, s, ix, iy, un, vn));
+}
+
+// kb_median5 + kb_warp<TI> (joint): the median of the flow into the partner buffers, warped with.
+template <typename TI>
+__global__ void __launch_bounds__(256) kb_median5_warp(Launch L, int l, int pu) {
+    BT_PIX(L, l)
+    const float *u = F.u[pu], *v = F.v[pu];
+    float mu[25], mv[25];
+    gather5x5(u, v, ix, iy, w, h, s, mu, mv);
+    const float un = median25(mu), vn = median25(mv);
+    F.u[pu ^ 1][pos] = un;
+    F.v[pu ^ 1][pos] = vn;
+    as<TI>(F.i1warp)[pos] = st<TI>(warp_sample(as<TI>(F.I1[l]), w, h, s, ix, iy, un, vn));
+}
+
 // Upscale from coarse level l+1 to level l (upscale.cu:14-27): the flow, and in joint mode
 // uc, uproj and Sk (Reconflow.cu:424-458), into the partner buffers.
 __global__ void __launch_bounds__(256) kb_upscale(Launch L, int l, int pu, int pf, int recon, float scale) {
@@ -1681,8 +1709,9 @@ hipError_t Batch::enqueue_t(bool recon) {
             if (fn) RF_LAUNCH(kb_prior, tiles(level, TC_PIX), level);
             RF_LAUNCH(kb_sync_x, tiles(FLAT, TC_PIX), pr.x);
         }
+        bool warped = false;  // this warp's image was warped by the previous warp's last kernel
         for (int wi = 0; wi < p_.n_warp_iters; ++wi) {
-            RF_LAUNCH(kb_warp<TI>, tiles(level, TC_PIX), level, pr.u);
+            if (!warped) RF_LAUNCH(kb_warp<TI>, tiles(level, TC_PIX), level, pr.u);
             RF_LAUNCH((kb_deriv<TI, TS>), tiles(level, TC_PIX), level);
             if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, TC_PIX), level, pr.d);
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
@@ -1722,10 +1751,16 @@ hipError_t Batch::enqueue_t(bool recon) {
             }
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi) + 1], st_));
             prof_.end(st_, RF_KID_FUSED_ITER, level, p_.n_solver_iters);
-            RF_LAUNCH(kb_med5_add, tiles(level, TC_PIX), level, pr.d, pr.u);
+            warped = wi + 1 < p_.n_warp_iters;
             if (recon) {
-                RF_LAUNCH(kb_median5, tiles(level, TC_PIX), level, pr.u);
+                RF_LAUNCH(kb_med5_add, tiles(level, TC_PIX), level, pr.d, pr.u);
+                if (warped) RF_LAUNCH(kb_median5_warp<TI>, tiles(level, TC_PIX), level, pr.u);
+                else RF_LAUNCH(kb_median5, tiles(level, TC_PIX), level, pr.u);
                 pr.u ^= 1;
+            } else if (warped) {
+                RF_LAUNCH(kb_med5_add_warp<TI>, tiles(level, TC_PIX), level, pr.d, pr.u);
+            } else {
+                RF_LAUNCH(kb_med5_add, tiles(level, TC_PIX), level, pr.d, pr.u);
             }
         }
         if (level > 0) {
