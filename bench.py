@@ -53,8 +53,9 @@ FLOW_ITER_BYTES = 60.0  # one fused TV-L1 iteration: I(3) read; uhat(2), p(4) re
 KID = {"fused_iter": 0, "data": 1, "dual": 2, "uvproj": 3, "solve3d": 4, "median3d": 5, "cleanup": 6,
        "solve_uc": 7, "update_sk": 8}
 TILING = {0: "auto", 1: "64x8-rpt2", 2: "32x16", 3: "16x16", 4: "16x16-occ5", 5: "16x16-k2", 6: "16x16-k3"}
-ITERS_PER_LAUNCH = {"16x16-k2": 2, "16x16-k3": 3, "32x16-k4": 4, "32x32-k8": 8}
-FLOW_TILING = {0: "auto", 1: "64x16-rpt4", 2: "16x16", 3: "32x16", 4: "32x16-k4", 5: "32x32-k8"}
+ITERS_PER_LAUNCH = {"16x16-k2": 2, "16x16-k3": 3, "32x16-k4": 4, "32x32-k8": 8, "32x16-k2": 2, "32x16-k6": 6}
+FLOW_TILING = {0: "auto", 1: "64x16-rpt4", 2: "16x16", 3: "32x16", 4: "32x16-k4", 5: "32x32-k8", 6: "32x16-k2",
+               7: "32x16-k6"}
 
 
 def make_solver(rf, path, graph, tri, joint_tile=0):

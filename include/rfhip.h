@@ -125,6 +125,8 @@ typedef struct rf_params {
 #define RF_FLOW_TILE_32x16 3      /* one pixel per lane, 31x15 pixels per workgroup */
 #define RF_FLOW_TILE_32x16_K4 4   /* four iterations per launch, 25x9 pixels per 32x16 workgroup */
 #define RF_FLOW_TILE_32x32_K8 5   /* eight iterations per launch, 17x17 pixels per 32x32 workgroup */
+#define RF_FLOW_TILE_32x16_K2 6   /* two iterations per launch, 29x13 pixels per 32x16 workgroup */
+#define RF_FLOW_TILE_32x16_K6 7   /* six iterations per launch, 21x5 pixels per 32x16 workgroup */
 
 /* Fills p with the reference defaults (Cudaflow.cu:99-101, Reconflow.cu:27-30) for
  * width x height: 1 warp, 100 iterations, lambda 100, theta 0.33, tau 0.25, alignment 32,

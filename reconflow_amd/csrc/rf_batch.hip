@@ -1182,7 +1182,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         return RF_ERR_ARG;
     }
     if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_K3 ||
-        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x32_K8) {
+        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x16_K6) {
         set_error("rf_batch_create: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
     }
@@ -1347,10 +1347,17 @@ TileSet& Batch::tiles(int level, int cls) { return sets_[level * TC_COUNT + cls]
 // Iteration tiling of a level for a warp (TV-L1 or joint): a fixed choice; during the tuning
 // solves the candidates in turn over the warps of as many solves as it takes to time each one;
 // afterwards the tuned choice (the first candidate until then).
-constexpr int kNumCand = 5;
-static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_FK4, TC_F32, TC_FK8, TC_FLOW},
-                                           {TC_J16, TC_J32, TC_JK2, TC_J16O, TC_JK3}};
-static const int kCandCount[2] = {5, 5};
+constexpr int kNumCand = 6;
+static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_FK4, TC_F32, TC_FK6, TC_FK8, TC_FLOW},
+                                           {TC_J16, TC_J32, TC_JK2, TC_J16O, TC_JK3, TC_JK3}};
+static const int kCandCount[2] = {6, 5};
+
+// The class of the last `rem` iterations of a warp when the level's class runs more per launch:
+// the most iterations per launch that do not pass rem.
+static int remainder_class(bool recon, int rem) {
+    if (recon) return rem >= 2 ? TC_JK2 : TC_J16;
+    return rem >= 4 ? TC_FK4 : (rem >= 2 ? TC_FK2 : TC_F16);
+}
 
 int Batch::tile_class(bool recon, int level, int warp) const {
     if (recon) {
@@ -1370,6 +1377,8 @@ int Batch::tile_class(bool recon, int level, int warp) const {
             case RF_FLOW_TILE_32x16: return TC_F32;
             case RF_FLOW_TILE_32x16_K4: return TC_FK4;
             case RF_FLOW_TILE_32x32_K8: return TC_FK8;
+            case RF_FLOW_TILE_32x16_K2: return TC_FK2;
+            case RF_FLOW_TILE_32x16_K6: return TC_FK6;
             default: break;
         }
     }
@@ -1398,6 +1407,8 @@ int Batch::flow_tiling(int level) const {
         case TC_F32: return RF_FLOW_TILE_32x16;
         case TC_FK4: return RF_FLOW_TILE_32x16_K4;
         case TC_FK8: return RF_FLOW_TILE_32x32_K8;
+        case TC_FK2: return RF_FLOW_TILE_32x16_K2;
+        case TC_FK6: return RF_FLOW_TILE_32x16_K6;
         default: return RF_FLOW_TILE_16x16;
     }
 }
@@ -1640,6 +1651,14 @@ static void launch_flow(int cls, bool single, hipStream_t st, const IterArgs& a)
             if (single) hipLaunchKernelGGL((kb_flow_multi<32, 32, 8, true, TS>), grid, dim3(32, 32), 0, st, a);
             else hipLaunchKernelGGL((kb_flow_multi<32, 32, 8, false, TS>), grid, dim3(32, 32), 0, st, a);
             break;
+        case TC_FK2:
+            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 16, 2, true, TS>), grid, dim3(32, 16), 0, st, a);
+            else hipLaunchKernelGGL((kb_flow_multi<32, 16, 2, false, TS>), grid, dim3(32, 16), 0, st, a);
+            break;
+        case TC_FK6:
+            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 16, 6, true, TS>), grid, dim3(32, 16), 0, st, a);
+            else hipLaunchKernelGGL((kb_flow_multi<32, 16, 6, false, TS>), grid, dim3(32, 16), 0, st, a);
+            break;
         default: hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), grid, dim3(TXW, TRW), 0, st, a); break;
     }
 }
@@ -1718,7 +1737,7 @@ hipError_t Batch::enqueue_t(bool recon) {
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi)], st_));
             for (int it = 0; it < p_.n_solver_iters;) {
                 int cls = tile_class(recon, level, wi);
-                if (kIters[cls] > p_.n_solver_iters - it) cls = recon ? TC_J16 : TC_F16;  // the remainder, one per launch
+                if (kIters[cls] > p_.n_solver_iters - it) cls = remainder_class(recon, p_.n_solver_iters - it);
                 const TileSet& ts = tiles(level, cls);
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,

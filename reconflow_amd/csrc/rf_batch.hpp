@@ -38,7 +38,8 @@ struct PairFields {
 // tile; TC_J16O: the 14x14 tiling compiled for occupancy rather than latency; TC_JK2, TC_JK3: 10x10
 // and 6x6 pixels of a 16x16 extended tile, two and three iterations per launch); TV-L1 iteration
 // 64x16 (four rows per lane), 15x15 and 31x15 (one pixel per lane of a 16x16 / 32x16 extended
-// tile), 25x9 and 17x17 (four / eight iterations per launch on a 32x16 / 32x32 extended tile).
+// tile), 25x9 and 17x17 (four / eight iterations per launch on a 32x16 / 32x32 extended tile),
+// 29x13 and 21x5 (two / six iterations per launch on a 32x16 extended tile).
 enum TileClass {
     TC_PIX = 0,
     TC_J64 = 1,
@@ -52,12 +53,14 @@ enum TileClass {
     TC_JK3 = 9,
     TC_FK4 = 10,
     TC_FK8 = 11,
-    TC_COUNT = 12
+    TC_FK2 = 12,
+    TC_FK6 = 13,
+    TC_COUNT = 14
 };
-constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14, 15, 31, 10, 6, 25, 17};
-constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14, 15, 15, 10, 6, 9, 17};
+constexpr int kTileW[TC_COUNT] = {64, 64, 64, 30, 14, 14, 15, 31, 10, 6, 25, 17, 29, 21};
+constexpr int kTileH[TC_COUNT] = {4, 8, 16, 14, 14, 14, 15, 15, 10, 6, 9, 17, 13, 5};
 // solver iterations one launch of the class runs
-constexpr int kIters[TC_COUNT] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 3, 4, 8};
+constexpr int kIters[TC_COUNT] = {1, 1, 1, 1, 1, 1, 1, 1, 2, 3, 4, 8, 2, 6};
 
 // One launch's tile list: (pair, tile index within the pair).
 struct TileSet {

@@ -91,7 +91,7 @@ int Solver::initialize(const rf_params& p) {
         return RF_ERR_ARG;
     }
     if (p.joint_tile < RF_JOINT_TILE_AUTO || p.joint_tile > RF_JOINT_TILE_16x16_K3 ||
-        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x32_K8) {
+        p.flow_tile < RF_FLOW_TILE_AUTO || p.flow_tile > RF_FLOW_TILE_32x16_K6) {
         set_error("rf_initialize: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
     }

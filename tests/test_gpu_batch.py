@@ -19,7 +19,7 @@ def _fetch(ptr, n):
     return fetch(ptr, n)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 3, 4], ids=["auto", "64x16-rpt4", "32x16", "32x16-k4"])
+@pytest.mark.parametrize("tile", [0, 1, 3, 4, 7], ids=["auto", "64x16-rpt4", "32x16", "32x16-k4", "32x16-k6"])
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
 def test_batch_flow(orc, graph, tile):
     print(f"case: batch TV-L1 {len(SIZES)} pairs graph={graph} tile={tile}", flush=True)

@@ -72,12 +72,13 @@ def test_flow_320x240_paths_equal_plain(tvl1_320, path, graph):
 @pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16], ids=["f32", "f16"])
 def test_flow_tilings_equal(orc, storage, ni):
     # every tiling of the fused TV-L1 iteration gives the same bits (uneven level sizes:
-    # 200x152 -> 100x76 -> 50x38, partial tiles on every level; 11 iterations leave a remainder
-    # after the four- and eight-iteration launches); FP32 storage equals the oracle
+    # 200x152 -> 100x76 -> 50x38, partial tiles on every level; 8 and 11 iterations leave
+    # remainders after the four-, six- and eight-iteration launches, which run as four-, two- and
+    # one-iteration launches); FP32 storage equals the oracle
     print(f"case: TV-L1 200x152 fused graph tilings storage={storage} iterations={ni}", flush=True)
     i0, i1, _, _ = rf.make_flow_pair(200, 152, 5)
     res = {}
-    for tile in (1, 2, 3, 4, 5, 0):
+    for tile in (1, 2, 3, 4, 5, 6, 7, 0):
         f = _flow(path=rf.PATH_FUSED, graph=True, w=200, h=152, L=3, nw=3, ni=ni, storage=storage, flow_tile=tile)
         for rep in range(2):
             f.copyImagesToDevice(i0, i1)
@@ -86,7 +87,7 @@ def test_flow_tilings_equal(orc, storage, ni):
         if tile == 0:
             tl = (C.c_int * 3)()
             assert f._lib.rf_flow_tiling(f._ctx, tl, 3) == 0
-            assert all(t in (1, 2, 3, 4, 5) for t in tl), list(tl)
+            assert all(t in (1, 2, 3, 4, 5, 6, 7) for t in tl), list(tl)
     base = res[(1, 0)]
     for k, (u, v) in res.items():
         assert np.array_equal(u[:, :200], base[0][:, :200]) and np.array_equal(v[:, :200], base[1][:, :200]), k
@@ -234,7 +235,7 @@ def test_tiny_images_every_tiling(orc):
         assert _same(_run_recon(r, d), ref, 40), tile
     i0, i1, _, _ = rf.make_flow_pair(40, 24, 18)
     ru, rv = orc.solve_flow(orc.flow_params(40, 24, 3, n_warp=2, n_iter=9), i0, i1, "f32")
-    for tile in range(1, 6):
+    for tile in range(1, 8):
         f = _flow(path=rf.PATH_FUSED, graph=False, w=40, h=24, L=3, nw=2, ni=9, flow_tile=tile)
         f.copyImagesToDevice(i0, i1)
         f.solveOpticalFlow(5.0)
