@@ -1355,7 +1355,7 @@ static const int kCandCount[2] = {6, 5};
 // The class of the last `rem` iterations of a warp when the level's class runs more per launch:
 // the most iterations per launch that do not pass rem.
 static int remainder_class(bool recon, int rem) {
-    if (recon) return rem >= 2 ? TC_JK2 : TC_J16;
+    if (recon) return rem >= 3 ? TC_JK3 : (rem >= 2 ? TC_JK2 : TC_J16);
     return rem >= 4 ? TC_FK4 : (rem >= 2 ? TC_FK2 : TC_F16);
 }
 
