@@ -1246,6 +1246,11 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         release();
         return RF_ERR_ALLOC;
     }
+    {
+        const std::vector<int> unused(n, 1);  // until a solve applies the rule
+        if (check(hipMemcpy(ws_bad_, unused.data(), sizeof(int) * n, hipMemcpyHostToDevice), "rf_batch_create"))
+            return RF_ERR_HIP;
+    }
     hp_.assign(n, PairFields{});
     float* cur = (float*)slab_;
     for (int i = 0; i < n; ++i) {

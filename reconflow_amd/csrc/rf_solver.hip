@@ -361,6 +361,9 @@ int Solver::copy_warm_start(const float* u, const float* v) {
     if (!ws_) {
         if (check(hipMalloc(&ws_, 2 * bytes + 256), "rf_copy_warm_start: allocation")) return RF_ERR_HIP;
         ws_bad_ = reinterpret_cast<int*>(reinterpret_cast<char*>(ws_) + 2 * bytes);
+        const int unused = 1;  // until a solve applies the rule
+        if (check(hipMemcpy(ws_bad_, &unused, sizeof unused, hipMemcpyHostToDevice), "rf_copy_warm_start"))
+            return RF_ERR_HIP;
     }
     if (check(hipMemcpyAsync(ws_, u, bytes, hipMemcpyHostToDevice, st_), "rf_copy_warm_start u")) return RF_ERR_HIP;
     if (check(hipMemcpyAsync(ws_ + g_.n, v, bytes, hipMemcpyHostToDevice, st_), "rf_copy_warm_start v"))

@@ -60,6 +60,7 @@ def test_flow_warm_start_equals_oracle(orc, path, graph):
     ref_cold = orc.solve_flow(p, i0, i1, "f32")
     f = _flow(path, graph)
     f.copyWarmStartToDevice(gu, gv)
+    assert not f.warmStartInUse()  # no solve has applied it yet
     for _ in range(2):  # a captured schedule replays the warm start
         assert _same(_flow_solve(f, i0, i1), ref_warm, W)
         assert f.warmStartInUse()
@@ -125,6 +126,7 @@ def test_batch_warm_start_device(orc, sizes):
         data.append(d)
     # pairs 0 and 2 warm-started from their ground truth, pair 1 cold; then pair 2's warm start
     # is made unusable
+    assert not any(b.warm_start_in_use(i) for i in range(n))
     wsd = {}
     for i in (0, 2):
         wsd[i] = (dev(data[i]["gt_u"].ravel()), dev(data[i]["gt_v"].ravel()))
