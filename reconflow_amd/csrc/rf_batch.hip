@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <string>
 
 #include "rf_batch.hpp"
@@ -200,22 +201,34 @@ __global__ void __launch_bounds__(256) kb_warp(Launch L, int l, int pu) {
     as<TS>(F.i1warp)[pos] = st<TS>(warp_sample(as<TS>(F.I1[l]), w, h, s, ix, iy, F.u[pu][pos], F.v[pu][pos]));
 }
 
+// ComputeDerivativesKernel at pixel (ix, iy) (computeDerivatives.cu:8-57): x and y derivatives as
+// the mean of the level image's and the warped image's 5-point derivatives over mirrored taps,
+// and the time derivative; element type TI, stride s.
+template <typename TI>
+__device__ __forceinline__ void deriv_px(const TI* I0, const TI* I1, int ix, int iy, int w, int h, int s, float& dx,
+                                         float& dy, float& dt) {
+    const long r = (long)iy * s;
+    const int xm2 = refl(ix - 2, w), xm1 = refl(ix - 1, w), xp1 = refl(ix + 1, w), xp2 = refl(ix + 2, w);
+    const int ym2 = refl(iy - 2, h), ym1 = refl(iy - 1, h), yp1 = refl(iy + 1, h), yp2 = refl(iy + 2, h);
+    auto g = [](const TI* p, long i) { return (float)p[i]; };
+    float t0 = d5(g(I0, r + xm2), g(I0, r + xm1), g(I0, r + xp1), g(I0, r + xp2));
+    float t1 = d5(g(I1, r + xm2), g(I1, r + xm1), g(I1, r + xp1), g(I1, r + xp2));
+    dx = (t0 + t1) * 0.5f;
+    dt = g(I1, r + ix) - g(I0, r + ix);
+    t0 = d5(g(I0, ix + (long)ym2 * s), g(I0, ix + (long)ym1 * s), g(I0, ix + (long)yp1 * s), g(I0, ix + (long)yp2 * s));
+    t1 = d5(g(I1, ix + (long)ym2 * s), g(I1, ix + (long)ym1 * s), g(I1, ix + (long)yp1 * s), g(I1, ix + (long)yp2 * s));
+    dy = (t0 + t1) * 0.5f;
+}
+
 // ComputeDerivativesKernel (computeDerivatives.cu:8-57).
 template <typename TI, typename TS>
 __global__ void __launch_bounds__(256) kb_deriv(Launch L, int l) {
     BT_PIX(L, l)
-    const TI *I0 = as<TI>(F.I0[l]), *I1 = as<TI>(F.i1warp);
-    const long r = (long)iy * s;
-    int xm2 = refl(ix - 2, w), xm1 = refl(ix - 1, w), xp1 = refl(ix + 1, w), xp2 = refl(ix + 2, w);
-    int ym2 = refl(iy - 2, h), ym1 = refl(iy - 1, h), yp1 = refl(iy + 1, h), yp2 = refl(iy + 2, h);
-    auto g = [](const TI* p, long i) { return (float)p[i]; };
-    float t0 = d5(g(I0, r + xm2), g(I0, r + xm1), g(I0, r + xp1), g(I0, r + xp2));
-    float t1 = d5(g(I1, r + xm2), g(I1, r + xm1), g(I1, r + xp1), g(I1, r + xp2));
-    as<TS>(F.Ix)[pos] = st<TS>((t0 + t1) * 0.5f);
-    as<TS>(F.Iz)[pos] = st<TS>(g(I1, pos) - g(I0, pos));
-    t0 = d5(g(I0, ix + (long)ym2 * s), g(I0, ix + (long)ym1 * s), g(I0, ix + (long)yp1 * s), g(I0, ix + (long)yp2 * s));
-    t1 = d5(g(I1, ix + (long)ym2 * s), g(I1, ix + (long)ym1 * s), g(I1, ix + (long)yp1 * s), g(I1, ix + (long)yp2 * s));
-    as<TS>(F.Iy)[pos] = st<TS>((t0 + t1) * 0.5f);
+    float dx, dy, dt;
+    deriv_px(as<TI>(F.I0[l]), as<TI>(F.i1warp), ix, iy, w, h, s, dx, dy, dt);
+    as<TS>(F.Ix)[pos] = st<TS>(dx);
+    as<TS>(F.Iz)[pos] = st<TS>(dt);
+    as<TS>(F.Iy)[pos] = st<TS>(dy);
 }
 
 // Median of uhat added to the flow (medianFilter.cu:3-75, addKernel.cu:3-11), in place.
@@ -297,6 +310,7 @@ __global__ void __launch_bounds__(256) kb_hsv(Launch L, int pu, float flowscale)
 struct JointView {
     const float *du0, *dv0, *p1u0, *p2u0, *p1v0, *p2v0, *U, *V, *sku0, *skv0, *up0, *vp0, *uc0, *vc0, *X0, *Y0, *Z0;
     const float *Ix, *Iy, *Iz, *ufn, *vfn, *mask;
+    const float *I0, *I1w;  // the level image and the warped image (derivatives computed in place)
     float *du1, *dv1, *p1u1, *p2u1, *p1v1, *p2v1, *sku1, *skv1, *up1, *vp1, *uc1, *vc1, *X1, *Y1, *Z1;
     int w, h, s, tiles_x;
     float inv_tiles_x;  // RN(1/tiles_x): tile row of a tile index without an integer division
@@ -318,6 +332,7 @@ __host__ __device__ inline JointView joint_view(const PairFields& F, const PairF
     v.sku0 = F.sku[pf], v.skv0 = F.skv[pf], v.up0 = F.up[pf], v.vp0 = F.vp[pf], v.uc0 = F.uc[pf], v.vc0 = F.vc[pf];
     v.X0 = F.X[px], v.Y0 = F.Y[px], v.Z0 = F.Z[px];
     v.Ix = F.Ix, v.Iy = F.Iy, v.Iz = F.Iz, v.ufn = F.ufn_l, v.vfn = F.vfn_l, v.mask = F.mask_l;
+    v.I0 = F.I0[l], v.I1w = F.i1warp;
     v.du1 = F.du[pd ^ 1], v.dv1 = F.dv[pd ^ 1], v.p1u1 = F.p1u[pd ^ 1], v.p2u1 = F.p2u[pd ^ 1],
     v.p1v1 = F.p1v[pd ^ 1], v.p2v1 = F.p2v[pd ^ 1];
     v.sku1 = F.sku[pf ^ 1], v.skv1 = F.skv[pf ^ 1], v.up1 = F.up[pf ^ 1], v.vp1 = F.vp[pf ^ 1], v.uc1 = F.uc[pf ^ 1],
@@ -1049,8 +1064,12 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_px(IterArgs a) {
 // below). State between the iterations stays in registers and LDS; the duals are rounded to the
 // storage type between iterations as a store and reload would round them. The launch stores state
 // n + K into the other parity, one flip per launch. Same per-pixel arithmetic as kb_flow_px.
-template <int EXW, int EXH, int K, bool SINGLE, typename TS>
+// TI (not void): the first launch of a warp, which computes the derivatives at its lanes' pixels
+// from the level image and the warped image (element type TI) as kb_deriv does, uses them rounded
+// to TS, and stores them for the warp's later launches.
+template <int EXW, int EXH, int K, bool SINGLE, typename TS, typename TI = void>
 __global__ void __launch_bounds__(EXW* EXH) kb_flow_multi(IterArgs a) {
+    constexpr bool DERIV = !std::is_void<TI>::value;
     constexpr int SM = SINGLE ? 2 : 0;
     constexpr int TW = EXW - 2 * K + 1, TH = EXH - 2 * K + 1, NE = EXW * EXH;
     static_assert(TW > 0 && TH > 0, "tile too small for K iterations");
@@ -1075,7 +1094,15 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_multi(IterArgs a) {
     float p1u = (float)el(p1u0, qt), p2u = (float)el(p2u0, qt), p1v = (float)el(p1v0, qt), p2v = (float)el(p2v0, qt);
     float p1ul = (float)el(p1u0, qlt), p1vl = (float)el(p1v0, qlt), p2ud = (float)el(p2u0, qut),
           p2vd = (float)el(p2v0, qut);
-    const float ix_ = (float)el(as<TS>(V.Ix), qt), iy_ = (float)el(as<TS>(V.Iy), qt), iz_ = (float)el(as<TS>(V.Iz), qt);
+    float ix_, iy_, iz_;
+    if constexpr (DERIV) {
+        const unsigned qi = qb * (unsigned)(sizeof(float) / sizeof(TI));  // the pair's images
+        float dx, dy, dtt;
+        deriv_px(as<TI>(V.I0) + qi, as<TI>(V.I1w) + qi, xc, yc, w, h, s, dx, dy, dtt);
+        ix_ = (float)st<TS>(dx), iy_ = (float)st<TS>(dy), iz_ = (float)st<TS>(dtt);
+    } else {
+        ix_ = (float)el(as<TS>(V.Ix), qt), iy_ = (float)el(as<TS>(V.Iy), qt), iz_ = (float)el(as<TS>(V.Iz), qt);
+    }
     const float fac = a.tau / a.theta;
     const int e_l = max(e - 1, 0), e_u = max(e - EXW, 0), e_r = min(e + 1, NE - 1), e_d = min(e + EXW, NE - 1);
 #pragma unroll 1
@@ -1107,6 +1134,11 @@ __global__ void __launch_bounds__(EXW* EXH) kb_flow_multi(IterArgs a) {
                 el(as<TS>(V.p2v1), qt) = st<TS>(o2v);
                 el(V.du1, q) = nduh;
                 el(V.dv1, q) = ndvh;
+                if constexpr (DERIV) {
+                    el(as<TS>(const_cast<float*>(V.Ix)), qt) = st<TS>(ix_);
+                    el(as<TS>(const_cast<float*>(V.Iy)), qt) = st<TS>(iy_);
+                    el(as<TS>(const_cast<float*>(V.Iz)), qt) = st<TS>(iz_);
+                }
             }
             return;
         }
@@ -1636,9 +1668,22 @@ static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
     }
 }
 
-template <typename TS>
-static void launch_flow(int cls, bool single, hipStream_t st, const IterArgs& a) {
+// deriv: the launch also computes the warp's derivatives (the multi-iteration classes; the caller
+// then skips kb_deriv)
+template <typename TI, typename TS>
+static void launch_flow(int cls, bool single, bool deriv, hipStream_t st, const IterArgs& a) {
     const dim3 grid(a.L.ntiles);
+#define RF_FM(EXW, EXH, K)                                                                                           \
+    do {                                                                                                             \
+        const dim3 blk(EXW, EXH);                                                                                    \
+        if (deriv) {                                                                                                 \
+            if (single) hipLaunchKernelGGL((kb_flow_multi<EXW, EXH, K, true, TS, TI>), grid, blk, 0, st, a);         \
+            else hipLaunchKernelGGL((kb_flow_multi<EXW, EXH, K, false, TS, TI>), grid, blk, 0, st, a);               \
+        } else {                                                                                                     \
+            if (single) hipLaunchKernelGGL((kb_flow_multi<EXW, EXH, K, true, TS>), grid, blk, 0, st, a);             \
+            else hipLaunchKernelGGL((kb_flow_multi<EXW, EXH, K, false, TS>), grid, blk, 0, st, a);                   \
+        }                                                                                                            \
+    } while (0)
     switch (cls) {
         case TC_F16:
             if (single) hipLaunchKernelGGL((kb_flow_px<16, 16, true, TS>), grid, dim3(16, 16), 0, st, a);
@@ -1648,25 +1693,17 @@ static void launch_flow(int cls, bool single, hipStream_t st, const IterArgs& a)
             if (single) hipLaunchKernelGGL((kb_flow_px<32, 16, true, TS>), grid, dim3(32, 16), 0, st, a);
             else hipLaunchKernelGGL((kb_flow_px<32, 16, false, TS>), grid, dim3(32, 16), 0, st, a);
             break;
-        case TC_FK4:
-            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 16, 4, true, TS>), grid, dim3(32, 16), 0, st, a);
-            else hipLaunchKernelGGL((kb_flow_multi<32, 16, 4, false, TS>), grid, dim3(32, 16), 0, st, a);
-            break;
-        case TC_FK8:
-            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 32, 8, true, TS>), grid, dim3(32, 32), 0, st, a);
-            else hipLaunchKernelGGL((kb_flow_multi<32, 32, 8, false, TS>), grid, dim3(32, 32), 0, st, a);
-            break;
-        case TC_FK2:
-            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 16, 2, true, TS>), grid, dim3(32, 16), 0, st, a);
-            else hipLaunchKernelGGL((kb_flow_multi<32, 16, 2, false, TS>), grid, dim3(32, 16), 0, st, a);
-            break;
-        case TC_FK6:
-            if (single) hipLaunchKernelGGL((kb_flow_multi<32, 16, 6, true, TS>), grid, dim3(32, 16), 0, st, a);
-            else hipLaunchKernelGGL((kb_flow_multi<32, 16, 6, false, TS>), grid, dim3(32, 16), 0, st, a);
-            break;
+        case TC_FK4: RF_FM(32, 16, 4); break;
+        case TC_FK8: RF_FM(32, 32, 8); break;
+        case TC_FK2: RF_FM(32, 16, 2); break;
+        case TC_FK6: RF_FM(32, 16, 6); break;
         default: hipLaunchKernelGGL((kb_flow_iter<RPT_FLOW, TS>), grid, dim3(TXW, TRW), 0, st, a); break;
     }
+#undef RF_FM
 }
+
+// The iteration classes whose kernel can compute the warp's derivatives in its first launch.
+static bool fuses_deriv(int cls) { return cls == TC_FK2 || cls == TC_FK4 || cls == TC_FK6 || cls == TC_FK8; }
 
 // sm: how the kernels find the fields (see tile_view)
 template <typename TS>
@@ -1736,7 +1773,11 @@ hipError_t Batch::enqueue_t(bool recon) {
         bool warped = false;  // this warp's image was warped by the previous warp's last kernel
         for (int wi = 0; wi < p_.n_warp_iters; ++wi) {
             if (!warped) RF_LAUNCH(kb_warp<TI>, tiles(level, TC_PIX), level, pr.u);
-            RF_LAUNCH((kb_deriv<TI, TS>), tiles(level, TC_PIX), level);
+            // TV-L1: a multi-iteration first launch computes the derivatives itself
+            int cls0 = tile_class(recon, level, wi);
+            if (kIters[cls0] > p_.n_solver_iters) cls0 = remainder_class(recon, p_.n_solver_iters);
+            const bool fused_deriv = !recon && p_.n_solver_iters > 0 && fuses_deriv(cls0);
+            if (!fused_deriv) RF_LAUNCH((kb_deriv<TI, TS>), tiles(level, TC_PIX), level);
             if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, TC_PIX), level, pr.d);
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
             if (tuning_) RF_TRY(hipEventRecord(tev_[2 * (level * p_.n_warp_iters + wi)], st_));
@@ -1761,7 +1802,7 @@ hipError_t Batch::enqueue_t(bool recon) {
                 }
                 if (ts.ntiles > 0) {
                     if (!recon)
-                        launch_flow<TS>(cls, single, st_, a);
+                        launch_flow<TI, TS>(cls, single, fused_deriv && it == 0, st_, a);
                     else
                         launch_joint<TS>(cls, single ? (n() == 1 ? 1 : 2) : 0, p_.tri_f64 != 0, fn, st_, a);
                     RF_TRY(hipGetLastError());
