@@ -617,7 +617,9 @@ struct JointIn {
 };
 
 // INNER: the pixel has neighbours on every side (not on column 0 or row 0), known at compile time.
-template <bool PRIOR, typename TS, bool INNER = false>
+// TI (not void): the derivatives are computed at the pixel from the level image and the warped
+// image (element type TI) as kb_deriv does and rounded to TS, instead of loaded.
+template <bool PRIOR, typename TS, bool INNER = false, typename TI = void>
 __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView& V, int xc, int yc, unsigned qb) {
     const int s = V.s;
     const unsigned q = qb + (unsigned)xc + (unsigned)yc * (unsigned)s;
@@ -632,8 +634,15 @@ __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView
     in.uu = el(V.U, q), in.vv = el(V.V, q);
     in.sku = el(V.sku0, q), in.skv = el(V.skv0, q), in.up0 = el(V.up0, q), in.vp0 = el(V.vp0, q);
     in.uc0 = el(V.uc0, q), in.vc0 = el(V.vc0, q);
-    const unsigned qt = q + qb * (unsigned)(sizeof(float) / sizeof(TS) - 1);  // the same element in TS fields
-    in.ix = (float)el(as<TS>(V.Ix), qt), in.iy = (float)el(as<TS>(V.Iy), qt), in.iz = (float)el(as<TS>(V.Iz), qt);
+    if constexpr (!std::is_void<TI>::value) {
+        const unsigned qi = qb * (unsigned)(sizeof(float) / sizeof(TI));  // the pair's images
+        float dx, dy, dt;
+        deriv_px(as<TI>(V.I0) + qi, as<TI>(V.I1w) + qi, xc, yc, V.w, V.h, s, dx, dy, dt);
+        in.ix = (float)st<TS>(dx), in.iy = (float)st<TS>(dy), in.iz = (float)st<TS>(dt);
+    } else {
+        const unsigned qt = q + qb * (unsigned)(sizeof(float) / sizeof(TS) - 1);  // the same element in TS fields
+        in.ix = (float)el(as<TS>(V.Ix), qt), in.iy = (float)el(as<TS>(V.Iy), qt), in.iz = (float)el(as<TS>(V.Iz), qt);
+    }
     const unsigned qr = q + 1u, qd = q + (unsigned)s;
     in.Xc = el(V.X0, q), in.Yc = el(V.Y0, q), in.Zc = el(V.Z0, q);
     in.Xr = el(V.X0, qr), in.Yr = el(V.Y0, qr), in.Zr = el(V.Z0, qr);
@@ -683,12 +692,6 @@ __device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const Joi
         solve3d_px<float>(const_view(cam(V.pq, po)), a.lf, a.lms, xc, yc, upn, vpn, in.Xc, in.Yc, in.Zc, in.Xr, in.Yr, in.Zr, in.Xd,
                           in.Yd, in.Zd, in.Xl, in.Yl, in.Zl, in.Xu, in.Yu, in.Zu, hl, hu, oX, oY, oZ);
     return JointFront{in.p1u, in.p2u, in.p1v, in.p2v, in.uu, in.vv, in.sku, in.skv, nduh, ndvh, upn, vpn, oX, oY, oZ};
-}
-
-template <bool F64TRI, bool PRIOR, typename TS, bool INNER = false>
-__device__ __forceinline__ JointFront joint_front(const IterArgs& a, const JointView& V, const PairOff& po, int xc,
-                                                  int yc) {
-    return joint_compute<F64TRI, PRIOR, INNER>(a, V, po, xc, yc, joint_load<PRIOR, TS, INNER>(a, V, xc, yc, po.qb));
 }
 
 // The back half of a joint iteration at the in-image pixel (x, y) whose extended-tile slot is e
@@ -766,6 +769,15 @@ __device__ __forceinline__ void joint_store(const JointView& V, unsigned q, cons
     el(V.skv1, q) = B.skv;
 }
 
+// Stores the derivatives that a launch computed at a pixel (element q of the pair at qb), in TS.
+template <typename TS>
+__device__ __forceinline__ void joint_store_deriv(const JointView& V, unsigned q, unsigned qb, const JointIn& in) {
+    const unsigned qt = q + qb * (unsigned)(sizeof(float) / sizeof(TS) - 1);
+    el(as<TS>(const_cast<float*>(V.Ix)), qt) = st<TS>(in.ix);
+    el(as<TS>(const_cast<float*>(V.Iy)), qt) = st<TS>(in.iy);
+    el(as<TS>(const_cast<float*>(V.Iz)), qt) = st<TS>(in.iz);
+}
+
 // One joint iteration with one pixel per lane: the workgroup is an EXW x EXH extended tile whose
 // outer ring of lanes only feeds its (EXW-2) x (EXH-2) interior. Every global load of a lane is issued up front
 // (coordinates clamped into the image so that the loads are in bounds; lanes off the image
@@ -782,7 +794,8 @@ __device__ __forceinline__ void joint_store(const JointView& V, unsigned q, cons
 // the FP64 triangulation, met without spilling, for occupancy on large levels).
 // The body of kb_joint_px for the tile at (x0, y0). INNER: every lane of the extended tile is a pixel
 // at least one pixel inside the image, so no coordinate is clamped and no border rule applies.
-template <int EXW, int EXH, bool INNER, bool F64TRI, bool PRIOR, typename TS>
+// TI (not void): the lanes compute the derivatives (joint_load) and interior lanes store them.
+template <int EXW, int EXH, bool INNER, bool F64TRI, bool PRIOR, typename TS, typename TI>
 __device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView& V, const PairOff& po, int x0,
                                               int y0, float* s_du, float* s_dv, float* s_X, float* s_Y, float* s_Z) {
     constexpr int NE = EXW * EXH;
@@ -790,7 +803,8 @@ __device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView
     const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
     const int x = x0 - 1 + ex, y = y0 - 1 + ey;
     const int xc = INNER ? x : min(max(x, 0), w - 1), yc = INNER ? y : min(max(y, 0), h - 1);
-    const JointFront J = joint_front<F64TRI, PRIOR, TS, INNER>(a, V, po, xc, yc);
+    const JointIn in = joint_load<PRIOR, TS, INNER, TI>(a, V, xc, yc, po.qb);
+    const JointFront J = joint_compute<F64TRI, PRIOR, INNER>(a, V, po, xc, yc, in);
     s_du[e] = J.nduh;
     s_dv[e] = J.ndvh;
     s_X[e] = J.oX;
@@ -805,9 +819,12 @@ __device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView
     const unsigned q = po.qb + (unsigned)x + (unsigned)y * (unsigned)s;
     joint_store(V, q, J,
                 joint_back_calc<EXW, NE, false, INNER>(a, ucp, J, x, y, w, h, e, s_du, s_dv, s_X, s_Y, s_Z));
+    if constexpr (!std::is_void<TI>::value) joint_store_deriv<TS>(V, q, po.qb, in);
 }
 
-template <int EXW, int EXH, int OCC, int SM, bool F64TRI, bool PRIOR, typename TS>
+// TI (not void): the first launch of a warp, which computes the derivatives at its lanes' pixels
+// and stores them for the warp's later launches (kb_deriv is not launched).
+template <int EXW, int EXH, int OCC, int SM, bool F64TRI, bool PRIOR, typename TS, typename TI = void>
 __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
     constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
     __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
@@ -820,9 +837,9 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     // lanes cover pixels x0 - 1 .. x0 + TW and y0 - 1 .. y0 + TH; the batched kernels keep one path
     // (with more scalars live, a second path makes the registers spill)
     if (SM == 1 && x0 >= 2 && y0 >= 2 && x0 + TW < V.w - 1 && y0 + TH < V.h - 1)
-        joint_px_tile<EXW, EXH, true, F64TRI, PRIOR, TS>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
+        joint_px_tile<EXW, EXH, true, F64TRI, PRIOR, TS, TI>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
     else
-        joint_px_tile<EXW, EXH, false, F64TRI, PRIOR, TS>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
+        joint_px_tile<EXW, EXH, false, F64TRI, PRIOR, TS, TI>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
 }
 
 // K joint iterations per launch (temporal blocking) on an EXW x EXH extended tile indexed by flat
@@ -836,8 +853,9 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
 // lanes at least 2k - 1 lanes inside the tile (the front half reads one lane around, the back
 // half one more), so the launch stores its (EXW - 2R) x (EXH - 2R) interior, pixels x0.. and
 // y0..: state n + K into the other parity, one flip per launch. Same per-pixel arithmetic as
-// kb_joint_px.
-template <int EXW, int EXH, int K, int SM, bool F64TRI, bool PRIOR, typename TS>
+// kb_joint_px. TI (not void): as for kb_joint_px, the warp's first launch computes and stores the
+// derivatives.
+template <int EXW, int EXH, int K, int SM, bool F64TRI, bool PRIOR, typename TS, typename TI = void>
 __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
     constexpr int R = 2 * K - 1, TW = EXW - 2 * R, TH = EXH - 2 * R, NE = EXW * EXH;
     static_assert(TW > 0 && TH > 0, "tile too small for K iterations");
@@ -863,7 +881,7 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
     const unsigned Ec = qb + (unsigned)min(max(cx + ry * s, 0), (h + 1) * s - 1);
     const float Xk = el(V.X0, Ec), Yk = el(V.Y0, Ec), Zk = el(V.Z0, Ec);
     const UcParams ucp = cam(V.ucp, po);  // before any store: scalar loads
-    JointIn in = joint_load<PRIOR, TS>(a, V, xc, yc, qb);
+    JointIn in = joint_load<PRIOR, TS, false, TI>(a, V, xc, yc, qb);
     const bool hl = xc > 0, hu = yc > 0;
     const int e_l = max(e - 1, 0), e_u = max(e - EXW, 0), e_r = min(e + 1, NE - 1), e_d = min(e + EXW, NE - 1);
 #pragma unroll 1
@@ -877,8 +895,11 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
         __syncthreads();
         const JointBack B = joint_back_calc<EXW, NE, true>(a, ucp, J, xc, yc, w, h, e, s_du, s_dv, s_X, s_Y, s_Z);
         if (k == K - 1) {
-            if (lx >= R && lx < EXW - R && ly >= R && ly < EXH - R && cx < w && ry < h)
-                joint_store(V, qb + (unsigned)cx + (unsigned)ry * (unsigned)s, J, B);
+            if (lx >= R && lx < EXW - R && ly >= R && ly < EXH - R && cx < w && ry < h) {
+                const unsigned q = qb + (unsigned)cx + (unsigned)ry * (unsigned)s;
+                joint_store(V, q, J, B);
+                if constexpr (!std::is_void<TI>::value) joint_store_deriv<TS>(V, q, qb, in);
+            }
             return;
         }
         const float Xn = pix ? B.X : Xk, Yn = pix ? B.Y : Yk, Zn = pix ? B.Z : Zk;
@@ -1643,27 +1664,28 @@ int Batch::warm_start_in_use(int i) {
     return bad ? 0 : 1;
 }
 
-template <typename TS, int SM, bool F64, bool PRIOR>
+// TK: the image type when the launch also computes the warp's derivatives, else void
+template <typename TK, typename TS, int SM, bool F64, bool PRIOR>
 static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
     const dim3 grid(a.L.ntiles);
     switch (cls) {
-        case TC_J64:
+        case TC_J64:  // never asked to compute the derivatives (fuses_deriv)
             hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, F64, PRIOR, TS>), grid, dim3(TXW, TRW), 0, st, a);
             break;
         case TC_J32:
-            hipLaunchKernelGGL((kb_joint_px<32, 16, 5, SM, F64, PRIOR, TS>), grid, dim3(32, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<32, 16, 5, SM, F64, PRIOR, TS, TK>), grid, dim3(32, 16), 0, st, a);
             break;
         case TC_J16O:
-            hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
             break;
         case TC_JK2:
-            hipLaunchKernelGGL((kb_joint_multi<16, 16, 2, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 2, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
             break;
         case TC_JK3:
-            hipLaunchKernelGGL((kb_joint_multi<16, 16, 3, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 3, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
             break;
         default:
-            hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SM, F64, PRIOR, TS>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
             break;
     }
 }
@@ -1703,26 +1725,37 @@ static void launch_flow(int cls, bool single, bool deriv, hipStream_t st, const 
 }
 
 // The iteration classes whose kernel can compute the warp's derivatives in its first launch.
-static bool fuses_deriv(int cls) { return cls == TC_FK2 || cls == TC_FK4 || cls == TC_FK6 || cls == TC_FK8; }
+static bool fuses_deriv(int cls) {
+    return cls == TC_FK2 || cls == TC_FK4 || cls == TC_FK6 || cls == TC_FK8 || cls == TC_J16 || cls == TC_J32 ||
+           cls == TC_J16O || cls == TC_JK2 || cls == TC_JK3;
+}
 
-// sm: how the kernels find the fields (see tile_view)
-template <typename TS>
-static void launch_joint(int cls, int sm, bool f64, bool prior, hipStream_t st, const IterArgs& a) {
+// sm: how the kernels find the fields (see tile_view); deriv: the launch also computes the warp's
+// derivatives (the caller then skips kb_deriv)
+template <typename TI, typename TS>
+static void launch_joint(int cls, int sm, bool f64, bool prior, bool deriv, hipStream_t st, const IterArgs& a) {
     const int k = (f64 ? 2 : 0) | (prior ? 1 : 0);
-#define RF_J(SM)                                                            \
-    switch (k) {                                                            \
-        case 0: launch_joint_t<TS, SM, false, false>(cls, st, a); break;    \
-        case 1: launch_joint_t<TS, SM, false, true>(cls, st, a); break;     \
-        case 2: launch_joint_t<TS, SM, true, false>(cls, st, a); break;     \
-        default: launch_joint_t<TS, SM, true, true>(cls, st, a); break;     \
+#define RF_J(TK, SM)                                                            \
+    switch (k) {                                                                \
+        case 0: launch_joint_t<TK, TS, SM, false, false>(cls, st, a); break;    \
+        case 1: launch_joint_t<TK, TS, SM, false, true>(cls, st, a); break;     \
+        case 2: launch_joint_t<TK, TS, SM, true, false>(cls, st, a); break;     \
+        default: launch_joint_t<TK, TS, SM, true, true>(cls, st, a); break;     \
     }
-    if (sm == 1) {
-        RF_J(1)
-    } else if (sm == 2) {
-        RF_J(2)
+#define RF_JS(TK)       \
+    if (sm == 1) {      \
+        RF_J(TK, 1)     \
+    } else if (sm == 2) { \
+        RF_J(TK, 2)     \
+    } else {            \
+        RF_J(TK, 0)     \
+    }
+    if (deriv) {
+        RF_JS(TI)
     } else {
-        RF_J(0)
+        RF_JS(void)
     }
+#undef RF_JS
 #undef RF_J
 }
 
@@ -1773,10 +1806,10 @@ hipError_t Batch::enqueue_t(bool recon) {
         bool warped = false;  // this warp's image was warped by the previous warp's last kernel
         for (int wi = 0; wi < p_.n_warp_iters; ++wi) {
             if (!warped) RF_LAUNCH(kb_warp<TI>, tiles(level, TC_PIX), level, pr.u);
-            // TV-L1: a multi-iteration first launch computes the derivatives itself
+            // the warp's first iteration launch computes the derivatives itself where its kernel can
             int cls0 = tile_class(recon, level, wi);
             if (kIters[cls0] > p_.n_solver_iters) cls0 = remainder_class(recon, p_.n_solver_iters);
-            const bool fused_deriv = !recon && p_.n_solver_iters > 0 && fuses_deriv(cls0);
+            const bool fused_deriv = p_.n_solver_iters > 0 && fuses_deriv(cls0);
             if (!fused_deriv) RF_LAUNCH((kb_deriv<TI, TS>), tiles(level, TC_PIX), level);
             if (p_.n_solver_iters == 0) RF_LAUNCH(kb_zero_uhat, tiles(level, TC_PIX), level, pr.d);
             prof_.begin(st_, RF_KID_FUSED_ITER, level);
@@ -1804,7 +1837,8 @@ hipError_t Batch::enqueue_t(bool recon) {
                     if (!recon)
                         launch_flow<TI, TS>(cls, single, fused_deriv && it == 0, st_, a);
                     else
-                        launch_joint<TS>(cls, single ? (n() == 1 ? 1 : 2) : 0, p_.tri_f64 != 0, fn, st_, a);
+                        launch_joint<TI, TS>(cls, single ? (n() == 1 ? 1 : 2) : 0, p_.tri_f64 != 0, fn,
+                                             fused_deriv && it == 0, st_, a);
                     RF_TRY(hipGetLastError());
                 }
                 pr.d ^= 1;  // one flip per launch, whatever its iteration count
