@@ -224,6 +224,26 @@ def test_recon_multi_iteration_launches_equal_oracle(orc, tile, w, h):
         assert _same(_run_recon(r, d), ref, w), rep
 
 
+@pytest.mark.parametrize("ni", [1, 2], ids=["1-iteration", "2-iterations"])
+def test_warps_shorter_than_a_launch_every_tiling(orc, ni):
+    # warps of fewer iterations than a multi-iteration tiling runs per launch: the warp's first
+    # launch is the remainder tiling, which then computes the warp's derivatives itself
+    print(f"case: joint and TV-L1 120x88 fused every tiling, {ni} iterations per warp", flush=True)
+    d = rf.make_recon_pair(120, 88, 19)
+    ref = _oracle_recon(orc, d, 120, 88, 3, "f32", tri=1, nw=2, ni=ni)
+    for tile in range(1, 7):
+        r = _recon(120, 88, 3, path=rf.PATH_FUSED, graph=True, tri=1, nw=2, ni=ni, joint_tile=tile)
+        assert _same(_run_recon(r, d), ref, 120), tile
+    i0, i1, _, _ = rf.make_flow_pair(120, 88, 20)
+    ru, rv = orc.solve_flow(orc.flow_params(120, 88, 3, n_warp=2, n_iter=ni), i0, i1, "f32")
+    for tile in range(1, 8):
+        f = _flow(path=rf.PATH_FUSED, graph=True, w=120, h=88, L=3, nw=2, ni=ni, flow_tile=tile)
+        f.copyImagesToDevice(i0, i1)
+        f.solveOpticalFlow(5.0)
+        u, v = f.copyOpticalFlowToHost()
+        assert np.array_equal(u[:, :120], ru[:, :120]) and np.array_equal(v[:, :120], rv[:, :120]), tile
+
+
 def test_tiny_images_every_tiling(orc):
     # levels smaller than one tile (40x24 -> 20x12 -> 10x6): every joint and TV-L1 tiling, the
     # multi-iteration ones with iteration counts they do not divide
