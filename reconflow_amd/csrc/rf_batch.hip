@@ -231,22 +231,65 @@ __global__ void __launch_bounds__(256) kb_deriv(Launch L, int l) {
     as<TS>(F.Iy)[pos] = st<TS>(dy);
 }
 
+// The 5x5 median kernels: a workgroup stages the (TXW + 4) x (TRW + 4) taps around its TXW x TRW
+// pixel tile of two fields in LDS, once, under median5_tap's rule (a tap's value depends only on
+// its own column and row: the nearest edge pixel when it is outside in one direction, 0 when
+// outside in both, as gather5x5 reads them), and each lane reads its 25 taps back at fixed
+// offsets. Every lane takes part in the staging; lanes off the image leave after the barrier.
+constexpr int MW = TXW + 4, MH = TRW + 4, MN = MW * MH;
+
+#define BT_MED5(L, LEVEL, A, B, MA, MB)                                                        \
+    const int2 pt = L.map[xcd_tile(blockIdx.x, L.ntiles)];                                    \
+    const PairFields& F = L.P[pt.x];                                                           \
+    const int w = F.W[LEVEL], h = F.H[LEVEL], s = F.S[LEVEL];                                  \
+    const int tiles_x = (w + TXW - 1) / TXW;                                                   \
+    const int x0 = (pt.y % tiles_x) * TXW, y0 = (pt.y / tiles_x) * TRW;                        \
+    const int ix = x0 + threadIdx.x, iy = y0 + threadIdx.y;                                    \
+    __shared__ float s_ma[MN], s_mb[MN];                                                       \
+    med5_stage((A), (B), x0, y0, w, h, s, s_ma, s_mb);                                         \
+    __syncthreads();                                                                           \
+    if (ix >= w || iy >= h) return;                                                            \
+    const long pos = ix + (long)iy * s;                                                        \
+    float MA[25], MB[25];                                                                      \
+    med5_taps(s_ma, s_mb, MA, MB);
+
+__device__ __forceinline__ void med5_stage(const float* a, const float* b, int x0, int y0, int w, int h, int s,
+                                           float* sa, float* sb) {
+    constexpr int NT = TXW * TRW;
+#pragma unroll
+    for (int r = 0; r < (MN + NT - 1) / NT; ++r) {
+        const int k = threadIdx.x + threadIdx.y * TXW + r * NT;
+        if (k >= MN) break;
+        const int row = y0 - 2 + k / MW, col = x0 - 2 + k % MW;
+        const bool zero = (row < 0 || row >= h) && (col < 0 || col >= w);
+        const unsigned q = (unsigned)min(max(row, 0), h - 1) * (unsigned)s + (unsigned)min(max(col, 0), w - 1);
+        sa[k] = zero ? 0.0f : a[q];
+        sb[k] = zero ? 0.0f : b[q];
+    }
+}
+
+// The lane's taps in gather5x5's order (row-major from (ix - 2, iy - 2)).
+__device__ __forceinline__ void med5_taps(const float* sa, const float* sb, float (&ma)[25], float (&mb)[25]) {
+    const int e = threadIdx.x + threadIdx.y * MW;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            ma[j * 5 + i] = sa[e + j * MW + i];
+            mb[j * 5 + i] = sb[e + j * MW + i];
+        }
+}
+
 // Median of uhat added to the flow (medianFilter.cu:3-75, addKernel.cu:3-11), in place.
 __global__ void __launch_bounds__(256) kb_med5_add(Launch L, int l, int pd, int pu) {
-    BT_PIX(L, l)
-    const float *du = F.du[pd], *dv = F.dv[pd];
-    float mu[25], mv[25];
-    gather5x5(du, dv, ix, iy, w, h, s, mu, mv);
+    BT_MED5(L, l, F.du[pd], F.dv[pd], mu, mv)
     F.u[pu][pos] = F.u[pu][pos] + median25(mu);
     F.v[pu][pos] = F.v[pu][pos] + median25(mv);
 }
 
 // Median of the flow (Reconflow.cu:418-420): u[pu] -> u[pu^1].
 __global__ void __launch_bounds__(256) kb_median5(Launch L, int l, int pu) {
-    BT_PIX(L, l)
-    const float *u = F.u[pu], *v = F.v[pu];
-    float mu[25], mv[25];
-    gather5x5(u, v, ix, iy, w, h, s, mu, mv);
+    BT_MED5(L, l, F.u[pu], F.v[pu], mu, mv)
     F.u[pu ^ 1][pos] = median25(mu);
     F.v[pu ^ 1][pos] = median25(mv);
 }
@@ -256,10 +299,7 @@ __global__ void __launch_bounds__(256) kb_median5(Launch L, int l, int pu) {
 // warp transition is saved. kb_med5_add + kb_warp<TI> (TV-L1):
 template <typename TI>
 __global__ void __launch_bounds__(256) kb_med5_add_warp(Launch L, int l, int pd, int pu) {
-    BT_PIX(L, l)
-    const float *du = F.du[pd], *dv = F.dv[pd];
-    float mu[25], mv[25];
-    gather5x5(du, dv, ix, iy, w, h, s, mu, mv);
+    BT_MED5(L, l, F.du[pd], F.dv[pd], mu, mv)
     const float un = F.u[pu][pos] + median25(mu), vn = F.v[pu][pos] + median25(mv);
     F.u[pu][pos] = un;
     F.v[pu][pos] = vn;
@@ -269,10 +309,7 @@ __global__ void __launch_bounds__(256) kb_med5_add_warp(Launch L, int l, int pd,
 // kb_median5 + kb_warp<TI> (joint): the median of the flow into the partner buffers, warped with.
 template <typename TI>
 __global__ void __launch_bounds__(256) kb_median5_warp(Launch L, int l, int pu) {
-    BT_PIX(L, l)
-    const float *u = F.u[pu], *v = F.v[pu];
-    float mu[25], mv[25];
-    gather5x5(u, v, ix, iy, w, h, s, mu, mv);
+    BT_MED5(L, l, F.u[pu], F.v[pu], mu, mv)
     const float un = median25(mu), vn = median25(mv);
     F.u[pu ^ 1][pos] = un;
     F.v[pu ^ 1][pos] = vn;
