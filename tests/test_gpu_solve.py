@@ -224,6 +224,24 @@ def test_recon_multi_iteration_launches_equal_oracle(orc, tile, w, h):
         assert _same(_run_recon(r, d), ref, w), rep
 
 
+@pytest.mark.parametrize("storage", [rf.STORAGE_F16, rf.STORAGE_F16_DERIV], ids=["f16", "f16-deriv"])
+def test_recon_fp16_tilings_equal(storage):
+    # FP16 storage: every joint tiling gives the bits of the 64x8 one, whose warps compute their
+    # derivatives in a launch of their own (the other tilings compute them in a warp's first
+    # iteration launch, from FP16 images in the f16 mode)
+    print(f"case: joint 200x152 fused graph every tiling storage={storage}", flush=True)
+    d = rf.make_recon_pair(200, 152, 23)
+    base = None
+    for tile in range(1, 7):
+        r = _recon(200, 152, 3, path=rf.PATH_FUSED, graph=True, tri=1, nw=2, ni=6, storage=storage, joint_tile=tile)
+        for rep in range(2):
+            got = _run_recon(r, d)
+            if base is None:
+                base = got
+            else:
+                assert _same(got, base, 200), (tile, rep)
+
+
 @pytest.mark.parametrize("ni", [1, 2], ids=["1-iteration", "2-iterations"])
 def test_warps_shorter_than_a_launch_every_tiling(orc, ni):
     # warps of fewer iterations than a multi-iteration tiling runs per launch: the warp's first
