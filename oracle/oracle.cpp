@@ -1048,7 +1048,56 @@ struct Params {
     int input_type;    // 0: float gray, 1: 8-bit RGB (uchar3), 2: 8-bit gray, 3: 16-bit gray
     int fix_coarse_k;  // 0: reference intrinsics pyramid (K[2][2] = 0 on levels >= 1); 1: K[2][2] = 1
     int tri_f64;       // 1: the triangulation step runs in FP64 whatever T is
+    // Test-only knobs (0 = off); neither is reference semantics.
+    //   mutation:   plants one known schedule mistake (MUT_* below) so that tests can show their
+    //               checks detect it;
+    //   round_mask: rounds the state groups whose RND_* bit is set to FP32 after every write
+    //               (FP64 arithmetic, FP32 storage of that group), to find which stored field
+    //               carries the FP32 solve's distance from the FP64 one.
+    int mutation;
+    int round_mask;
 };
+
+// Planted schedule mistakes (Params::mutation). TV-L1 schedule (Cudaflow.cu:403-621):
+enum {
+    MUT_NONE = 0,
+    MUT_FLOW_NO_WARP_RESET = 1,   // uhat and duals not zeroed at the start of each warp (:474-487)
+    MUT_FLOW_NO_MEDIAN = 2,       // the 5x5 median before the add skipped (:580-583)
+    MUT_FLOW_NO_DUAL_SWAP = 3,    // the duals' swap after the dual step dropped (:568-571)
+    MUT_FLOW_UPSCALE_NO_SCALE = 4,// upscale without the fScale factor (:602-611)
+    MUT_FLOW_WARP_ZERO_FLOW = 5,  // frame 1 warped with zero flow (:490)
+    MUT_FLOW_NO_UHAT_SWAP = 6,    // the uhat swap after the data step dropped (:548-549)
+    // joint schedule (Reconflow.cu:147-466):
+    MUT_RECON_NO_UV_MEDIAN = 11,  // the extra MedianFilter(u, v) per warp skipped (:418-420)
+    MUT_RECON_SK_RESET = 12,      // sk zeroed at every warp (the reference keeps it: :223-224, 260-261 are commented out)
+    MUT_RECON_NO_UC_UPSCALE = 13, // uc, vc not upscaled to the next level (:438-443)
+    MUT_RECON_NO_SK_SWAP = 14,    // the sk swap after UpdateSk dropped (:405)
+    MUT_RECON_NO_CLEANUP = 15,    // CleanUp3D skipped (:383-391)
+    MUT_RECON_NO_PROJ_UPSCALE = 16, // uproj, vproj not upscaled (:445-450)
+    MUT_RECON_NO_WARP_RESET = 17, // uhat and duals not zeroed at the start of each warp (:245-258)
+    MUT_RECON_XYZ_LEVEL_RESET = 18, // X, Y, Z zeroed at every level (the reference zeroes them once, :216-218)
+    MUT_RECON_UC_LEVEL0_K = 19,   // SolveUc with the level-0 intrinsics on every level (:397)
+};
+
+// State groups of Params::round_mask.
+enum {
+    RND_IMG = 1,    // image pyramids and the warped image
+    RND_DER = 2,    // Ix, Iy, Iz
+    RND_UHAT = 4,   // uhat (dumed, dvmed)
+    RND_DUAL = 8,   // p (pu1, pu2, pv1, pv2)
+    RND_PROJ = 16,  // uproj, vproj
+    RND_XYZ = 32,   // X, Y, Z and their 3x3 medians
+    RND_UC = 64,    // uc, vc
+    RND_SK = 128,   // sku, skv
+    RND_UV = 256,   // u, v
+};
+
+template <typename T>
+static void round_group(const Params& P, int bit, long n, std::initializer_list<T*> bufs) {
+    if (sizeof(T) == sizeof(float) || !(P.round_mask & bit)) return;
+    for (T* b : bufs)
+        for (long i = 0; i < n; ++i) b[i] = (T)(float)b[i];
+}
 
 // The triangulation step of the FP64-triangulation mode on T-typed fields: solve3d_fma.
 template <typename T>
@@ -1189,6 +1238,8 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
 
     gray_input(P, i0, pI0[0], s0);
     gray_input(P, i1, pI1[0], s0);
+    round_group<T>(P, RND_IMG, n, {pI0[0], pI1[0]});
+    const int mut = P.mutation;
     if (cg) {
         deriv_single(pI0[0], pW[0], pH[0], s0, pIx0[0], pIy0[0]);
         deriv_single(pI1[0], pW[0], pH[0], s0, pIx1[0], pIy1[0]);
@@ -1200,6 +1251,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
         };
         down(pI0);
         down(pI1);
+        round_group<T>(P, RND_IMG, n, {pI0[l], pI1[l]});
         if (cg) {
             down(pIx0);
             down(pIx1);
@@ -1215,8 +1267,13 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
     for (int level = P.n_levels - 1; level >= 0; --level) {
         const int w = pW[level], h = pH[level], s = pS[level];
         for (int wi = 0; wi < P.n_warp_iters; ++wi) {
-            for (T* b : {du, dv, dus, dvs, dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
-            warp(pI1[level], w, h, s, u, v, i1warp);
+            if (!(mut == MUT_FLOW_NO_WARP_RESET && (wi > 0 || level < P.n_levels - 1)))
+                for (T* b : {du, dv, dus, dvs, dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
+            if (mut == MUT_FLOW_WARP_ZERO_FLOW)
+                std::copy(pI1[level], pI1[level] + n, i1warp);
+            else
+                warp(pI1[level], w, h, s, u, v, i1warp);
+            round_group<T>(P, RND_IMG, n, {i1warp});
             if (cg) {
                 warp(pIx1[level], w, h, s, u, v, ix1warp);
                 warp(pIy1[level], w, h, s, u, v, iy1warp);
@@ -1225,6 +1282,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
                 derivatives_patch(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
             else
                 derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            round_group<T>(P, RND_DER, n, {Ix, Iy, Iz});
             if (cg) {
                 derivatives(pIx0[level], ix1warp, w, h, s, Ixx, Ixy, Ixz);
                 derivatives(pIy0[level], iy1warp, w, h, s, Iyx, Iyy, Iyz);
@@ -1241,26 +1299,36 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
                 } else {
                     data_l1(dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda, theta, dumeds, dvmeds);
                 }
+                if (mut != MUT_FLOW_NO_UHAT_SWAP) {
+                    std::swap(dumed, dumeds);
+                    std::swap(dvmed, dvmeds);
+                }
+                round_group<T>(P, RND_UHAT, n, {dumed, dvmed});
+                dual_tv(dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, theta, pu1s, pu2s, pv1s, pv2s);
+                if (mut != MUT_FLOW_NO_DUAL_SWAP) {
+                    std::swap(pu1, pu1s);
+                    std::swap(pu2, pu2s);
+                    std::swap(pv1, pv1s);
+                    std::swap(pv2, pv2s);
+                }
+                round_group<T>(P, RND_DUAL, n, {pu1, pu2, pv1, pv2});
+            }
+            if (mut != MUT_FLOW_NO_MEDIAN) {
+                median5(dumed, dvmed, w, h, s, dumeds, dvmeds);
                 std::swap(dumed, dumeds);
                 std::swap(dvmed, dvmeds);
-                dual_tv(dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, theta, pu1s, pu2s, pv1s, pv2s);
-                std::swap(pu1, pu1s);
-                std::swap(pu2, pu2s);
-                std::swap(pv1, pv1s);
-                std::swap(pv2, pv2s);
             }
-            median5(dumed, dvmed, w, h, s, dumeds, dvmeds);
-            std::swap(dumed, dumeds);
-            std::swap(dvmed, dvmeds);
             add(u, dumed, (long)h * s, u);
             add(v, dvmed, (long)h * s, v);
+            round_group<T>(P, RND_UV, n, {u, v});
         }
         if (level > 0) {
-            T sc = P.scale;
+            T sc = mut == MUT_FLOW_UPSCALE_NO_SCALE ? T(1) : (T)P.scale;
             upscale(u, w, h, s, pW[level - 1], pH[level - 1], pS[level - 1], sc, us);
             upscale(v, w, h, s, pW[level - 1], pH[level - 1], pS[level - 1], sc, vs);
             std::swap(u, us);
             std::swap(v, vs);
+            round_group<T>(P, RND_UV, n, {u, v});
         }
     }
     for (long i = 0; i < n; ++i) {
@@ -1348,11 +1416,14 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
     k_pyramid(K0, P.n_levels, P.scale, P.fix_coarse_k, pK0);
     k_pyramid(K1, P.n_levels, P.scale, P.fix_coarse_k, pK1);
 
+    const int mut = P.mutation;
     gray_input(P, i0, pI0[0], s0);
     gray_input(P, i1, pI1[0], s0);
+    round_group<T>(P, RND_IMG, n, {pI0[0], pI1[0]});
     for (int l = 1; l < P.n_levels; ++l) {
         downscale(pI0[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI0[l]);
         downscale(pI1[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI1[l]);
+        round_group<T>(P, RND_IMG, n, {pI0[l], pI1[l]});
     }
     for (T* b : {u, v, uproj, vproj, uc, vc, X, Y, Z, sku, skv}) zero(b, n);
     apply_warm_start(wsu, wsv, pW, pH, pS, pool, u, v);
@@ -1365,6 +1436,9 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
         double Pm[12], Qm[12];
         kmul(pK0[level].data(), R0, t0, Pm);
         kmul(pK1[level].data(), R1, t1, Qm);
+        const double* Kuc = mut == MUT_RECON_UC_LEVEL0_K ? pK1[0].data() : pK1[level].data();
+        if (mut == MUT_RECON_XYZ_LEVEL_RESET)
+            for (T* b : {X, Y, Z, Xs, Ys, Zs}) zero(b, n);
         if (fn) {
             float sc = (float)pW[level] / (float)pW[0];
             downscale(ufn, pW[0], pH[0], pS[0], w, h, s, (T)sc, true, ufn_l);
@@ -1372,9 +1446,14 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
             downscale(mask, pW[0], pH[0], pS[0], w, h, s, T(1), false, mask_l);
         }
         for (int wi = 0; wi < P.n_warp_iters; ++wi) {
-            for (T* b : {dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
+            if (!(mut == MUT_RECON_NO_WARP_RESET && (wi > 0 || level < P.n_levels - 1)))
+                for (T* b : {dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
+            if (mut == MUT_RECON_SK_RESET)
+                for (T* b : {sku, skv}) zero(b, n);
             warp(pI1[level], w, h, s, u, v, i1warp);
+            round_group<T>(P, RND_IMG, n, {i1warp});
             derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            round_group<T>(P, RND_DER, n, {Ix, Iy, Iz});
             for (int it = 0; it < P.n_solver_iters; ++it) {
                 if (fn)
                     recon_data_l1_fn(u, v, dumed, dvmed, uproj, vproj, ufn_l, vfn_l, sku, skv, pu1, pu2, pv1,
@@ -1385,12 +1464,15 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                                   w, h, s, lambda, alphaTv, alphaProj, dumeds, dvmeds);
                 std::swap(dumed, dumeds);
                 std::swap(dvmed, dvmeds);
+                round_group<T>(P, RND_UHAT, n, {dumed, dvmed});
                 dual_tv(dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, thetaTv, pu1s, pu2s, pv1s, pv2s);
                 std::swap(pu1, pu1s);
                 std::swap(pu2, pu2s);
                 std::swap(pv1, pv1s);
                 std::swap(pv2, pv2s);
+                round_group<T>(P, RND_DUAL, n, {pu1, pu2, pv1, pv2});
                 uvproj(u, v, dumed, dvmed, uc, vc, sku, skv, w, h, s, lambdaf, alphaProj, uproj, vproj);
+                round_group<T>(P, RND_PROJ, n, {uproj, vproj});
                 if (P.tri_f64)
                     solve3d_wide(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, Xs, Ys, Zs);
                 else
@@ -1398,24 +1480,35 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                 std::swap(X, Xs);
                 std::swap(Y, Ys);
                 std::swap(Z, Zs);
-                median3d(X, Y, Z, w, h, s, Xm, Ym, Zm);
-                cleanup3d(X, Y, Z, Xm, Ym, Zm, w, h, s, Xs, Ys, Zs);
-                std::swap(X, Xs);
-                std::swap(Y, Ys);
-                std::swap(Z, Zs);
-                solve_uc(X, Y, Z, R1, t1, pK1[level].data(), w, h, s, uc, vc);
+                round_group<T>(P, RND_XYZ, n, {X, Y, Z});
+                if (mut != MUT_RECON_NO_CLEANUP) {
+                    median3d(X, Y, Z, w, h, s, Xm, Ym, Zm);
+                    cleanup3d(X, Y, Z, Xm, Ym, Zm, w, h, s, Xs, Ys, Zs);
+                    std::swap(X, Xs);
+                    std::swap(Y, Ys);
+                    std::swap(Z, Zs);
+                    round_group<T>(P, RND_XYZ, n, {X, Y, Z});
+                }
+                solve_uc(X, Y, Z, R1, t1, Kuc, w, h, s, uc, vc);
+                round_group<T>(P, RND_UC, n, {uc, vc});
                 update_sk(sku, skv, u, v, dumed, dvmed, uproj, vproj, w, h, s, skus, skvs);
-                std::swap(sku, skus);
-                std::swap(skv, skvs);
+                if (mut != MUT_RECON_NO_SK_SWAP) {
+                    std::swap(sku, skus);
+                    std::swap(skv, skvs);
+                }
+                round_group<T>(P, RND_SK, n, {sku, skv});
             }
             median5(dumed, dvmed, w, h, s, dumeds, dvmeds);
             std::swap(dumed, dumeds);
             std::swap(dvmed, dvmeds);
             add(u, dumed, (long)h * s, u);
             add(v, dvmed, (long)h * s, v);
-            median5(u, v, w, h, s, dumeds, dvmeds);
-            std::swap(u, dumeds);
-            std::swap(v, dvmeds);
+            if (mut != MUT_RECON_NO_UV_MEDIAN) {
+                median5(u, v, w, h, s, dumeds, dvmeds);
+                std::swap(u, dumeds);
+                std::swap(v, dvmeds);
+            }
+            round_group<T>(P, RND_UV, n, {u, v});
         }
         if (level > 0) {
             const int W = pW[level - 1], H = pH[level - 1], S = pS[level - 1];
@@ -1424,18 +1517,26 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
             upscale(v, w, h, s, W, H, S, sc, vs);
             std::swap(u, us);
             std::swap(v, vs);
-            upscale(uc, w, h, s, W, H, S, sc, us);
-            upscale(vc, w, h, s, W, H, S, sc, vs);
-            std::swap(uc, us);
-            std::swap(vc, vs);
-            upscale(uproj, w, h, s, W, H, S, sc, us);
-            upscale(vproj, w, h, s, W, H, S, sc, vs);
-            std::swap(uproj, us);
-            std::swap(vproj, vs);
+            if (mut != MUT_RECON_NO_UC_UPSCALE) {
+                upscale(uc, w, h, s, W, H, S, sc, us);
+                upscale(vc, w, h, s, W, H, S, sc, vs);
+                std::swap(uc, us);
+                std::swap(vc, vs);
+            }
+            if (mut != MUT_RECON_NO_PROJ_UPSCALE) {
+                upscale(uproj, w, h, s, W, H, S, sc, us);
+                upscale(vproj, w, h, s, W, H, S, sc, vs);
+                std::swap(uproj, us);
+                std::swap(vproj, vs);
+            }
             upscale(sku, w, h, s, W, H, S, sc, us);
             upscale(skv, w, h, s, W, H, S, sc, vs);
             std::swap(sku, us);
             std::swap(skv, vs);
+            round_group<T>(P, RND_UV, n, {u, v});
+            round_group<T>(P, RND_UC, n, {uc, vc});
+            round_group<T>(P, RND_PROJ, n, {uproj, vproj});
+            round_group<T>(P, RND_SK, n, {sku, skv});
         }
     }
     for (long i = 0; i < n; ++i) {
@@ -1608,6 +1709,18 @@ void orc_pyramid(const orc::Params* P, int* W, int* H, int* S) {
 }
 
 int orc_params_size(void) { return (int)sizeof(orc::Params); }
+
+// The per-level intrinsics and camera matrices the joint solve uses (setCameraMatrices,
+// Reconflow.cu:78-116, and K*[R|t], Reconflow.cu:225-228): Kl (n_levels x 9) and Pl (n_levels x 12).
+void orc_camera_pyramid(const double* K, const double* R, const double* t, int n_levels, float scale,
+                        int fix_coarse_k, double* Kl, double* Pl) {
+    std::vector<std::vector<double>> pK;
+    orc::k_pyramid(K, n_levels, scale, fix_coarse_k, pK);
+    for (int l = 0; l < n_levels; ++l) {
+        for (int i = 0; i < 9; ++i) Kl[l * 9 + i] = pK[l][i];
+        orc::kmul(pK[l].data(), R, t, Pl + l * 12);
+    }
+}
 
 // Host threads of the OpenMP loops (for timing the restatement at a stated core count).
 void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }

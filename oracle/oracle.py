@@ -25,8 +25,19 @@ class Params(C.Structure):
         ("tau", C.c_float), ("n_warp_iters", C.c_int), ("n_solver_iters", C.c_int), ("lambdaf", C.c_float),
         ("lambdams", C.c_float), ("alpha_tv", C.c_float), ("alpha_proj", C.c_float), ("alpha_fn", C.c_float),
         ("stride_alignment", C.c_int), ("input_type", C.c_int), ("fix_coarse_k", C.c_int),
-        ("tri_f64", C.c_int),
+        ("tri_f64", C.c_int), ("mutation", C.c_int), ("round_mask", C.c_int),
     ]
+
+
+# Planted schedule mistakes (Params.mutation; oracle.cpp MUT_*), for tests that show a check
+# detects them. Never reference semantics.
+MUT = dict(flow_no_warp_reset=1, flow_no_median=2, flow_no_dual_swap=3, flow_upscale_no_scale=4,
+           flow_warp_zero_flow=5, flow_no_uhat_swap=6,
+           recon_no_uv_median=11, recon_sk_reset=12, recon_no_uc_upscale=13, recon_no_sk_swap=14,
+           recon_no_cleanup=15, recon_no_proj_upscale=16, recon_no_warp_reset=17, recon_xyz_level_reset=18,
+           recon_uc_level0_k=19)
+# State groups of Params.round_mask (oracle.cpp RND_*): FP64 arithmetic with that group stored in FP32.
+RND = dict(img=1, der=2, uhat=4, dual=8, proj=16, xyz=32, uc=64, sk=128, uv=256)
 
 
 def build() -> Path:
@@ -44,6 +55,8 @@ def lib() -> C.CDLL:
         if not LIB.exists():
             build()
         _lib = C.CDLL(str(LIB))
+        _lib.orc_params_size.restype = C.c_int
+        _lib.orc_params_size.argtypes = []
         assert _lib.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
     return _lib
 
@@ -56,87 +69,210 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def _fn(name, kind):
-    f = getattr(lib(), f"orc_{name}_{kind}")
+# Signatures of every orc_* symbol (oracle.cpp, extern "C"). Codes: T = pointer to the kind's
+# float type (float64 for _f64, float32 for _f32), D = pointer to float64, F = pointer to float32,
+# Q = optional T pointer (None allowed), B = pointer to uint8, W = pointer to uint16,
+# i = int, l = long, f = float. A call with the wrong arity, a wrong scalar type or an array of
+# the wrong dtype or layout raises TypeError before anything reaches the library.
+_T9, _T6 = "T" * 9, "T" * 6
+SIGS = {
+    "gray_rgb": ("BTiii", None), "gray_u8": ("BTiii", None), "gray_u16": ("WTiii", None),
+    "downscale": ("Tiiiiii" + "T", None),
+    "downscale_scaled": ("Tiiiiiif" + "T", None),
+    "upscale": ("Tiiiiiif" + "T", None),
+    "warp": ("TiiiTTT", None),
+    "derivatives": ("TTiiiTTT", None),
+    "data_l1": (_T9 + "iiiffTT", None),
+    "dual_tv": (_T6 + "iiiffTTTT", None),
+    "median5": ("TTiiiTT", None),
+    "median3d": ("TTTiiiTTT", None),
+    "add": ("TTlT", None),
+    "recon_data_l1": ("T" * 15 + "iiifffTT", None),
+    "recon_data_l1_fn": ("T" * 17 + "iiiffffQTT", None),
+    "uvproj": ("T" * 8 + "iiiffTT", None),
+    "solve3d": ("TTTTTDDffiiiTTT", None),
+    "solve3d_fma": ("TTTTTDDffiiiTTT", None),
+    "cleanup3d": (_T6 + "iiiTTT", None),
+    "solve_uc": ("TTTDDDiiiTT", None),
+    "update_sk": ("T" * 8 + "iiiTT", None),
+    "scene_flow_to_hsv": ("TiiiTf", None),
+    "flow_to_hsv": ("TTiiiTf", None),
+    "remove_edge": ("TTTifiiiTTT", None),
+    "deriv_single": ("TiiiTT", None),
+    "derivatives_patch": ("TTiiiTTT", None),
+    "data_charb": ("T" * 11 + "Q" * 6 + "iiifffTTTT", None),
+    "data_l1_inpaint": ("T" * 10 + "iiiffTT", None),
+    "scene_flow": ("TTTTffffiiiTTT", None),
+}
+_CT = {"T": C.c_void_p, "D": C.c_void_p, "F": C.c_void_p, "Q": C.c_void_p, "B": C.c_void_p, "W": C.c_void_p,
+       "i": C.c_int, "l": C.c_long, "f": C.c_float}
+_DT = {"D": np.float64, "F": np.float32, "B": np.uint8, "W": np.uint16}
+_bound = {}
+
+
+def _bind(sym: str, codes: str, restype):
+    f = _bound.get(sym)
+    if f is None:
+        f = getattr(lib(), sym)
+        f.argtypes = [_CT[c] for c in codes]
+        f.restype = restype
+        _bound[sym] = f
     return f
 
 
-def _ints(*vals):
-    return [C.c_int(int(v)) for v in vals]
+def _conv(code: str, a, kind: str, pos: int, name: str):
+    where = f"orc_{name}_{kind} argument {pos}"
+    if code in "if":
+        if isinstance(a, (bool, np.bool_)) or not isinstance(a, (int, float, np.integer, np.floating)):
+            raise TypeError(f"{where}: expected a number, got {type(a).__name__}")
+        if code == "i":
+            if isinstance(a, (float, np.floating)):
+                raise TypeError(f"{where}: expected an int, got a float")
+            return int(a)
+        return float(a)
+    if code == "l":
+        if not isinstance(a, (int, np.integer)) or isinstance(a, bool):
+            raise TypeError(f"{where}: expected an int count")
+        return int(a)
+    if a is None:
+        if code == "Q":
+            return None
+        raise TypeError(f"{where}: expected an array, got None")
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"{where}: expected a numpy array, got {type(a).__name__}")
+    want = _np_t(kind) if code in "TQ" else _DT[code]
+    if a.dtype != want:
+        raise TypeError(f"{where}: expected {np.dtype(want).name}, got {a.dtype.name}")
+    if not a.flags.c_contiguous:
+        raise TypeError(f"{where}: array must be C-contiguous")
+    return _p(a)
 
 
-def _fl(*vals):
-    return [C.c_float(float(v)) for v in vals]
-
-
-def _field(shape_len, kind):
-    return np.zeros(shape_len, _np_t(kind))
-
-
-# ---- per-kernel calls: every array is a flat buffer in the kind's dtype ----------------
 def call(name: str, kind: str, *args):
-    """Calls orc_<name>_<kind> with numpy arrays (pointers), ints and floats as given.
-    Python floats become C float, Python ints C int, numpy arrays pointers, and
-    ``('double*', arr)`` tuples double pointers."""
-    cargs = []
-    for a in args:
-        if isinstance(a, np.ndarray):
-            cargs.append(_p(a))
-        elif isinstance(a, bool):
-            cargs.append(C.c_int(int(a)))
-        elif isinstance(a, (int, np.integer)):
-            cargs.append(C.c_long(int(a)) if name == "add" and len(cargs) == 2 else C.c_int(int(a)))
-        elif isinstance(a, (float, np.floating)):
-            cargs.append(C.c_float(float(a)))
-        elif a is None:
-            cargs.append(C.c_void_p(0))
-        else:
-            raise TypeError(type(a))
-    f = _fn(name, kind)
-    f.restype = C.c_int
-    return f(*cargs)
+    """Calls orc_<name>_<kind> (a per-kernel restatement) through its declared signature.
+    Arrays are flat buffers: those of the kind's float type, plus float64 matrices (P, Q, R, t,
+    K) and uint8 / uint16 inputs where the function takes them."""
+    if name not in SIGS:
+        raise KeyError(f"unknown oracle function {name!r}")
+    if kind not in ("f64", "f32"):
+        raise ValueError(kind)
+    codes, restype = SIGS[name]
+    if len(args) != len(codes):
+        raise TypeError(f"orc_{name}_{kind} takes {len(codes)} arguments, got {len(args)}")
+    cargs = [_conv(c, a, kind, i, name) for i, (c, a) in enumerate(zip(codes, args))]
+    return _bind(f"orc_{name}_{kind}", codes, restype)(*cargs)
+
+
+_VP, _I, _F = C.c_void_p, C.c_int, C.c_float
+_PP = C.POINTER(Params)
+# Whole-solve and helper symbols: (argtypes, restype).
+_SOLVE_SIGS = {
+    "orc_params_size": ([], _I),
+    "orc_set_threads": ([_I], None),
+    "orc_get_threads": ([], _I),
+    "orc_pyramid": ([_PP, _VP, _VP, _VP], None),
+    "orc_warm_start_usable": ([_VP, _VP, _I, _I, _I], _I),
+    "orc_camera_pyramid": ([_VP, _VP, _VP, _I, _F, _I, _VP, _VP], None),
+    "orc_solve_flow_f64": ([_PP] + [_VP] * 6, _I), "orc_solve_flow_f32": ([_PP] + [_VP] * 6, _I),
+    "orc_solve_flow_masked_f64": ([_PP] + [_VP] * 5, _I), "orc_solve_flow_masked_f32": ([_PP] + [_VP] * 5, _I),
+    "orc_solve_recon_f64": ([_PP] + [_VP] * 18, _I), "orc_solve_recon_f32": ([_PP] + [_VP] * 18, _I),
+}
+
+
+def _sym(name: str):
+    f = _bound.get(name)
+    if f is None:
+        f = getattr(lib(), name)
+        f.argtypes, f.restype = _SOLVE_SIGS[name]
+        _bound[name] = f
+    return f
+
+
+def _kind(kind: str) -> str:
+    if kind not in ("f64", "f32"):
+        raise ValueError(f"kind must be 'f64' or 'f32', got {kind!r}")
+    return kind
+
+
+_IMG = {0: (np.float32, ()), 1: (np.uint8, (3,)), 2: (np.uint8, ()), 3: (np.uint16, ())}
+
+
+def _image(p: Params, a, s0: int):
+    """An input image of p.input_type, (h, stride[, 3]), checked against the solve's size."""
+    dt, tail = _IMG[p.input_type]
+    a = np.asarray(a)
+    if a.dtype != dt or a.shape != (p.height, s0, *tail):
+        raise TypeError(f"image must be {np.dtype(dt).name} of shape {(p.height, s0, *tail)}, got "
+                        f"{a.dtype.name} {a.shape}")
+    return np.ascontiguousarray(a)
+
+
+def _plane(a, p: Params, s0: int, what: str):
+    a = np.ascontiguousarray(a, np.float32)
+    if a.shape != (p.height, s0):
+        raise TypeError(f"{what} must be float32 of shape {(p.height, s0)}, got {a.shape}")
+    return a
+
+
+def _mat(a, n: int, what: str):
+    a = np.ascontiguousarray(np.asarray(a, np.float64).ravel())
+    if a.size != n:
+        raise TypeError(f"{what} must hold {n} doubles, got {a.size}")
+    return a
 
 
 def set_threads(n: int) -> None:
     """OpenMP threads of the restatement's row loops."""
-    lib().orc_set_threads(int(n))
+    _sym("orc_set_threads")(int(n))
 
 
 def get_threads() -> int:
-    return int(lib().orc_get_threads())
+    return int(_sym("orc_get_threads")())
 
 
 def pyramid(p: Params):
     n = p.n_levels
     W, H, S = (C.c_int * n)(), (C.c_int * n)(), (C.c_int * n)()
-    lib().orc_pyramid(C.byref(p), W, H, S)
+    _sym("orc_pyramid")(C.byref(p), W, H, S)
     return list(W), list(H), list(S)
 
 
-def _ws(ws):
+def camera_pyramid(K, R, t, n_levels: int, scale: float, fix_coarse_k: int = 0):
+    """Per-level intrinsics (n_levels, 3, 3) and K_l [R|t] (n_levels, 3, 4) of the joint solve."""
+    Kd, Rd, td = _mat(K, 9, "K"), _mat(R, 9, "R"), _mat(t, 3, "t")
+    Kl = np.zeros((n_levels, 3, 3))
+    Pl = np.zeros((n_levels, 3, 4))
+    _sym("orc_camera_pyramid")(_p(Kd), _p(Rd), _p(td), int(n_levels), float(scale), int(fix_coarse_k), _p(Kl),
+                               _p(Pl))
+    return Kl, Pl
+
+
+def _ws(ws, p: Params, s0: int):
     """A warm start (u, v), level-0 (h, stride) float32 planes, as two pointers (or two NULLs)."""
     if ws is None:
         return [None, None], []
-    keep = [np.ascontiguousarray(a, np.float32) for a in ws]
+    keep = [_plane(a, p, s0, "warm start") for a in ws]
     return [_p(a) for a in keep], keep
 
 
 def warm_start_usable(wsu, wsv, w, h) -> bool:
     """The warm-start rule: every component over the w x h pixels finite and at most max(w, h)."""
     a, b = (np.ascontiguousarray(x, np.float32) for x in (wsu, wsv))
-    return bool(lib().orc_warm_start_usable(_p(a), _p(b), int(w), int(h), int(a.shape[1])))
+    if a.shape != b.shape or a.ndim != 2 or a.shape[0] < h or a.shape[1] < w:
+        raise TypeError("warm start planes must be (h, stride) float32 of the same shape")
+    return bool(_sym("orc_warm_start_usable")(_p(a), _p(b), int(w), int(h), int(a.shape[1])))
 
 
 def solve_flow(p: Params, i0: np.ndarray, i1: np.ndarray, kind: str = "f64", ws=None):
-    """Whole CudaFlow TV-L1 solve. Images are (h, stride) float32 (or uint8 rgb). ws: an optional
-    warm start (u, v) that replaces the zero initial flow at the coarsest level."""
+    """Whole CudaFlow TV-L1 solve. Images are (h, stride) of p.input_type (float32, or uint8 rgb
+    (h, stride, 3), uint8, uint16). ws: an optional warm start (u, v) that replaces the zero initial
+    flow at the coarsest level."""
     s0 = pyramid(p)[2][0]
+    a0, a1 = _image(p, i0, s0), _image(p, i1, s0)
     u = np.zeros((p.height, s0), np.float32)
     v = np.zeros_like(u)
-    f = getattr(lib(), f"orc_solve_flow_{kind}")
-    f.restype = C.c_int
-    wsp, keep = _ws(ws)
-    rc = f(C.byref(p), _p(np.ascontiguousarray(i0)), _p(np.ascontiguousarray(i1)), *wsp, _p(u), _p(v))
+    wsp, keep = _ws(ws, p, s0)
+    rc = _sym(f"orc_solve_flow_{_kind(kind)}")(C.byref(p), _p(a0), _p(a1), *wsp, _p(u), _p(v))
     assert rc == 0
     del keep
     return u, v
@@ -145,27 +281,28 @@ def solve_flow(p: Params, i0: np.ndarray, i1: np.ndarray, kind: str = "f64", ws=
 def solve_flow_masked(p: Params, i0: np.ndarray, i1: np.ndarray, mask0: np.ndarray, kind: str = "f64"):
     """CudaFlow::solveInpainting: the TV-L1 solve with the masked L1 step (mask0 (h, stride) float32)."""
     s0 = pyramid(p)[2][0]
+    a0, a1 = _image(p, i0, s0), _image(p, i1, s0)
     u = np.zeros((p.height, s0), np.float32)
     v = np.zeros_like(u)
-    f = getattr(lib(), f"orc_solve_flow_masked_{kind}")
-    f.restype = C.c_int
-    m = np.ascontiguousarray(mask0, dtype=np.float32)
-    rc = f(C.byref(p), _p(np.ascontiguousarray(i0)), _p(np.ascontiguousarray(i1)), _p(m), _p(u), _p(v))
+    m = _plane(mask0, p, s0, "mask0")
+    rc = _sym(f"orc_solve_flow_masked_{_kind(kind)}")(C.byref(p), _p(a0), _p(a1), _p(m), _p(u), _p(v))
     assert rc == 0
     return u, v
 
 
 def solve_recon(p: Params, K0, K1, R0, t0, R1, t1, i0, i1, kind="f64", ufn=None, vfn=None, mask=None, ws=None):
+    """Whole ReconFlow joint solve: returns (u, v, X, Y, Z), each (h, stride) float32. ufn, vfn,
+    mask: the prior flow of methods 17 / 20; ws: an optional warm start."""
     s0 = pyramid(p)[2][0]
+    a0, a1 = _image(p, i0, s0), _image(p, i1, s0)
     outs = [np.zeros((p.height, s0), np.float32) for _ in range(5)]
-    dd = [np.ascontiguousarray(np.asarray(m, np.float64).ravel()) for m in (K0, K1, R0, t0, R1, t1)]
-    f = getattr(lib(), f"orc_solve_recon_{kind}")
-    f.restype = C.c_int
-    optk = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (ufn, vfn, mask)]
+    dd = [_mat(m, n, w) for m, n, w in ((K0, 9, "K0"), (K1, 9, "K1"), (R0, 9, "R0"), (t0, 3, "t0"), (R1, 9, "R1"),
+                                        (t1, 3, "t1"))]
+    optk = [None if a is None else _plane(a, p, s0, "prior flow") for a in (ufn, vfn, mask)]
     opt = [None if a is None else _p(a) for a in optk]
-    wsp, keep = _ws(ws)
-    rc = f(C.byref(p), *[_p(d) for d in dd], _p(np.ascontiguousarray(i0)), _p(np.ascontiguousarray(i1)),
-           *opt, *wsp, *[_p(o) for o in outs])
+    wsp, keep = _ws(ws, p, s0)
+    rc = _sym(f"orc_solve_recon_{_kind(kind)}")(C.byref(p), *[_p(d) for d in dd], _p(a0), _p(a1), *opt, *wsp,
+                                                *[_p(o) for o in outs])
     assert rc == 0
     del keep, optk
     return tuple(outs)

@@ -1,0 +1,408 @@
+"""Pins of the oracle's solve schedules, camera set-up, prior-flow data step and add (CPU only).
+
+The per-kernel functions are pinned in test_oracle_pins*.py. Here the parts that assemble them
+are checked against facts that do not come from the oracle itself:
+
+* ground truth: on the seeded synthetic pairs, whose true flow (and depth) the generator knows,
+  the FP64 TV-L1 solve (CudaFlow::_solveOpticalFlow, Cudaflow.cu:403-621) and the FP64 joint solve
+  (ReconFlow::solveReconFlow, Reconflow.cu:147-466, with K[2][2] = 1 on the coarse levels and the
+  FP64 triangulation) recover it to a stated accuracy;
+* the camera pyramid (setCameraMatrices, Reconflow.cu:78-116) and P = K_l [R|t]
+  (Reconflow.cu:225-228) against numpy;
+* the prior-flow data step (SolveReconDataL1Fn, solveReconDataL1.cu:139-274, 276-409) against
+  the plain joint data step at alphaFn = 0 and against the closed-form three-term blend;
+* Add (addKernel.cu:3-18) against numpy;
+* the schedules at small sizes against a composition of the separately pinned per-kernel
+  functions, read step by step off the reference loops (the cited lines), bit for bit;
+* planted mistakes (the oracle's test-only ``mutation`` knob: a dropped swap, a missing reset,
+  median or upscale, wrong intrinsics) each fail at least one of these checks.
+"""
+import numpy as np
+import pytest
+
+import reconflow_amd as rf
+
+BORDER = 8
+
+
+def _interior(a, w, h, b=BORDER):
+    return a[b:h - b, b:w - b]
+
+
+# ------------------------------------------------------------------ ground truth
+def _flow_epe(u, v, gu, gv, w, h):
+    return np.hypot(_interior(u - gu, w, h), _interior(v - gv, w, h))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tvl1_recovers_generator_flow(orc, seed):
+    """320x240 pairs of the config-0 generator (seeded band-limited texture moved by a smooth
+    flow, with noise), the config-0 schedule: mean end-point error against the generator's flow,
+    8 px from the border, measured 0.068-0.076 px on these seeds."""
+    w, h = 320, 240
+    i0, i1, gu, gv = rf.make_flow_pair(w, h, seed)
+    u, v = orc.solve_flow(orc.flow_params(w, h, 4), i0, i1, "f64")
+    e = _flow_epe(u, v, gu, gv, w, h)
+    assert e.mean() < 0.09, (seed, e.mean())
+    assert np.percentile(e, 99) < 0.3, (seed, np.percentile(e, 99))
+
+
+def _recon_errors(orc, seed, w=320, h=240, L=4, mutation=0):
+    d = rf.make_recon_pair(w, h, seed)
+    p = orc.recon_params(w, h, L, fix_coarse_k=1, tri_f64=1)
+    p.mutation = mutation
+    u, v, X, Y, Z = orc.solve_recon(p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"],
+                                    "f64")
+    e = _flow_epe(u, v, d["gt_u"], d["gt_v"], w, h)
+    zr = np.abs(_interior(Z, w, h) / _interior(d["gt_Z"], w, h) - 1.0)
+    return e, zr
+
+
+# Bounds for the joint solve on scenes of 8 planes (depths 2.2-6.7, rotation <= 3 deg,
+# translation 0.1, focal 250 px at 320x240). Plane boundaries and disocclusions leave a few
+# percent of pixels far off, so the bounds are on the median and on the fraction of pixels within
+# 0.25 px. Measured on seeds 0-2: median EPE 0.091-0.097 px, 84-93% within 0.25 px, median
+# depth error 1.2-1.4%.
+JOINT_MED_EPE, JOINT_FRAC, JOINT_MED_Z = 0.115, 0.82, 0.016
+
+
+def _joint_ok(e, zr):
+    return np.median(e) < JOINT_MED_EPE and np.mean(e < 0.25) > JOINT_FRAC and np.median(zr) < JOINT_MED_Z
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_joint_recovers_generator_flow_and_depth(orc, seed):
+    e, zr = _recon_errors(orc, seed)
+    assert _joint_ok(e, zr), (seed, np.median(e), np.mean(e < 0.25), np.median(zr))
+
+
+@pytest.mark.parametrize("name,seed", [("recon_no_uv_median", 0), ("recon_no_proj_upscale", 0),
+                                       ("recon_no_cleanup", 0), ("recon_uc_level0_k", 1)])
+def test_ground_truth_check_sees_planted_joint_mistakes(orc, name, seed):
+    e, zr = _recon_errors(orc, seed, mutation=orc.MUT[name])
+    assert not _joint_ok(e, zr), name
+
+
+@pytest.mark.parametrize("name", ["flow_no_median", "flow_no_dual_swap", "flow_warp_zero_flow",
+                                  "flow_no_uhat_swap"])
+def test_ground_truth_check_sees_planted_tvl1_mistakes(orc, name):
+    w, h = 320, 240
+    i0, i1, gu, gv = rf.make_flow_pair(w, h, 0)
+    p = orc.flow_params(w, h, 4)
+    p.mutation = orc.MUT[name]
+    u, v = orc.solve_flow(p, i0, i1, "f64")
+    assert _flow_epe(u, v, gu, gv, w, h).mean() > 0.09, name
+
+
+# ------------------------------------------------------------------ camera set-up
+@pytest.mark.parametrize("fix", [0, 1])
+def test_camera_pyramid_and_projection_matrices(orc, fix):
+    rng = np.random.default_rng(7)
+    K = np.array([[512.3, 0.7, 317.9], [0.0, 498.1, 241.3], [0.0, 0.0, 1.0]])
+    ang = 0.04
+    a = rng.standard_normal(3)
+    a /= np.linalg.norm(a)
+    Ax = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    R = np.eye(3) + np.sin(ang) * Ax + (1 - np.cos(ang)) * Ax @ Ax
+    t = rng.standard_normal(3) * 0.1
+    L, scale = 5, 1.7
+    Kl, Pl = orc.camera_pyramid(K, R, t, L, scale, fix)
+    exp = K.copy()
+    fs = float(np.float32(scale))
+    for lv in range(L):
+        if lv > 0:  # Reconflow.cu:98-110: a zero matrix with fx, cx, fy, cy divided by fScale
+            prev = exp
+            exp = np.zeros((3, 3))
+            exp[0, 0], exp[0, 2] = prev[0, 0] / fs, prev[0, 2] / fs
+            exp[1, 1], exp[1, 2] = prev[1, 1] / fs, prev[1, 2] / fs
+            if fix:
+                exp[2, 2] = 1.0
+        np.testing.assert_array_equal(Kl[lv], exp)
+        np.testing.assert_allclose(Pl[lv], exp @ np.hstack([R, t[:, None]]), rtol=1e-14, atol=1e-12)
+    if not fix:  # the reference's coarse cameras lose the homogeneous row
+        assert np.all(Pl[1:, 2, :] == 0)
+
+
+# ------------------------------------------------------------------ prior-flow data step
+W, H, S = 37, 23, 64
+
+
+def _put(a2d):
+    f = np.zeros((H + 1) * S)
+    f.reshape(H + 1, S)[:H, :W] = a2d
+    return f
+
+
+def _get(flat):
+    return flat.reshape(-1, S)[:H, :W]
+
+
+def _fn_inputs(rng, zero_grad=False, zero_dual=False):
+    g = lambda sc: _put(rng.standard_normal((H, W)) * sc)  # noqa: E731
+    u, v, du, dv, up, vp, ufn, vfn, sku, skv = (g(1.0) for _ in range(10))
+    p = [np.zeros((H + 1) * S) if zero_dual else g(0.2) for _ in range(4)]
+    I = [np.zeros((H + 1) * S) if zero_grad else g(sc) for sc in (0.1, 0.1)] + [g(0.05)]
+    return u, v, du, dv, up, vp, ufn, vfn, sku, skv, p, I
+
+
+def test_prior_data_step_with_zero_prior_weight_is_the_plain_step(orc):
+    rng = np.random.default_rng(21)
+    u, v, du, dv, up, vp, ufn, vfn, sku, skv, p, I = _fn_inputs(rng)
+    a1, a2, b1, b2, c1, c2 = (np.zeros((H + 1) * S) for _ in range(6))
+    orc.call("recon_data_l1", "f64", u, v, du, dv, up, vp, sku, skv, *p, *I, W, H, S, 50.0, 33.3, 60.0, a1, a2)
+    orc.call("recon_data_l1_fn", "f64", u, v, du, dv, up, vp, ufn, vfn, sku, skv, *p, *I, W, H, S, 50.0, 33.3,
+             60.0, 0.0, None, b1, b2)
+    # sparse form with an all-zero mask: the prior weight is alphaFn * mask = 0 everywhere
+    orc.call("recon_data_l1_fn", "f64", u, v, du, dv, up, vp, ufn, vfn, sku, skv, *p, *I, W, H, S, 50.0, 33.3,
+             60.0, 100.0, np.zeros((H + 1) * S), c1, c2)
+    for x, y in ((b1, a1), (b2, a2), (c1, a1), (c2, a2)):
+        np.testing.assert_array_equal(_get(x), _get(y))
+    # and a mask of ones is the dense form
+    d1, d2, e1, e2 = (np.zeros((H + 1) * S) for _ in range(4))
+    orc.call("recon_data_l1_fn", "f64", u, v, du, dv, up, vp, ufn, vfn, sku, skv, *p, *I, W, H, S, 50.0, 33.3,
+             60.0, 100.0, None, d1, d2)
+    orc.call("recon_data_l1_fn", "f64", u, v, du, dv, up, vp, ufn, vfn, sku, skv, *p, *I, W, H, S, 50.0, 33.3,
+             60.0, 100.0, _put(np.ones((H, W))), e1, e2)
+    np.testing.assert_array_equal(_get(d1), _get(e1))
+    assert not np.allclose(_get(d1), _get(a1))
+
+
+def test_prior_data_step_blend_and_threshold_closed_form(orc):
+    """Zero image gradient and zero duals: the step returns the minimiser of
+    aFn|d - (ufn - u)|^2 + aTv|d - uhat|^2 + aP|d - (uproj - u - sk)|^2. With a gradient it is the
+    textbook L1 threshold (weight lambda / A, A = aTv + aP + aFn) of that blend plus div(p) / A."""
+    rng = np.random.default_rng(22)
+    aT, aP, aF, lam = (np.float32(x).item() for x in (33.3, 60.0, 100.0, 50.0))
+    A = aT + aP + aF
+    for zero in (True, False):
+        u, v, du, dv, up, vp, ufn, vfn, sku, skv, p, I = _fn_inputs(rng, zero_grad=zero, zero_dual=zero)
+        o1, o2 = np.zeros((H + 1) * S), np.zeros((H + 1) * S)
+        orc.call("recon_data_l1_fn", "f64", u, v, du, dv, up, vp, ufn, vfn, sku, skv, *p, *I, W, H, S, lam, aT, aP,
+                 aF, None, o1, o2)
+        G = lambda f: _get(f)  # noqa: E731
+        bu = (aF * (G(ufn) - G(u)) + aT * G(du) + aP * (G(up) - G(u) - G(sku))) / A
+        bv = (aF * (G(vfn) - G(v)) + aT * G(dv) + aP * (G(vp) - G(v) - G(skv))) / A
+        if zero:
+            np.testing.assert_allclose(G(o1), bu, rtol=1e-12, atol=1e-14)
+            np.testing.assert_allclose(G(o2), bv, rtol=1e-12, atol=1e-14)
+            continue
+        gx, gy, it = (G(x) for x in I)
+        g2 = gx * gx + gy * gy
+        rho = gx * bu + gy * bv + it
+        lt = lam / A
+        step_u = np.where(rho < -lt * g2, lt * gx, np.where(rho > lt * g2, -lt * gx, -rho * gx / g2))
+        step_v = np.where(rho < -lt * g2, lt * gy, np.where(rho > lt * g2, -lt * gy, -rho * gy / g2))
+
+        def div(q1, q2):  # -adjoint of the forward gradient: backward differences, zero outside
+            a, b = G(q1), G(q2)
+            return a - np.pad(a, ((0, 0), (1, 0)))[:, :-1] + b - np.pad(b, ((1, 0), (0, 0)))[:-1]
+
+        np.testing.assert_allclose(G(o1), bu + step_u + div(p[0], p[1]) / A, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(G(o2), bv + step_v + div(p[2], p[3]) / A, rtol=1e-10, atol=1e-12)
+
+
+def test_add_is_elementwise_sum(orc):
+    rng = np.random.default_rng(23)
+    for kind, dt in (("f64", np.float64), ("f32", np.float32)):
+        a, b = rng.standard_normal(1000).astype(dt), rng.standard_normal(1000).astype(dt)
+        out = np.full(1000, 7.0, dt)
+        orc.call("add", kind, a, b, 731, out)
+        np.testing.assert_array_equal(out[:731], a[:731] + b[:731])
+        assert np.all(out[731:] == 7.0)  # count is an element count: nothing beyond it
+
+
+# ------------------------------------------------------------------ schedule composition
+def _bufs(n, k):
+    return [np.zeros(n) for _ in range(k)]
+
+
+def compose_flow(orc, p, i0, i1):
+    """CudaFlow::_solveOpticalFlow for METHOD_TVL1 composed from the per-kernel functions (FP64),
+    on level-0-sized flat buffers with a guard row, as the reference allocates them."""
+    pW, pH, pS = orc.pyramid(p)
+    L, s0, h0 = p.n_levels, pS[0], pH[0]
+    n = (h0 + 1) * s0
+    pI0, pI1 = _bufs(n, L), _bufs(n, L)
+    for img, dst in ((i0, pI0[0]), (i1, pI1[0])):  # Cv32fToGray: the w x h pixels
+        dst.reshape(h0 + 1, s0)[:h0, :p.width] = img[:, :p.width]
+    for lv in range(1, L):  # Cudaflow.cu:417-424
+        for py in (pI0, pI1):
+            orc.call("downscale", "f64", py[lv - 1], pW[lv - 1], pH[lv - 1], pS[lv - 1], pW[lv], pH[lv], pS[lv],
+                     py[lv])
+    i1warp, Ix, Iy, Iz = _bufs(n, 4)
+    du, dv, dus, dvs, dumed, dvmed, dumeds, dvmeds = _bufs(n, 8)
+    pu1, pu2, pv1, pv2, pu1s, pu2s, pv1s, pv2s = _bufs(n, 8)
+    u, v, us, vs = _bufs(n, 4)  # zeroed at :467-468
+    for level in range(L - 1, -1, -1):  # :470
+        w, h, s = pW[level], pH[level], pS[level]
+        for _ in range(p.n_warp_iters):  # :471
+            for b in (du, dv, dus, dvs, dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2):  # :474-487
+                b[:] = 0
+            orc.call("warp", "f64", pI1[level], w, h, s, u, v, i1warp)  # :490
+            orc.call("derivatives", "f64", pI0[level], i1warp, w, h, s, Ix, Iy, Iz)  # :501
+            for _ in range(p.n_solver_iters):  # :510
+                orc.call("data_l1", "f64", dumed, dvmed, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, p.lambda_, p.theta,
+                         dumeds, dvmeds)  # :541-547
+                dumed, dumeds, dvmed, dvmeds = dumeds, dumed, dvmeds, dvmed  # :548-549
+                orc.call("dual_tv", "f64", dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, p.tau, p.theta, pu1s, pu2s,
+                         pv1s, pv2s)  # :563-567
+                pu1, pu1s, pu2, pu2s = pu1s, pu1, pu2s, pu2  # :568-571
+                pv1, pv1s, pv2, pv2s = pv1s, pv1, pv2s, pv2
+            orc.call("median5", "f64", dumed, dvmed, w, h, s, dumeds, dvmeds)  # :580-581
+            dumed, dumeds, dvmed, dvmeds = dumeds, dumed, dvmeds, dvmed  # :582-583
+            orc.call("add", "f64", u, dumed, h * s, u)  # :586-587
+            orc.call("add", "f64", v, dvmed, h * s, v)
+        if level > 0:  # :600-612
+            W, Hh, Ss = pW[level - 1], pH[level - 1], pS[level - 1]
+            orc.call("upscale", "f64", u, w, h, s, W, Hh, Ss, p.scale, us)
+            orc.call("upscale", "f64", v, w, h, s, W, Hh, Ss, p.scale, vs)
+            u, us, v, vs = us, u, vs, v
+    return [a[:h0 * s0].reshape(h0, s0).astype(np.float32) for a in (u, v)]
+
+
+def compose_recon(orc, p, K, R1, t1, i0, i1, ufn=None, vfn=None, mask=None):
+    """ReconFlow::solveReconFlow composed from the per-kernel functions (FP64, camera 0 at the
+    origin, the same K for both cameras): METHODR_TVL1_MS, or with a prior flow (ufn, vfn) the
+    dense (17) or sparse (20, with mask) prior-flow data step. p.tri_f64 selects the FP64
+    triangulation step, p.fix_coarse_k the intrinsics option."""
+    pW, pH, pS = orc.pyramid(p)
+    L, s0, h0 = p.n_levels, pS[0], pH[0]
+    n = (h0 + 1) * s0
+    pI0, pI1 = _bufs(n, L), _bufs(n, L)
+    for img, dst in ((i0, pI0[0]), (i1, pI1[0])):
+        dst.reshape(h0 + 1, s0)[:h0, :p.width] = img[:, :p.width]
+    for lv in range(1, L):  # Reconflow.cu:173-184
+        for py in (pI0, pI1):
+            orc.call("downscale", "f64", py[lv - 1], pW[lv - 1], pH[lv - 1], pS[lv - 1], pW[lv], pH[lv], pS[lv],
+                     py[lv])
+    Kl, P0 = orc.camera_pyramid(K, np.eye(3), np.zeros(3), L, p.scale, p.fix_coarse_k)
+    _, P1 = orc.camera_pyramid(K, R1, t1, L, p.scale, p.fix_coarse_k)
+    fn = p.method in (17, 20) and ufn is not None
+    if fn:
+        ufn0, vfn0, m0, ufn_l, vfn_l, m_l = _bufs(n, 6)
+        for a, dst in ((ufn, ufn0), (vfn, vfn0), (np.ones((h0, s0)) if mask is None else mask, m0)):
+            dst[:h0 * s0] = np.asarray(a, np.float64).ravel()
+    i1warp, Ix, Iy, Iz = _bufs(n, 4)
+    dumed, dvmed, dumeds, dvmeds = _bufs(n, 4)
+    pu1, pu2, pv1, pv2, pu1s, pu2s, pv1s, pv2s = _bufs(n, 8)
+    u, v, us, vs = _bufs(n, 4)  # :209-210
+    X, Y, Z, Xs, Ys, Zs, Xm, Ym, Zm = _bufs(n, 9)  # :216-218
+    uproj, vproj, uc, vc, sku, skv, skus, skvs = _bufs(n, 8)  # :211-214, 219-220
+    lam, tau, aT, aP = p.lambda_, p.tau, p.alpha_tv, p.alpha_proj
+    thetaTv = float(np.float32(1.0) / np.float32(p.alpha_tv))  # :32
+    R1f, t1f = np.ascontiguousarray(R1, np.float64).ravel(), np.ascontiguousarray(t1, np.float64).ravel()
+    for level in range(L - 1, -1, -1):  # :222
+        w, h, s = pW[level], pH[level], pS[level]
+        Pm, Qm = P0[level].ravel().copy(), P1[level].ravel().copy()  # :225-228
+        if fn:  # :232-241
+            sc = float(np.float32(pW[level]) / np.float32(pW[0]))
+            orc.call("downscale_scaled", "f64", ufn0, pW[0], pH[0], pS[0], w, h, s, sc, ufn_l)
+            orc.call("downscale_scaled", "f64", vfn0, pW[0], pH[0], pS[0], w, h, s, sc, vfn_l)
+            orc.call("downscale", "f64", m0, pW[0], pH[0], pS[0], w, h, s, m_l)
+        for _ in range(p.n_warp_iters):  # :242
+            for b in (dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2):  # :245-258
+                b[:] = 0
+            orc.call("warp", "f64", pI1[level], w, h, s, u, v, i1warp)  # :264
+            orc.call("derivatives", "f64", pI0[level], i1warp, w, h, s, Ix, Iy, Iz)  # :275
+            for _ in range(p.n_solver_iters):  # :284
+                if fn:  # :302-312 (17), :317-327 (20)
+                    orc.call("recon_data_l1_fn", "f64", u, v, dumed, dvmed, uproj, vproj, ufn_l, vfn_l, sku, skv, pu1,
+                             pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lam, aT, aP, p.alpha_fn,
+                             m_l if p.method == 20 else None, dumeds, dvmeds)
+                else:
+                    orc.call("recon_data_l1", "f64", u, v, dumed, dvmed, uproj, vproj, sku, skv, pu1, pu2, pv1, pv2,
+                             Ix, Iy, Iz, w, h, s, lam, aT, aP, dumeds, dvmeds)  # :288-297
+                dumed, dumeds, dvmed, dvmeds = dumeds, dumed, dvmeds, dvmed  # :298
+                orc.call("dual_tv", "f64", dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, thetaTv, pu1s, pu2s, pv1s,
+                         pv2s)  # :346-350
+                pu1, pu1s, pu2, pu2s, pv1, pv1s, pv2, pv2s = pu1s, pu1, pu2s, pu2, pv1s, pv1, pv2s, pv2  # :351
+                orc.call("uvproj", "f64", u, v, dumed, dvmed, uc, vc, sku, skv, w, h, s, p.lambdaf, aP, uproj,
+                         vproj)  # :355-360
+                orc.call("solve3d_fma" if p.tri_f64 else "solve3d", "f64", uproj, vproj, X, Y, Z, Pm, Qm, p.lambdaf,
+                         p.lambdams, w, h, s, Xs, Ys, Zs)  # :374-378
+                X, Xs, Y, Ys, Z, Zs = Xs, X, Ys, Y, Zs, Z  # :379-381
+                orc.call("median3d", "f64", X, Y, Z, w, h, s, Xm, Ym, Zm)  # :383-385
+                orc.call("cleanup3d", "f64", X, Y, Z, Xm, Ym, Zm, w, h, s, Xs, Ys, Zs)  # :386-388
+                X, Xs, Y, Ys, Z, Zs = Xs, X, Ys, Y, Zs, Z  # :389-391
+                orc.call("solve_uc", "f64", X, Y, Z, R1f, t1f, Kl[level].ravel().copy(), w, h, s, uc, vc)  # :397
+                orc.call("update_sk", "f64", sku, skv, u, v, dumed, dvmed, uproj, vproj, w, h, s, skus,
+                         skvs)  # :402-404
+                sku, skus, skv, skvs = skus, sku, skvs, skv  # :405
+            orc.call("median5", "f64", dumed, dvmed, w, h, s, dumeds, dvmeds)  # :410-412
+            dumed, dumeds, dvmed, dvmeds = dumeds, dumed, dvmeds, dvmed
+            orc.call("add", "f64", u, dumed, h * s, u)  # :415-416
+            orc.call("add", "f64", v, dvmed, h * s, v)
+            orc.call("median5", "f64", u, v, w, h, s, dumeds, dvmeds)  # :418-420
+            u, dumeds, v, dvmeds = dumeds, u, dvmeds, v
+        if level > 0:  # :424-458: u, v, uc, vc, uproj, vproj, sku, skv upscaled; X, Y, Z kept
+            W, Hh, Ss = pW[level - 1], pH[level - 1], pS[level - 1]
+            pairs = []
+            for a, b in ((u, v), (uc, vc), (uproj, vproj), (sku, skv)):
+                orc.call("upscale", "f64", a, w, h, s, W, Hh, Ss, p.scale, us)
+                orc.call("upscale", "f64", b, w, h, s, W, Hh, Ss, p.scale, vs)
+                pairs.append((us, vs))
+                us, vs = a, b
+            (u, v), (uc, vc), (uproj, vproj), (sku, skv) = pairs
+    return [a[:h0 * s0].reshape(h0, s0).astype(np.float32) for a in (u, v, X, Y, Z)]
+
+
+FW, FH, FL = 40, 30, 2  # stride 64 (padding), two levels (20x15, stride 32)
+
+
+def _flow_case(orc, mutation=0):
+    i0, i1, _, _ = rf.make_flow_pair(FW, FH, 11)
+    p = orc.flow_params(FW, FH, FL, n_warp=2, n_iter=2)
+    p.mutation = mutation
+    return p, i0, i1
+
+
+def _recon_case(orc, mutation=0):
+    d = rf.make_recon_pair(FW, FH, 12)
+    p = orc.recon_params(FW, FH, FL, n_warp=2, n_iter=2)
+    p.mutation = mutation
+    return p, d
+
+
+def _solve_recon(orc, p, d):
+    return orc.solve_recon(p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f64")
+
+
+def _eq(a, b):
+    return all(np.array_equal(x[:, :FW], y[:, :FW], equal_nan=True) for x, y in zip(a, b))
+
+
+def test_tvl1_schedule_equals_kernel_composition(orc):
+    p, i0, i1 = _flow_case(orc)
+    assert _eq(orc.solve_flow(p, i0, i1, "f64"), compose_flow(orc, p, i0, i1))
+
+
+@pytest.mark.parametrize("tri,fix", [(0, 0), (1, 0), (1, 1)])
+def test_joint_schedule_equals_kernel_composition(orc, tri, fix):
+    p, d = _recon_case(orc)
+    p.tri_f64, p.fix_coarse_k = tri, fix
+    got = _solve_recon(orc, p, d)
+    assert all(np.isfinite(g[:, :FW]).all() for g in got)
+    assert _eq(got, compose_recon(orc, p, d["K"], d["R"], d["t"], d["i0"], d["i1"]))
+
+
+@pytest.mark.parametrize("method", [17, 20])
+def test_prior_flow_schedule_equals_kernel_composition(orc, method):
+    p, d = _recon_case(orc)
+    p.method = method
+    mask = (np.random.default_rng(5).random(d["i0"].shape) > 0.6).astype(np.float32) if method == 20 else None
+    got = orc.solve_recon(p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f64",
+                          ufn=d["gt_u"], vfn=d["gt_v"], mask=mask)
+    assert _eq(got, compose_recon(orc, p, d["K"], d["R"], d["t"], d["i0"], d["i1"], d["gt_u"], d["gt_v"], mask))
+    assert not _eq(got, _solve_recon(orc, _recon_case(orc)[0], d))
+
+
+@pytest.mark.parametrize("name", sorted(k for k in __import__("oracle.oracle", fromlist=["MUT"]).MUT))
+def test_composition_check_sees_every_planted_mistake(orc, name):
+    """Each planted mistake changes the solve on the composition check's case (two levels, two
+    warps, two iterations), so the check above would fail for it."""
+    if name.startswith("flow"):
+        p, i0, i1 = _flow_case(orc, orc.MUT[name])
+        assert not _eq(orc.solve_flow(p, i0, i1, "f64"), compose_flow(orc, p, i0, i1)), name
+    else:
+        p, d = _recon_case(orc, orc.MUT[name])
+        assert not _eq(_solve_recon(orc, p, d), compose_recon(orc, p, d["K"], d["R"], d["t"], d["i0"], d["i1"])), name
