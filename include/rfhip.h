@@ -83,7 +83,9 @@ typedef struct rf_params {
     int method;            /* RF_METHOD_* / RF_METHODR_* */
     float lambda, lambdagrad, theta, tau;
     int n_warp_iters, n_solver_iters;
-    int with_visualization;
+    int with_visualization; /* 1: each solve ends with the FlowToHSV colour map of the flow (uvrgb),
+                               as CudaFlow does with withVisualization (Cudaflow.cu:614-618) and
+                               ReconFlow::solveReconFlow always does (Reconflow.cu:460) */
     /* ReconFlow (joint) parameters; theta is replaced by 1/alpha_tv (Reconflow.cu:32) */
     float lambdaf, lambdams, alpha_tv, alpha_proj, alpha_fn;
     /* launch geometry of the reference constructor (Cudaflow.cu:10-14) */
@@ -130,7 +132,7 @@ typedef struct rf_params {
 
 /* Fills p with the reference defaults (Cudaflow.cu:99-101, Reconflow.cu:27-30) for
  * width x height: 1 warp, 100 iterations, lambda 100, theta 0.33, tau 0.25, alignment 32,
- * plain FP32 path without a graph. */
+ * visualization on, plain FP32 path without a graph. */
 void rf_params_default(rf_params* p, int width, int height);
 
 typedef struct rf_context rf_context;
@@ -210,7 +212,8 @@ int rf_synchronize(rf_context* ctx);
 
 /* ---- outputs ---- */
 /* CudaFlow::copyOpticalFlowToHost(u, v[, uvrgb]) (Cudaflow.cu:313-324): stride x height
- * floats each; uvrgb (3 floats per pixel, BGR) may be NULL. Synchronous. */
+ * floats each; uvrgb (3 floats per pixel, BGR) may be NULL. A non-NULL uvrgb after a solve that
+ * ran without visualization is RF_ERR_STATE (no colour map was computed). Synchronous. */
 int rf_copy_flow(rf_context* ctx, float* u, float* v, float* uvrgb);
 /* ReconFlow::copy3dToHost(X, Y, Z) (Reconflow.cu:65-70): camera-0 frame points. */
 int rf_copy_3d(rf_context* ctx, float* X, float* Y, float* Z);

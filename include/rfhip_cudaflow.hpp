@@ -322,6 +322,9 @@ class ReconFlow : public CudaFlow {
     void setInputType(int cvType) { inputType_ = cvType; }
     void setTriangulationFp64(bool on) { triF64_ = on; }
     void setFixCoarseIntrinsics(bool on) { fixK_ = on; }
+    // solveReconFlow ends with FlowToHSV (Reconflow.cu:460), so copyOpticalFlowToHost(u, v, uvrgb)
+    // returns the colour map; false skips that kernel (uvrgb is then unavailable), e.g. for timing.
+    void setVisualization(bool on) { visualization_ = on; }
 
     int setCameraMatrices(const Mat&) { return 0; }  // Reconflow.cu:72-76 is a no-op
     // K0, K1: 3x3 row-major CV_64F
@@ -411,7 +414,7 @@ class ReconFlow : public CudaFlow {
         p.alpha_proj = alphaProj_;
         p.alpha_fn = alphaFn_;
         p.stride_alignment = StrideAlignment_;
-        p.with_visualization = 0;
+        p.with_visualization = visualization_ ? 1 : 0;
         p.tri_f64 = triF64_ ? 1 : 0;
         p.fix_coarse_k = fixK_ ? 1 : 0;
         applyOptions(p);
@@ -426,7 +429,7 @@ class ReconFlow : public CudaFlow {
         return prior ? rf_initialize_large_displacement(ctx_) : 0;
     }
     int inputType_ = RF_INPUT_8UC3;
-    bool triF64_ = false, fixK_ = false;
+    bool triF64_ = false, fixK_ = false, visualization_ = true;
 };
 
 }  // namespace rfhip

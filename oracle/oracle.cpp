@@ -740,6 +740,92 @@ void update_sk(const T* sku0, const T* skv0, const T* u0, const T* v0, const T* 
 }
 
 // ---------------------------------------------------------------------------------
+// Precise-projection mode (Params::proj_f64; this implementation's arithmetic, the reference
+// evaluates these steps in FP32). The projection blend (solveUVProj.cu:19-20), the Sk update
+// (solveUcSk.cu:93-94) and the joint data step's blend (solveReconDataL1.cu:35-36) add and subtract
+// quantities of the size of the flow (u, uproj, sk: 20 px and more at 640x480) to form differences
+// of ~1e-3 px, so each FP32 evaluation loses ~1e-6 px, and the Sk sum carries the loss through the
+// 1250 iterations of a frame. The selected steps are evaluated in binary64 from the stored FP32
+// values (sums of the launch constants in binary64 too, divisions as products with their binary64
+// reciprocals) and rounded once to the field type.
+enum { PREC_PROJ = 1, PREC_BLEND = 2, PREC_SK = 4, PREC_DATA = 8 };
+
+template <typename T>
+void uvproj_wide(const T* u0, const T* v0, const T* du0, const T* dv0, const T* uc0, const T* vc0, const T* sku0,
+                 const T* skv0, int w, int h, int s, T lambdaf, T alphaProj, T* uproj1, T* vproj1) {
+    const double lf = lambdaf, aP = alphaProj, r = 1.0 / (lf + aP);
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            uproj1[p] = (T)((lf * (double)uc0[p] + aP * (((double)u0[p] + (double)du0[p]) + (double)sku0[p])) * r);
+            vproj1[p] = (T)((lf * (double)vc0[p] + aP * (((double)v0[p] + (double)dv0[p]) + (double)skv0[p])) * r);
+        }
+}
+
+template <typename T>
+void update_sk_wide(const T* sku0, const T* skv0, const T* u0, const T* v0, const T* du0, const T* dv0,
+                    const T* uproj0, const T* vproj0, int w, int h, int s, T* sku1, T* skv1) {
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            sku1[p] = (T)((((double)sku0[p] + (double)u0[p]) + (double)du0[p]) - (double)uproj0[p]);
+            skv1[p] = (T)((((double)skv0[p] + (double)v0[p]) + (double)dv0[p]) - (double)vproj0[p]);
+        }
+}
+
+// The joint data step (solveReconDataL1.cu:4-102) with its blend in binary64 (PREC_BLEND: the
+// threshold and the divergence then in T, as recon_data_l1) or the whole step in binary64
+// (PREC_DATA).
+template <typename T, typename C>
+static void recon_data_px(C dix, C diy, C dit, C dusub, C dvsub, C lambda, C A, C divu, C divv, T& o1, T& o2) {
+    C rho = (dix * dusub + diy * dvsub) + dit;
+    C g2 = dix * dix + diy * diy;
+    C upper = (lambda * g2) / A;
+    C lower = ((-lambda) * g2) / A;
+    C du = std::numeric_limits<C>::quiet_NaN(), dv = du;
+    if ((rho <= upper) && (rho >= lower)) {
+        if (g2 != C(0)) {
+            du = dusub - (rho * dix) / g2;
+            dv = dvsub - (rho * diy) / g2;
+        } else {
+            du = dusub;
+            dv = dvsub;
+        }
+    } else if (rho < lower) {
+        du = dusub + (lambda * dix) / A;
+        dv = dvsub + (lambda * diy) / A;
+    } else if (rho > upper) {
+        du = dusub - (lambda * dix) / A;
+        dv = dvsub - (lambda * diy) / A;
+    }
+    o1 = (T)(du + divu / A);
+    o2 = (T)(dv + divv / A);
+}
+
+template <typename T>
+void recon_data_l1_wide(const T* u0, const T* v0, const T* duhat0, const T* dvhat0, const T* uproj0,
+                        const T* vproj0, const T* sku0, const T* skv0, const T* pu1, const T* pu2, const T* pv1,
+                        const T* pv2, const T* Ix, const T* Iy, const T* It, int w, int h, int s, T lambda,
+                        T alphaTv, T alphaProj, bool whole, T* duhat1, T* dvhat1) {
+    const double aT = alphaTv, aP = alphaProj, Ad = aT + aP, rA = 1.0 / Ad;
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            double bu = (aT * (double)duhat0[p] + aP * (((double)uproj0[p] - (double)u0[p]) - (double)sku0[p])) * rA;
+            double bv = (aT * (double)dvhat0[p] + aP * (((double)vproj0[p] - (double)v0[p]) - (double)skv0[p])) * rA;
+            if (whole)
+                recon_data_px<T, double>(Ix[p], Iy[p], It[p], bu, bv, lambda, Ad, div_p(pu1, pu2, ix, iy, s),
+                                         div_p(pv1, pv2, ix, iy, s), duhat1[p], dvhat1[p]);
+            else
+                recon_data_px<T, T>(Ix[p], Iy[p], It[p], (T)bu, (T)bv, lambda, alphaTv + alphaProj,
+                                    div_p(pu1, pu2, ix, iy, s), div_p(pv1, pv2, ix, iy, s), duhat1[p], dvhat1[p]);
+        }
+}
+
+// ---------------------------------------------------------------------------------
 // f2 / f3 / f4: display, post-filter and alternate data terms.
 
 // ComputeColorKernel (flowToHSV.cu:194-302): Middlebury colour wheel, BGR output. The
@@ -1056,6 +1142,33 @@ struct Params {
     //               carries the FP32 solve's distance from the FP64 one.
     int mutation;
     int round_mask;
+    //   f32_steps:  in the FP64 joint solve, runs the steps whose STEP_* bit is set in FP32
+    //               arithmetic (their fields converted to float and back), to find which step's
+    //               arithmetic carries the FP32 solve's distance from the FP64 one.
+    int f32_steps;
+    int proj_f64;  // PREC_* bits: the precise-projection mode (for FP32 fields)
+};
+
+// Steps of Params::f32_steps (joint solve).
+enum {
+    STEP_WARP = 1, STEP_DERIV = 2, STEP_DATA = 4, STEP_DUAL = 8, STEP_PROJ = 16, STEP_TRI = 32,
+    STEP_CLEAN = 64, STEP_UC = 128, STEP_SK = 256, STEP_MED = 512, STEP_UPS = 1024,
+};
+
+// Float copies of a step's fields for Params::f32_steps: in(slot, src) converts a field to float
+// (keeping what the step does not overwrite), out(slot, dst) converts a step output back.
+template <typename T>
+struct F32Fields {
+    long m = 0;
+    std::vector<std::vector<float>> b{16};
+    float* in(int slot, const T* src) {
+        b[slot].resize(m);
+        for (long i = 0; i < m; ++i) b[slot][i] = (float)src[i];
+        return b[slot].data();
+    }
+    void out(int slot, T* dst) {
+        for (long i = 0; i < m; ++i) dst[i] = (T)b[slot][i];
+    }
 };
 
 // Planted schedule mistakes (Params::mutation). TV-L1 schedule (Cudaflow.cu:403-621):
@@ -1417,6 +1530,7 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
     k_pyramid(K1, P.n_levels, P.scale, P.fix_coarse_k, pK1);
 
     const int mut = P.mutation;
+    const int prec = sizeof(T) == sizeof(float) ? P.proj_f64 : 0;  // binary64 already in the FP64 solve
     gray_input(P, i0, pI0[0], s0);
     gray_input(P, i1, pI1[0], s0);
     round_group<T>(P, RND_IMG, n, {pI0[0], pI1[0]});
@@ -1450,30 +1564,88 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                 for (T* b : {dumed, dvmed, dumeds, dvmeds, pu1, pu2, pv1, pv2}) zero(b, n);
             if (mut == MUT_RECON_SK_RESET)
                 for (T* b : {sku, skv}) zero(b, n);
-            warp(pI1[level], w, h, s, u, v, i1warp);
+            F32Fields<T> q;
+            q.m = (long)(h + 1) * s;
+            auto a32 = [&](int bit) { return sizeof(T) == sizeof(double) && (P.f32_steps & bit); };
+            if (a32(STEP_WARP)) {
+                warp<float>(q.in(0, pI1[level]), w, h, s, q.in(1, u), q.in(2, v), q.in(3, i1warp));
+                q.out(3, i1warp);
+            } else {
+                warp(pI1[level], w, h, s, u, v, i1warp);
+            }
             round_group<T>(P, RND_IMG, n, {i1warp});
-            derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            if (a32(STEP_DERIV)) {
+                derivatives<float>(q.in(0, pI0[level]), q.in(1, i1warp), w, h, s, q.in(2, Ix), q.in(3, Iy),
+                                   q.in(4, Iz));
+                q.out(2, Ix);
+                q.out(3, Iy);
+                q.out(4, Iz);
+            } else {
+                derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
+            }
             round_group<T>(P, RND_DER, n, {Ix, Iy, Iz});
             for (int it = 0; it < P.n_solver_iters; ++it) {
                 if (fn)
                     recon_data_l1_fn(u, v, dumed, dvmed, uproj, vproj, ufn_l, vfn_l, sku, skv, pu1, pu2, pv1,
                                      pv2, Ix, Iy, Iz, w, h, s, lambda, alphaTv, alphaProj, alphaFn,
                                      P.method == 20 ? mask_l : nullptr, dumeds, dvmeds);
+                else if (a32(STEP_DATA)) {
+                    recon_data_l1<float>(q.in(0, u), q.in(1, v), q.in(2, dumed), q.in(3, dvmed), q.in(4, uproj),
+                                         q.in(5, vproj), q.in(6, sku), q.in(7, skv), q.in(8, pu1), q.in(9, pu2),
+                                         q.in(10, pv1), q.in(11, pv2), q.in(12, Ix), q.in(13, Iy), q.in(14, Iz), w,
+                                         h, s, (float)lambda, (float)alphaTv, (float)alphaProj, q.in(0, dumeds),
+                                         q.in(1, dvmeds));
+                    q.out(0, dumeds);
+                    q.out(1, dvmeds);
+                } else if (prec & (PREC_BLEND | PREC_DATA))
+                    recon_data_l1_wide(u, v, dumed, dvmed, uproj, vproj, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz,
+                                       w, h, s, lambda, alphaTv, alphaProj, (prec & PREC_DATA) != 0, dumeds,
+                                       dvmeds);
                 else
                     recon_data_l1(u, v, dumed, dvmed, uproj, vproj, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz,
                                   w, h, s, lambda, alphaTv, alphaProj, dumeds, dvmeds);
                 std::swap(dumed, dumeds);
                 std::swap(dvmed, dvmeds);
                 round_group<T>(P, RND_UHAT, n, {dumed, dvmed});
+                if (a32(STEP_DUAL)) {
+                    dual_tv<float>(q.in(0, dumed), q.in(1, dvmed), q.in(2, pu1), q.in(3, pu2), q.in(4, pv1),
+                                   q.in(5, pv2), w, h, s, (float)tau, (float)thetaTv, q.in(6, pu1s), q.in(7, pu2s),
+                                   q.in(8, pv1s), q.in(9, pv2s));
+                    q.out(6, pu1s);
+                    q.out(7, pu2s);
+                    q.out(8, pv1s);
+                    q.out(9, pv2s);
+                } else
                 dual_tv(dumed, dvmed, pu1, pu2, pv1, pv2, w, h, s, tau, thetaTv, pu1s, pu2s, pv1s, pv2s);
                 std::swap(pu1, pu1s);
                 std::swap(pu2, pu2s);
                 std::swap(pv1, pv1s);
                 std::swap(pv2, pv2s);
                 round_group<T>(P, RND_DUAL, n, {pu1, pu2, pv1, pv2});
-                uvproj(u, v, dumed, dvmed, uc, vc, sku, skv, w, h, s, lambdaf, alphaProj, uproj, vproj);
+                if (a32(STEP_PROJ)) {
+                    uvproj<float>(q.in(0, u), q.in(1, v), q.in(2, dumed), q.in(3, dvmed), q.in(4, uc), q.in(5, vc),
+                                  q.in(6, sku), q.in(7, skv), w, h, s, (float)lambdaf, (float)alphaProj,
+                                  q.in(8, uproj), q.in(9, vproj));
+                    q.out(8, uproj);
+                    q.out(9, vproj);
+                } else if (prec & PREC_PROJ)
+                    uvproj_wide(u, v, dumed, dvmed, uc, vc, sku, skv, w, h, s, lambdaf, alphaProj, uproj, vproj);
+                else
+                    uvproj(u, v, dumed, dvmed, uc, vc, sku, skv, w, h, s, lambdaf, alphaProj, uproj, vproj);
                 round_group<T>(P, RND_PROJ, n, {uproj, vproj});
-                if (P.tri_f64)
+                if (a32(STEP_TRI)) {  // float fields; FP64 arithmetic inside with tri_f64, as on the GPU
+                    float *a = q.in(0, uproj), *b = q.in(1, vproj), *x0 = q.in(2, X), *y0 = q.in(3, Y),
+                          *z0 = q.in(4, Z), *x1 = q.in(5, Xs), *y1 = q.in(6, Ys), *z1 = q.in(7, Zs);
+                    if (P.tri_f64)
+                        solve3d_wide<float>(a, b, x0, y0, z0, Pm, Qm, (float)lambdaf, (float)lambdams, w, h, s, x1,
+                                            y1, z1);
+                    else
+                        solve3d<float>(a, b, x0, y0, z0, Pm, Qm, (float)lambdaf, (float)lambdams, w, h, s, x1, y1,
+                                       z1);
+                    q.out(5, Xs);
+                    q.out(6, Ys);
+                    q.out(7, Zs);
+                } else if (P.tri_f64)
                     solve3d_wide(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, Xs, Ys, Zs);
                 else
                     solve3d(uproj, vproj, X, Y, Z, Pm, Qm, lambdaf, lambdams, w, h, s, Xs, Ys, Zs);
@@ -1481,7 +1653,22 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                 std::swap(Y, Ys);
                 std::swap(Z, Zs);
                 round_group<T>(P, RND_XYZ, n, {X, Y, Z});
-                if (mut != MUT_RECON_NO_CLEANUP) {
+                if (mut != MUT_RECON_NO_CLEANUP && a32(STEP_CLEAN)) {
+                    float *x = q.in(0, X), *y = q.in(1, Y), *z = q.in(2, Z), *xm = q.in(3, Xm), *ym = q.in(4, Ym),
+                          *zm = q.in(5, Zm);
+                    median3d<float>(x, y, z, w, h, s, xm, ym, zm);
+                    cleanup3d<float>(x, y, z, xm, ym, zm, w, h, s, q.in(6, Xs), q.in(7, Ys), q.in(8, Zs));
+                    q.out(3, Xm);
+                    q.out(4, Ym);
+                    q.out(5, Zm);
+                    q.out(6, Xs);
+                    q.out(7, Ys);
+                    q.out(8, Zs);
+                    std::swap(X, Xs);
+                    std::swap(Y, Ys);
+                    std::swap(Z, Zs);
+                    round_group<T>(P, RND_XYZ, n, {X, Y, Z});
+                } else if (mut != MUT_RECON_NO_CLEANUP) {
                     median3d(X, Y, Z, w, h, s, Xm, Ym, Zm);
                     cleanup3d(X, Y, Z, Xm, Ym, Zm, w, h, s, Xs, Ys, Zs);
                     std::swap(X, Xs);
@@ -1489,20 +1676,48 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                     std::swap(Z, Zs);
                     round_group<T>(P, RND_XYZ, n, {X, Y, Z});
                 }
+                if (a32(STEP_UC)) {
+                    solve_uc<float>(q.in(0, X), q.in(1, Y), q.in(2, Z), R1, t1, Kuc, w, h, s, q.in(3, uc),
+                                    q.in(4, vc));
+                    q.out(3, uc);
+                    q.out(4, vc);
+                } else
                 solve_uc(X, Y, Z, R1, t1, Kuc, w, h, s, uc, vc);
                 round_group<T>(P, RND_UC, n, {uc, vc});
-                update_sk(sku, skv, u, v, dumed, dvmed, uproj, vproj, w, h, s, skus, skvs);
+                if (a32(STEP_SK)) {
+                    update_sk<float>(q.in(0, sku), q.in(1, skv), q.in(2, u), q.in(3, v), q.in(4, dumed),
+                                     q.in(5, dvmed), q.in(6, uproj), q.in(7, vproj), w, h, s, q.in(8, skus),
+                                     q.in(9, skvs));
+                    q.out(8, skus);
+                    q.out(9, skvs);
+                } else if (prec & PREC_SK)
+                    update_sk_wide(sku, skv, u, v, dumed, dvmed, uproj, vproj, w, h, s, skus, skvs);
+                else
+                    update_sk(sku, skv, u, v, dumed, dvmed, uproj, vproj, w, h, s, skus, skvs);
                 if (mut != MUT_RECON_NO_SK_SWAP) {
                     std::swap(sku, skus);
                     std::swap(skv, skvs);
                 }
                 round_group<T>(P, RND_SK, n, {sku, skv});
             }
+            if (a32(STEP_MED)) {
+                float *a = q.in(0, dumed), *b = q.in(1, dvmed), *c = q.in(2, dumeds), *d = q.in(3, dvmeds),
+                      *uu = q.in(4, u), *vv = q.in(5, v);
+                median5<float>(a, b, w, h, s, c, d);
+                add<float>(uu, c, (long)h * s, uu);
+                add<float>(vv, d, (long)h * s, vv);
+                q.out(2, dumeds);
+                q.out(3, dvmeds);
+                q.out(4, u);
+                q.out(5, v);
+            } else
             median5(dumed, dvmed, w, h, s, dumeds, dvmeds);
             std::swap(dumed, dumeds);
             std::swap(dvmed, dvmeds);
-            add(u, dumed, (long)h * s, u);
-            add(v, dvmed, (long)h * s, v);
+            if (!a32(STEP_MED)) {
+                add(u, dumed, (long)h * s, u);
+                add(v, dvmed, (long)h * s, v);
+            }
             if (mut != MUT_RECON_NO_UV_MEDIAN) {
                 median5(u, v, w, h, s, dumeds, dvmeds);
                 std::swap(u, dumeds);
@@ -1513,8 +1728,22 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
         if (level > 0) {
             const int W = pW[level - 1], H = pH[level - 1], S = pS[level - 1];
             T sc = P.scale;
+            if (sizeof(T) == sizeof(double) && (P.f32_steps & STEP_UPS)) {
+                // FP32 arithmetic for every upscale of this level change
+                auto up32 = [&](T* src, T* dst) {
+                    const long m = (long)(H + 1) * S, mc = (long)(h + 1) * s;
+                    std::vector<float> a(src, src + std::max(m, mc)), b(dst, dst + m);
+                    for (long i = 0; i < (long)a.size(); ++i) a[i] = (float)src[i];
+                    for (long i = 0; i < m; ++i) b[i] = (float)dst[i];
+                    upscale<float>(a.data(), w, h, s, W, H, S, (float)sc, b.data());
+                    for (long i = 0; i < m; ++i) dst[i] = (T)b[i];
+                };
+                up32(u, us);
+                up32(v, vs);
+            } else {
             upscale(u, w, h, s, W, H, S, sc, us);
             upscale(v, w, h, s, W, H, S, sc, vs);
+            }
             std::swap(u, us);
             std::swap(v, vs);
             if (mut != MUT_RECON_NO_UC_UPSCALE) {

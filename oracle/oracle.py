@@ -26,6 +26,7 @@ class Params(C.Structure):
         ("lambdams", C.c_float), ("alpha_tv", C.c_float), ("alpha_proj", C.c_float), ("alpha_fn", C.c_float),
         ("stride_alignment", C.c_int), ("input_type", C.c_int), ("fix_coarse_k", C.c_int),
         ("tri_f64", C.c_int), ("mutation", C.c_int), ("round_mask", C.c_int),
+        ("f32_steps", C.c_int), ("proj_f64", C.c_int),
     ]
 
 
@@ -38,6 +39,10 @@ MUT = dict(flow_no_warp_reset=1, flow_no_median=2, flow_no_dual_swap=3, flow_ups
            recon_uc_level0_k=19)
 # State groups of Params.round_mask (oracle.cpp RND_*): FP64 arithmetic with that group stored in FP32.
 RND = dict(img=1, der=2, uhat=4, dual=8, proj=16, xyz=32, uc=64, sk=128, uv=256)
+# Steps of Params.f32_steps (oracle.cpp STEP_*): the FP64 joint solve with that step in FP32 arithmetic.
+# Precise-projection steps of Params.proj_f64 (oracle.cpp PREC_*), evaluated in binary64 in the FP32 solve.
+PREC = dict(proj=1, blend=2, sk=4, data=8)
+STEP = dict(warp=1, deriv=2, data=4, dual=8, proj=16, tri=32, clean=64, uc=128, sk=256, med=512, ups=1024)
 
 
 def build() -> Path:

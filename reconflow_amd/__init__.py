@@ -231,9 +231,10 @@ class ReconFlow(CudaFlow):
     def _initializeR(self, width, height, channels, nLevels, scale, method, lambda_=100.0, lambdagrad=400.0,
                      lambdaf=100.0, lambdams=100.0, alphaTv=3.0, alphaProj=1.0, tau=0.25, nWarpIters=1,
                      nSolverIters=100, *, cvType=INPUT_8UC3, path=PATH_PLAIN, storage=STORAGE_F32,
-                     use_graph=False, withVisualization=False):
+                     use_graph=False, withVisualization=True):
         """ReconFlow::_initializeR (Reconflow.cu:27-63). The reference accepts only 8-bit RGB;
-        ``cvType`` extends it to the other CudaFlow input types."""
+        ``cvType`` extends it to the other CudaFlow input types. Every solve ends with the flow's
+        colour map, as solveReconFlow does (Reconflow.cu:460); ``withVisualization=False`` skips it."""
         p = RFParams()
         self._lib.rf_params_default(C.byref(p), width, height)
         p.channels, p.input_type, p.n_levels, p.scale, p.method = channels, cvType, nLevels, scale, method
@@ -380,7 +381,7 @@ class Batch:
 
 def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50.0, theta=0.3, tau=0.25,
                  alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1, lambdams=100.0, use_graph=True, tri_f64=0,
-                 storage=STORAGE_F32, joint_tile=0, flow_tile=0) -> RFParams:
+                 storage=STORAGE_F32, joint_tile=0, flow_tile=0, with_visualization=False) -> RFParams:
     """Parameters for a Batch: TV-L1 flow (method 0) or the joint flow/depth solve (method 14),
     float gray input, scale 2 pyramid."""
     lib = _native.lib()
@@ -392,6 +393,7 @@ def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50
     p.lambdaf, p.lambdams, p.alpha_tv, p.alpha_proj = lambdaf, lambdams, alpha_tv, alpha_proj
     p.path, p.use_graph, p.tri_f64, p.storage = PATH_FUSED, int(use_graph), tri_f64, storage
     p.joint_tile, p.flow_tile = joint_tile, flow_tile
+    p.with_visualization = int(bool(with_visualization))
     return p
 
 

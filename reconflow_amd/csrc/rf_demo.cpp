@@ -1,7 +1,7 @@
 // Command-line program over the C++ drop-in classes (include/rfhip_cudaflow.hpp), in the shape of
 // the reference demo (test_reconflow/main.cpp:16-141): build the solver, copy a pair in, solve,
 // copy the results out. The pair comes from the seeded generator; results are written as raw
-// float32 planes (u, v[, X, Y, Z]) of stride x height each.
+// float32 planes (u, v[, X, Y, Z, uvrgb]) of stride x height each (uvrgb: 3 floats per pixel).
 //
 //   rf_demo flow  <w> <h> <levels> <warps> <iters> <seed> <out.bin> [fused]
 //   rf_demo recon <w> <h> <levels> <warps> <iters> <seed> <out.bin> [fused] [ply]
@@ -72,10 +72,12 @@ int main(int argc, char** argv) {
             mv(v.data(), h, s, s * 4);
         if (rec.copyImagesToDevice(m0, m1)) return fail("copyImagesToDevice");
         if (rec.solveR(mR, mt, 50.0f)) return fail("solveR");
-        std::vector<float> X((size_t)s * h), Y((size_t)s * h), Z((size_t)s * h);
-        rfhip::Mat mX(X.data(), h, s, s * 4), mY(Y.data(), h, s, s * 4), mZ(Z.data(), h, s, s * 4);
-        if (rec.copyOpticalFlowToHost(mu, mv) || rec.copy3dToHost(mX, mY, mZ)) return fail("copy out");
-        for (auto* p : {&u, &v, &X, &Y, &Z}) std::fwrite(p->data(), 4, p->size(), f);
+        std::vector<float> X((size_t)s * h), Y((size_t)s * h), Z((size_t)s * h), rgb((size_t)s * h * 3);
+        rfhip::Mat mX(X.data(), h, s, s * 4), mY(Y.data(), h, s, s * 4), mZ(Z.data(), h, s, s * 4),
+            mrgb(rgb.data(), h, s, s * 12);
+        // the reference demo's copy-out (test_reconflow/main.cpp:105-106)
+        if (rec.copyOpticalFlowToHost(mu, mv, mrgb) || rec.copy3dToHost(mX, mY, mZ)) return fail("copy out");
+        for (auto* p : {&u, &v, &X, &Y, &Z, &rgb}) std::fwrite(p->data(), 4, p->size(), f);
         if (ply) {
             std::string plyname = std::string(out) + ".ply";
             if (rf_write_ply(plyname.c_str(), X.data(), Y.data(), Z.data(), nullptr, w, h, s, 10.0f, 75.0f, 75.0f))

@@ -59,6 +59,7 @@ int Solver::release() {
         ws_bad_ = nullptr;
     }
     ws_set_ = false;
+    uvrgb_valid_ = false;
     if (slab_) {
         (void)hipFree(slab_);
         slab_ = nullptr;
@@ -737,13 +738,20 @@ int Solver::solve_flow(float flow_scale) {
         set_error("rf_solve_flow: not a CudaFlow method");
         return RF_ERR_UNSUPPORTED;
     }
-    if (batch_) return batch_->solve(false, flow_scale);
-    if (p_.use_graph && p_.with_visualization && flow_scale != flow_scale_ && gexec_[0]) {
-        (void)hipGraphExecDestroy(gexec_[0]);  // recaptured with the new scale
-        gexec_[0] = nullptr;
+    uvrgb_valid_ = false;
+    int rc;
+    if (batch_) {
+        rc = batch_->solve(false, flow_scale);
+    } else {
+        if (p_.use_graph && p_.with_visualization && flow_scale != flow_scale_ && gexec_[0]) {
+            (void)hipGraphExecDestroy(gexec_[0]);  // recaptured with the new scale
+            gexec_[0] = nullptr;
+        }
+        flow_scale_ = flow_scale;
+        rc = run_schedule(0);
     }
-    flow_scale_ = flow_scale;
-    return run_schedule(0);
+    uvrgb_valid_ = rc == RF_OK && p_.with_visualization;
+    return rc;
 }
 
 int Solver::solve_recon(const double* R0, const double* t0, const double* R1, const double* t1, float flow_scale) {
@@ -773,9 +781,12 @@ int Solver::solve_recon(const double* R0, const double* t0, const double* R1, co
     memcpy(t0_, t0 ? t0 : Z3, sizeof t0_);
     memcpy(R1_, R1, sizeof R1_);
     memcpy(t1_, t1, sizeof t1_);
+    uvrgb_valid_ = false;
     if (batch_) {
         int rc = batch_->set_camera(0, K0raw_, K1raw_, R0_, t0_, R1_, t1_);
-        return rc ? rc : batch_->solve(true, flow_scale);
+        rc = rc ? rc : batch_->solve(true, flow_scale);
+        uvrgb_valid_ = rc == RF_OK && p_.with_visualization;
+        return rc;
     }
     // P = K0[l]*[R0|t0], Q = K1[l]*[R1|t1] in double (Reconflow.cu:225-226), cast to float
     for (int l = 0; l < g_.L; ++l) {
@@ -809,7 +820,9 @@ int Solver::solve_recon(const double* R0, const double* t0, const double* R1, co
         gexec_[1] = nullptr;
     }
     flow_scale_ = flow_scale;
-    return run_schedule(1);
+    int rc = run_schedule(1);
+    uvrgb_valid_ = rc == RF_OK && p_.with_visualization;
+    return rc;
 }
 
 int Solver::sync() { return check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
@@ -827,6 +840,10 @@ int Solver::copy_flow(float* u, float* v, float* uvrgb) {
     hipStream_t st = stream();
     if (check(hipMemcpyAsync(u, du, bytes, hipMemcpyDeviceToHost, st), "copy u")) return RF_ERR_HIP;
     if (check(hipMemcpyAsync(v, dv, bytes, hipMemcpyDeviceToHost, st), "copy v")) return RF_ERR_HIP;
+    if (uvrgb && !uvrgb_valid_) {
+        set_error("rf_copy_flow: no colour map: the last solve ran without visualization");
+        return RF_ERR_STATE;
+    }
     if (uvrgb && check(hipMemcpyAsync(uvrgb, drgb, bytes * 3, hipMemcpyDeviceToHost, st), "copy uvrgb"))
         return RF_ERR_HIP;
     return check(hipStreamSynchronize(st), "copy flow");
@@ -883,7 +900,10 @@ int Solver::solve_inpainting(float flow_scale) {
         gexec_[2] = nullptr;
     }
     flow_scale_ = flow_scale;
-    return run_schedule(2);
+    uvrgb_valid_ = false;
+    int rc = run_schedule(2);
+    uvrgb_valid_ = rc == RF_OK && p_.with_visualization;
+    return rc;
 }
 
 // CudaFlow::_solveSceneFlow / _solveSceneFlowAndPointCloud (Cudaflow.cu:836-871): the flow solve,
@@ -1001,7 +1021,7 @@ void rf_params_default(rf_params* p, int width, int height) {
     p->tau = 0.25f;
     p->n_warp_iters = 1;
     p->n_solver_iters = 100;
-    p->with_visualization = 0;
+    p->with_visualization = 1;
     p->lambdaf = 100.0f;
     p->lambdams = 100.0f;
     p->alpha_tv = 3.0f;

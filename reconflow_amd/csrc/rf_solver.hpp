@@ -135,6 +135,8 @@ class Solver {
     rf_params p_{};
     bool inited_ = false, prior_ = false, sparse_ = false, camera_ = false;
     bool masks_ = false, depth_ = false, scene_ = false, masked_run_ = false;
+    // the colour map (uvrgb) holds the last solve's flow: that solve ran with visualization
+    bool uvrgb_valid_ = false;
     Geometry g_;
     hipStream_t st_ = nullptr;
     void* slab_ = nullptr;

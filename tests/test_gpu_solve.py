@@ -183,6 +183,47 @@ def test_recon_640x480_fused_epe(orc, joint_640, storage, bound):
     assert e.mean() <= bound
 
 
+def _hsv_ref(orc, u, v, w, h, s, fs):
+    out = np.zeros((h + 1) * s * 3, np.float32)
+    uu, vv = (np.concatenate([a.ravel(), np.zeros(s, np.float32)]) for a in (u, v))
+    orc.call("flow_to_hsv", "f32", uu, vv, w, h, s, out, fs)
+    return out[:h * s * 3].reshape(h, s, 3)
+
+
+@pytest.mark.parametrize("path,graph", [(rf.PATH_PLAIN, False), (rf.PATH_PLAIN, True), (rf.PATH_FUSED, True)],
+                         ids=["plain", "plain-graph", "fused-graph"])
+def test_recon_colour_map_after_solve(orc, path, graph):
+    """solveReconFlow ends with FlowToHSV (Reconflow.cu:460), so copyOpticalFlowToHost(u, v, uvrgb)
+    after solveR (the reference demo's call, test_reconflow/main.cpp:104-105) returns the colour map
+    of the returned flow: within 1e-5 of the oracle's (atan2f, sqrtf), for each flow scale."""
+    print(f"case: joint 176x136 colour map path={path} graph={graph}", flush=True)
+    w, h, L = 176, 136, 3
+    d = rf.make_recon_pair(w, h, 5)
+    r = _recon(w, h, L, path=path, graph=graph, tri=1, nw=2, ni=10)
+    r.setCameraMatrices(d["K"], d["K"])
+    r.copyImagesToDevice(d["i0"], d["i1"])
+    s = r.stride
+    for fs in (50.0, 20.0, 50.0):  # a changed scale is applied to a captured schedule too
+        r.solveR(d["R"], d["t"], fs)
+        u, v, rgb = r.copyOpticalFlowToHost(with_rgb=True)
+        ref = _hsv_ref(orc, u, v, w, h, s, fs)
+        assert np.abs(ref[:, :w]).max() > 0.1
+        np.testing.assert_allclose(rgb[:, :w], ref[:, :w], rtol=0, atol=1e-5)
+    # without visualization no colour map is computed, and asking for it is an error
+    q = rf.ReconFlow()
+    q._initializeR(w, h, 1, L, 2.0, 14, 50.0, 0.0, 0.1, 100.0, 33.3, 60.0, 0.125, 2, 10, cvType=rf.INPUT_32F,
+                   path=path, use_graph=graph, withVisualization=False)
+    q.params.tri_f64 = 1
+    q._init_params(q.params)
+    q.setCameraMatrices(d["K"], d["K"])
+    q.copyImagesToDevice(d["i0"], d["i1"])
+    q.solveR(d["R"], d["t"], 50.0)
+    qu, qv = q.copyOpticalFlowToHost()
+    assert np.array_equal(qu[:, :w], u[:, :w]) and np.array_equal(qv[:, :w], v[:, :w])
+    with pytest.raises(rf.RFError):
+        q.copyOpticalFlowToHost(with_rgb=True)
+
+
 @pytest.mark.parametrize("path,graph", [(rf.PATH_PLAIN, True), (rf.PATH_FUSED, False), (rf.PATH_FUSED, True)],
                          ids=["plain-graph", "fused", "fused-graph"])
 @pytest.mark.parametrize("tri,fix", [(0, 0), (1, 1)], ids=["ref", "tri64-fixk"])
