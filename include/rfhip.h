@@ -96,7 +96,10 @@ typedef struct rf_params {
     int use_graph;         /* capture the static solve schedule in a hipGraph and replay it */
     int fix_coarse_k;      /* 1: K[2][2] = 1 on every pyramid level (the reference leaves it 0 on
                               levels >= 1, Reconflow.cu:99-110, which breaks the coarse triangulation) */
-    int tri_f64;           /* 1: evaluate the triangulation (Solve3dMinimalArea) in FP64 */
+    int tri_f64;           /* 1: the FP64 mode: binary64 arithmetic where FP32 loses the joint solve's
+                              accuracy: the triangulation (Solve3dMinimalArea), the projection blend
+                              (SolveUVProj) and the data step's blend (SolveReconDataL1); fields stay
+                              FP32. 0: every step in FP32 in the reference's operation order */
     int joint_tile;        /* fused joint iteration tiling: RF_JOINT_TILE_* (results are identical) */
     int flow_tile;         /* fused TV-L1 iteration tiling: RF_FLOW_TILE_* (results are identical) */
 } rf_params;

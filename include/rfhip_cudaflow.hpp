@@ -320,6 +320,7 @@ class ReconFlow : public CudaFlow {
                     alphaProj, alphaFn, tau, nWarpIters, nSolverIters, true);
     }
     void setInputType(int cvType) { inputType_ = cvType; }
+    // the FP64 mode (rf_params::tri_f64): binary64 triangulation, projection blend and data blend
     void setTriangulationFp64(bool on) { triF64_ = on; }
     void setFixCoarseIntrinsics(bool on) { fixK_ = on; }
     // solveReconFlow ends with FlowToHSV (Reconflow.cu:460), so copyOpticalFlowToHost(u, v, uvrgb)

@@ -825,6 +825,31 @@ void recon_data_l1_wide(const T* u0, const T* v0, const T* duhat0, const T* dvha
         }
 }
 
+// The prior-flow data step (solveReconDataL1.cu:139-240, 276-378) with its blend in binary64:
+// AA = (aTv + aP) + aFn formed in binary64 (aFn = alphaFn times the mask in T), the blend divided
+// by it and rounded once; the threshold and the divergence in T with the T-typed AA.
+template <typename T>
+void recon_data_l1_fn_wide(const T* u0, const T* v0, const T* duhat0, const T* dvhat0, const T* uproj0,
+                           const T* vproj0, const T* ufn0, const T* vfn0, const T* sku0, const T* skv0, const T* pu1,
+                           const T* pu2, const T* pv1, const T* pv2, const T* Ix, const T* Iy, const T* It, int w,
+                           int h, int s, T lambda, T alphaTv, T alphaProj, T alphaFn0, const T* mask, T* duhat1,
+                           T* dvhat1) {
+    const double aT = alphaTv, aP = alphaProj;
+#pragma omp parallel for
+    for (int iy = 0; iy < h; ++iy)
+        for (int ix = 0; ix < w; ++ix) {
+            long p = ix + (long)iy * s;
+            T alphaFn = mask ? alphaFn0 * mask[p] : alphaFn0;
+            const double aF = alphaFn, AAd = (aT + aP) + aF;
+            T bu = (T)(((aF * ((double)ufn0[p] - (double)u0[p]) + aT * (double)duhat0[p]) +
+                        aP * (((double)uproj0[p] - (double)u0[p]) - (double)sku0[p])) / AAd);
+            T bv = (T)(((aF * ((double)vfn0[p] - (double)v0[p]) + aT * (double)dvhat0[p]) +
+                        aP * (((double)vproj0[p] - (double)v0[p]) - (double)skv0[p])) / AAd);
+            recon_data_px<T, T>(Ix[p], Iy[p], It[p], bu, bv, lambda, (alphaTv + alphaProj) + alphaFn,
+                                div_p(pu1, pu2, ix, iy, s), div_p(pv1, pv2, ix, iy, s), duhat1[p], dvhat1[p]);
+        }
+}
+
 // ---------------------------------------------------------------------------------
 // f2 / f3 / f4: display, post-filter and alternate data terms.
 
@@ -1133,7 +1158,8 @@ struct Params {
     int stride_alignment;
     int input_type;    // 0: float gray, 1: 8-bit RGB (uchar3), 2: 8-bit gray, 3: 16-bit gray
     int fix_coarse_k;  // 0: reference intrinsics pyramid (K[2][2] = 0 on levels >= 1); 1: K[2][2] = 1
-    int tri_f64;       // 1: the triangulation step runs in FP64 whatever T is
+    int tri_f64;       // 1: the FP64 mode: the triangulation step in FP64 whatever T is, and in the FP32
+                       //    solve the projection and data blends in binary64 (PREC_PROJ | PREC_BLEND)
     // Test-only knobs (0 = off); neither is reference semantics.
     //   mutation:   plants one known schedule mistake (MUT_* below) so that tests can show their
     //               checks detect it;
@@ -1530,7 +1556,9 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
     k_pyramid(K1, P.n_levels, P.scale, P.fix_coarse_k, pK1);
 
     const int mut = P.mutation;
-    const int prec = sizeof(T) == sizeof(float) ? P.proj_f64 : 0;  // binary64 already in the FP64 solve
+    // binary64 blends: the FP64 mode (tri_f64) takes the projection and data blends along with the
+    // triangulation; proj_f64 selects them one by one. The FP64 solve computes in binary64 anyway.
+    const int prec = sizeof(T) == sizeof(float) ? (P.proj_f64 | (P.tri_f64 ? PREC_PROJ | PREC_BLEND : 0)) : 0;
     gray_input(P, i0, pI0[0], s0);
     gray_input(P, i1, pI1[0], s0);
     round_group<T>(P, RND_IMG, n, {pI0[0], pI1[0]});
@@ -1585,7 +1613,11 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
             }
             round_group<T>(P, RND_DER, n, {Ix, Iy, Iz});
             for (int it = 0; it < P.n_solver_iters; ++it) {
-                if (fn)
+                if (fn && (prec & PREC_BLEND))
+                    recon_data_l1_fn_wide(u, v, dumed, dvmed, uproj, vproj, ufn_l, vfn_l, sku, skv, pu1, pu2, pv1,
+                                          pv2, Ix, Iy, Iz, w, h, s, lambda, alphaTv, alphaProj, alphaFn,
+                                          P.method == 20 ? mask_l : nullptr, dumeds, dvmeds);
+                else if (fn)
                     recon_data_l1_fn(u, v, dumed, dvmed, uproj, vproj, ufn_l, vfn_l, sku, skv, pu1, pu2, pv1,
                                      pv2, Ix, Iy, Iz, w, h, s, lambda, alphaTv, alphaProj, alphaFn,
                                      P.method == 20 ? mask_l : nullptr, dumeds, dvmeds);

@@ -400,6 +400,9 @@ struct IterArgs {
     // the launch-uniform divisors aTv + aP and lf + aP (every joint kernel without a prior flow
     // divides by the first, every joint kernel by the second)
     UDiv uA, uP;
+    // FP64 mode: 1 / (aTv + aP) and 1 / (lf + aP) with the sums formed in binary64 (the blends'
+    // reciprocals, recon_blend_wide / uvproj_px_wide)
+    double rA64, rP64;
 };
 
 // Where a tile's pair lives relative to the view: element q of the pair's fields is element
@@ -500,14 +503,23 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
         if (PRIOR) {
             const float aFn = mask ? a.aFn * mask[q] : a.aFn;
             const float AA = (a.aTv + a.aP) + aFn;
-            const float bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, duh, up0[q], uu, sku, ufn[q]);
-            const float bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, dvh, vp0[q], vv, skv, vfn[q]);
+            float bu, bv;
+            if (F64TRI) {
+                const double AAd = ((double)a.aTv + (double)a.aP) + (double)aFn;
+                bu = recon_blend_fn_wide(a.aTv, a.aP, aFn, AAd, duh, up0[q], uu, sku, ufn[q]);
+                bv = recon_blend_fn_wide(a.aTv, a.aP, aFn, AAd, dvh, vp0[q], vv, skv, vfn[q]);
+            } else {
+                bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, duh, up0[q], uu, sku, ufn[q]);
+                bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, dvh, vp0[q], vv, skv, vfn[q]);
+            }
             l1_threshold_div(Ix[q], Iy[q], Iz[q], bu, bv, a.lambda, AA, du, dv);
             nduh = du + div_p(p1u, p1ul, p2u, p2ud, hl, hu) / AA;
             ndvh = dv + div_p(p1v, p1vl, p2v, p2vd, hl, hu) / AA;
         } else {
-            const float bu = recon_blend(a.aTv, a.aP, a.uA, duh, up0[q], uu, sku);
-            const float bv = recon_blend(a.aTv, a.aP, a.uA, dvh, vp0[q], vv, skv);
+            const float bu = F64TRI ? recon_blend_wide(a.aTv, a.aP, a.rA64, duh, up0[q], uu, sku)
+                                    : recon_blend(a.aTv, a.aP, a.uA, duh, up0[q], uu, sku);
+            const float bv = F64TRI ? recon_blend_wide(a.aTv, a.aP, a.rA64, dvh, vp0[q], vv, skv)
+                                    : recon_blend(a.aTv, a.aP, a.uA, dvh, vp0[q], vv, skv);
             l1_threshold_div(Ix[q], Iy[q], Iz[q], bu, bv, a.lambda, a.uA, du, dv);
             nduh = du + udiv(div_p(p1u, p1ul, p2u, p2ud, hl, hu), a.uA);
             ndvh = dv + udiv(div_p(p1v, p1vl, p2v, p2vd, hl, hu), a.uA);
@@ -515,8 +527,10 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
         const int e = ex + ey * EX;
         s_du[e] = nduh;
         s_dv[e] = ndvh;
-        s_up[e] = uvproj_px(a.lf, a.aP, a.uP, uc0[q], uu, nduh, sku);
-        s_vp[e] = uvproj_px(a.lf, a.aP, a.uP, vc0[q], vv, ndvh, skv);
+        s_up[e] = F64TRI ? uvproj_px_wide(a.lf, a.aP, a.rP64, uc0[q], uu, nduh, sku)
+                         : uvproj_px(a.lf, a.aP, a.uP, uc0[q], uu, nduh, sku);
+        s_vp[e] = F64TRI ? uvproj_px_wide(a.lf, a.aP, a.rP64, vc0[q], vv, ndvh, skv)
+                         : uvproj_px(a.lf, a.aP, a.uP, vc0[q], vv, ndvh, skv);
         if (keep) {
             keep[0] = p1u;
             keep[1] = p2u;
@@ -705,20 +719,31 @@ __device__ __forceinline__ JointFront joint_compute(const IterArgs& a, const Joi
     if (PRIOR) {
         const float aFn = a.sparse ? a.aFn * in.msk : a.aFn;
         const float AA = (a.aTv + a.aP) + aFn;
-        const float bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.duh, in.up0, in.uu, in.sku, in.ufn);
-        const float bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.dvh, in.vp0, in.vv, in.skv, in.vfn);
+        float bu, bv;
+        if (F64TRI) {
+            const double AAd = ((double)a.aTv + (double)a.aP) + (double)aFn;
+            bu = recon_blend_fn_wide(a.aTv, a.aP, aFn, AAd, in.duh, in.up0, in.uu, in.sku, in.ufn);
+            bv = recon_blend_fn_wide(a.aTv, a.aP, aFn, AAd, in.dvh, in.vp0, in.vv, in.skv, in.vfn);
+        } else {
+            bu = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.duh, in.up0, in.uu, in.sku, in.ufn);
+            bv = recon_blend_fn(a.aTv, a.aP, aFn, AA, in.dvh, in.vp0, in.vv, in.skv, in.vfn);
+        }
         l1_threshold_div(in.ix, in.iy, in.iz, bu, bv, a.lambda, AA, du, dv);
         nduh = du + div_p(in.p1u, in.p1ul, in.p2u, in.p2ud, hl, hu) / AA;
         ndvh = dv + div_p(in.p1v, in.p1vl, in.p2v, in.p2vd, hl, hu) / AA;
     } else {
-        const float bu = recon_blend(a.aTv, a.aP, a.uA, in.duh, in.up0, in.uu, in.sku);
-        const float bv = recon_blend(a.aTv, a.aP, a.uA, in.dvh, in.vp0, in.vv, in.skv);
+        const float bu = F64TRI ? recon_blend_wide(a.aTv, a.aP, a.rA64, in.duh, in.up0, in.uu, in.sku)
+                                : recon_blend(a.aTv, a.aP, a.uA, in.duh, in.up0, in.uu, in.sku);
+        const float bv = F64TRI ? recon_blend_wide(a.aTv, a.aP, a.rA64, in.dvh, in.vp0, in.vv, in.skv)
+                                : recon_blend(a.aTv, a.aP, a.uA, in.dvh, in.vp0, in.vv, in.skv);
         l1_threshold_div(in.ix, in.iy, in.iz, bu, bv, a.lambda, a.uA, du, dv);
         nduh = du + udiv(div_p(in.p1u, in.p1ul, in.p2u, in.p2ud, hl, hu), a.uA);
         ndvh = dv + udiv(div_p(in.p1v, in.p1vl, in.p2v, in.p2vd, hl, hu), a.uA);
     }
-    const float upn = uvproj_px(a.lf, a.aP, a.uP, in.uc0, in.uu, nduh, in.sku);
-    const float vpn = uvproj_px(a.lf, a.aP, a.uP, in.vc0, in.vv, ndvh, in.skv);
+    const float upn = F64TRI ? uvproj_px_wide(a.lf, a.aP, a.rP64, in.uc0, in.uu, nduh, in.sku)
+                             : uvproj_px(a.lf, a.aP, a.uP, in.uc0, in.uu, nduh, in.sku);
+    const float vpn = F64TRI ? uvproj_px_wide(a.lf, a.aP, a.rP64, in.vc0, in.vv, ndvh, in.skv)
+                             : uvproj_px(a.lf, a.aP, a.uP, in.vc0, in.vv, ndvh, in.skv);
 
     // ---- triangulation
     float oX, oY, oZ;
@@ -1858,7 +1883,9 @@ hipError_t Batch::enqueue_t(bool recon) {
                 IterArgs a{Launch{dp_, ts.map, ts.ntiles}, level, pr.d, pr.f, pr.x, pr.u, it == 0, p_.lambda, theta,
                            p_.tau, p_.alpha_tv, p_.alpha_proj, p_.alpha_fn, p_.lambdaf, p_.lambdams,
                            (fn && sparse_) ? 1 : 0, JointView{}, 0, 0.0f, 0u,
-                           make_udiv(p_.alpha_tv + p_.alpha_proj), make_udiv(p_.lambdaf + p_.alpha_proj)};
+                           make_udiv(p_.alpha_tv + p_.alpha_proj), make_udiv(p_.lambdaf + p_.alpha_proj),
+                           1.0 / ((double)p_.alpha_tv + (double)p_.alpha_proj),
+                           1.0 / ((double)p_.lambdaf + (double)p_.alpha_proj)};
                 // one pair, or pairs of one size: the fields travel in the kernel arguments (no
                 // dependent descriptor load)
                 const bool single = (n() == 1 || pair_elems_ > 0) && cls != TC_J64 && cls != TC_FLOW;

@@ -123,7 +123,7 @@ int rf_k_solve_recon_data_l1(const float* u, const float* v, const float* duhat0
     GEOM("rf_k_solve_recon_data_l1");
     return done(launch_recon_data_l1(u, v, duhat0, dvhat0, uproj, vproj, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w,
                                      h, s, lambda, alpha_tv, alpha_proj, nullptr, nullptr, 0.0f, nullptr, duhat1,
-                                     dvhat1, ST),
+                                     dvhat1, false, ST),
                 "rf_k_solve_recon_data_l1");
 }
 int rf_k_solve_recon_data_l1_fn(const float* u, const float* v, const float* duhat0, const float* dvhat0,
@@ -136,14 +136,14 @@ int rf_k_solve_recon_data_l1_fn(const float* u, const float* v, const float* duh
     if (!ufn || !vfn) return RF_ERR_ARG;
     return done(launch_recon_data_l1(u, v, duhat0, dvhat0, uproj, vproj, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w,
                                      h, s, lambda, alpha_tv, alpha_proj, ufn, vfn, alpha_fn, fnmask, duhat1, dvhat1,
-                                     ST),
+                                     false, ST),
                 "rf_k_solve_recon_data_l1_fn");
 }
 int rf_k_solve_uvproj(const float* u, const float* v, const float* du, const float* dv, const float* uc,
                       const float* vc, const float* sku, const float* skv, int w, int h, int s, float lambdaf,
                       float alpha_proj, float* uproj, float* vproj, void* stream) {
     GEOM("rf_k_solve_uvproj");
-    return done(launch_uvproj(u, v, du, dv, uc, vc, sku, skv, w, h, s, lambdaf, alpha_proj, uproj, vproj, ST),
+    return done(launch_uvproj(u, v, du, dv, uc, vc, sku, skv, w, h, s, lambdaf, alpha_proj, uproj, vproj, false, ST),
                 "rf_k_solve_uvproj");
 }
 int rf_k_solve_3d_minimal_area(const float* uproj, const float* vproj, const float* X0, const float* Y0,

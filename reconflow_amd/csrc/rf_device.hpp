@@ -250,6 +250,31 @@ __device__ __forceinline__ float uvproj_px(float lf, float aP, const UDiv& P, fl
     return udiv(lf * uc + aP * ((u + du) + sk), P);
 }
 
+// Binary64 forms of the two blends for the FP64 mode (this implementation's arithmetic; the
+// reference evaluates them in FP32). Both add and subtract values of the size of the flow (u, uproj
+// and sk reach 20 px and more at 640x480) to form differences of ~1e-3 px, so an FP32 evaluation
+// loses ~1e-6 px each time, and the Sk sum carries the loss through every iteration of a frame.
+// Here the stored FP32 values are widened, the launch constants' sums are formed in binary64, the
+// launch-uniform divisions are products with the binary64 reciprocals rA = 1 / (aTv + aP) and
+// rP = 1 / (lf + aP), and the result is rounded once. The host and the oracle use the same
+// expressions in the same order (no contraction).
+__device__ __forceinline__ float recon_blend_wide(double aT, double aP, double rA, float uhat, float uproj, float u,
+                                                  float sk) {
+    return (float)((aT * (double)uhat + aP * (((double)uproj - (double)u) - (double)sk)) * rA);
+}
+
+// With a prior flow: AA = (aTv + aP) + aFn in binary64 (aFn = alphaFn times the mask in FP32).
+__device__ __forceinline__ float recon_blend_fn_wide(double aT, double aP, double aFn, double AA, float uhat,
+                                                     float uproj, float u, float sk, float ufn) {
+    return (float)(((aFn * ((double)ufn - (double)u) + aT * (double)uhat) + aP * (((double)uproj - (double)u) - (double)sk)) /
+                   AA);
+}
+
+__device__ __forceinline__ float uvproj_px_wide(double lf, double aP, double rP, float uc, float u, float du,
+                                                float sk) {
+    return (float)((lf * (double)uc + aP * (((double)u + (double)du) + (double)sk)) * rP);
+}
+
 // Sk update (solveUcSk.cu:93-94).
 __device__ __forceinline__ float sk_px(float sk, float u, float du, float uproj) {
     return ((sk + u) + du) - uproj;

@@ -209,8 +209,9 @@ __global__ void __launch_bounds__(256) k_add(const float* a, const float* b, lon
     if (i < count) out[i] = a[i] + b[i];
 }
 
-// solveReconDataL1.cu:4-102 (prior == false) and :139-378 (prior == true; mask may be null)
-template <bool PRIOR>
+// solveReconDataL1.cu:4-102 (prior == false) and :139-378 (prior == true; mask may be null).
+// WIDE: the blend in binary64 (the FP64 mode, recon_blend_wide / recon_blend_fn_wide).
+template <bool PRIOR, bool WIDE>
 __global__ void __launch_bounds__(256) k_recon_data_l1(const float* u, const float* v, const float* du0,
                                                         const float* dv0, const float* up, const float* vp,
                                                         const float* sku, const float* skv, const float* pu1,
@@ -224,12 +225,24 @@ __global__ void __launch_bounds__(256) k_recon_data_l1(const float* u, const flo
     if (PRIOR) {
         float aFn = mask ? aFn0 * mask[pos] : aFn0;
         A = (aTv + aP) + aFn;
-        bu = recon_blend_fn(aTv, aP, aFn, A, du0[pos], up[pos], u[pos], sku[pos], ufn[pos]);
-        bv = recon_blend_fn(aTv, aP, aFn, A, dv0[pos], vp[pos], v[pos], skv[pos], vfn[pos]);
+        if (WIDE) {
+            const double AAd = ((double)aTv + (double)aP) + (double)aFn;
+            bu = recon_blend_fn_wide(aTv, aP, aFn, AAd, du0[pos], up[pos], u[pos], sku[pos], ufn[pos]);
+            bv = recon_blend_fn_wide(aTv, aP, aFn, AAd, dv0[pos], vp[pos], v[pos], skv[pos], vfn[pos]);
+        } else {
+            bu = recon_blend_fn(aTv, aP, aFn, A, du0[pos], up[pos], u[pos], sku[pos], ufn[pos]);
+            bv = recon_blend_fn(aTv, aP, aFn, A, dv0[pos], vp[pos], v[pos], skv[pos], vfn[pos]);
+        }
     } else {
         A = aTv + aP;
-        bu = recon_blend(aTv, aP, A, du0[pos], up[pos], u[pos], sku[pos]);
-        bv = recon_blend(aTv, aP, A, dv0[pos], vp[pos], v[pos], skv[pos]);
+        if (WIDE) {
+            const double rA = 1.0 / ((double)aTv + (double)aP);
+            bu = recon_blend_wide(aTv, aP, rA, du0[pos], up[pos], u[pos], sku[pos]);
+            bv = recon_blend_wide(aTv, aP, rA, dv0[pos], vp[pos], v[pos], skv[pos]);
+        } else {
+            bu = recon_blend(aTv, aP, A, du0[pos], up[pos], u[pos], sku[pos]);
+            bv = recon_blend(aTv, aP, A, dv0[pos], vp[pos], v[pos], skv[pos]);
+        }
     }
     float du, dv;
     l1_threshold_div(Ix[pos], Iy[pos], Iz[pos], bu, bv, lambda, A, du, dv);
@@ -240,14 +253,21 @@ __global__ void __launch_bounds__(256) k_recon_data_l1(const float* u, const flo
     dv1[pos] = dv + divv / A;
 }
 
-// solveUVProj.cu:4-22
+// solveUVProj.cu:4-22. WIDE: the blend in binary64 (the FP64 mode, uvproj_px_wide).
+template <bool WIDE>
 __global__ void __launch_bounds__(256) k_uvproj(const float* u, const float* v, const float* du, const float* dv,
                                                  const float* uc, const float* vc, const float* sku,
                                                  const float* skv, int w, int h, int s, float lf, float aP,
                                                  float* up, float* vp) {
     RF_PIX
-    up[pos] = uvproj_px(lf, aP, uc[pos], u[pos], du[pos], sku[pos]);
-    vp[pos] = uvproj_px(lf, aP, vc[pos], v[pos], dv[pos], skv[pos]);
+    if (WIDE) {
+        const double rP = 1.0 / ((double)lf + (double)aP);
+        up[pos] = uvproj_px_wide(lf, aP, rP, uc[pos], u[pos], du[pos], sku[pos]);
+        vp[pos] = uvproj_px_wide(lf, aP, rP, vc[pos], v[pos], dv[pos], skv[pos]);
+    } else {
+        up[pos] = uvproj_px(lf, aP, uc[pos], u[pos], du[pos], sku[pos]);
+        vp[pos] = uvproj_px(lf, aP, vc[pos], v[pos], dv[pos], skv[pos]);
+    }
 }
 
 // solve3dMinimalArea.cu:3-193. Reads column w and row h of X0 (see solve3d_px).
@@ -662,18 +682,25 @@ hipError_t launch_recon_data_l1(const float* u, const float* v, const float* du,
                                 const float* pu2, const float* pv1, const float* pv2, const float* Ix,
                                 const float* Iy, const float* Iz, int w, int h, int s, float lambda, float aTv,
                                 float aP, const float* ufn, const float* vfn, float aFn, const float* mask,
-                                float* du1, float* dv1, hipStream_t st) {
+                                float* du1, float* dv1, bool wide, hipStream_t st) {
     if (ufn && vfn) {
-        RF_L2(k_recon_data_l1<true>, u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s,
+        if (wide)
+            RF_L2((k_recon_data_l1<true, true>), u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h,
+                  s, lambda, aTv, aP, ufn, vfn, aFn, mask, du1, dv1);
+        RF_L2((k_recon_data_l1<true, false>), u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s,
               lambda, aTv, aP, ufn, vfn, aFn, mask, du1, dv1);
     }
-    RF_L2(k_recon_data_l1<false>, u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s, lambda,
-          aTv, aP, ufn, vfn, aFn, mask, du1, dv1);
+    if (wide)
+        RF_L2((k_recon_data_l1<false, true>), u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s,
+              lambda, aTv, aP, ufn, vfn, aFn, mask, du1, dv1);
+    RF_L2((k_recon_data_l1<false, false>), u, v, du, dv, up, vp, sku, skv, pu1, pu2, pv1, pv2, Ix, Iy, Iz, w, h, s,
+          lambda, aTv, aP, ufn, vfn, aFn, mask, du1, dv1);
 }
 hipError_t launch_uvproj(const float* u, const float* v, const float* du, const float* dv, const float* uc,
                          const float* vc, const float* sku, const float* skv, int w, int h, int s, float lf,
-                         float aP, float* up, float* vp, hipStream_t st) {
-    RF_L2(k_uvproj, u, v, du, dv, uc, vc, sku, skv, w, h, s, lf, aP, up, vp);
+                         float aP, float* up, float* vp, bool wide, hipStream_t st) {
+    if (wide) RF_L2(k_uvproj<true>, u, v, du, dv, uc, vc, sku, skv, w, h, s, lf, aP, up, vp);
+    RF_L2(k_uvproj<false>, u, v, du, dv, uc, vc, sku, skv, w, h, s, lf, aP, up, vp);
 }
 hipError_t launch_solve3d(const float* up, const float* vp, const float* X0, const float* Y0, const float* Z0,
                           const PQ& pq, const PQ* dpq, float lf, float lms, int w, int h, int s, float* X, float* Y,

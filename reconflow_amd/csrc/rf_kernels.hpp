@@ -46,10 +46,10 @@ hipError_t launch_recon_data_l1(const float* u, const float* v, const float* du,
                                 const float* pu2, const float* pv1, const float* pv2, const float* Ix,
                                 const float* Iy, const float* Iz, int w, int h, int s, float lambda, float aTv,
                                 float aP, const float* ufn, const float* vfn, float aFn, const float* mask,
-                                float* du1, float* dv1, hipStream_t st);
+                                float* du1, float* dv1, bool wide, hipStream_t st);
 hipError_t launch_uvproj(const float* u, const float* v, const float* du, const float* dv, const float* uc,
                          const float* vc, const float* sku, const float* skv, int w, int h, int s, float lf,
-                         float aP, float* up, float* vp, hipStream_t st);
+                         float aP, float* up, float* vp, bool wide, hipStream_t st);
 // dpq / dc (device pointers, may be null) override the by-value camera data.
 hipError_t launch_solve3d(const float* up, const float* vp, const float* X0, const float* Y0, const float* Z0,
                           const PQ& pq, const PQ* dpq, float lf, float lms, int w, int h, int s, float* X, float* Y,

@@ -564,7 +564,7 @@ hipError_t Solver::enqueue_recon_plain(Fields& f) {
                                             f.pv1, f.pv2, f.Ix, f.Iy, f.Iz, w, h, s, p_.lambda, p_.alpha_tv,
                                             p_.alpha_proj, fn ? f.ufn_l : nullptr, fn ? f.vfn_l : nullptr,
                                             p_.alpha_fn, (fn && sparse_) ? f.mask_l : nullptr, f.dumeds, f.dvmeds,
-                                            st_));
+                                            p_.tri_f64 != 0, st_));
                 std::swap(f.dumed, f.dumeds);
                 std::swap(f.dvmed, f.dvmeds);
                 RF_PROF(RF_KID_DUAL, launch_dual_tv(f.dumed, f.dvmed, f.pu1, f.pu2, f.pv1, f.pv2, w, h, s, p_.tau,
@@ -574,7 +574,7 @@ hipError_t Solver::enqueue_recon_plain(Fields& f) {
                 std::swap(f.pv1, f.pv1s);
                 std::swap(f.pv2, f.pv2s);
                 RF_PROF(RF_KID_UVPROJ, launch_uvproj(f.u, f.v, f.dumed, f.dvmed, f.uc, f.vc, f.sku, f.skv, w, h, s, p_.lambdaf,
-                                     p_.alpha_proj, f.uproj, f.vproj, st_));
+                                     p_.alpha_proj, f.uproj, f.vproj, p_.tri_f64 != 0, st_));
                 if (p_.tri_f64)
                     RF_PROF(RF_KID_SOLVE3D, launch_solve3d_f64(f.uproj, f.vproj, f.X, f.Y, f.Z, PQd{}, dpqd, p_.lambdaf, p_.lambdams, w,
                                               h, s, f.Xs, f.Ys, f.Zs, st_));

@@ -5,11 +5,12 @@ Bars:
   * the plain FP32 path equals the FP32 restatement bit for bit (whole solve);
   * TV-L1 (320x240, 4 levels, 3 warps, 30 iterations): mean EPE against
     the FP64 oracle <= 1e-3 px;
-  * joint flow/depth (640x480, 5 levels, 5 warps, 50 iterations): with the
-    triangulation in FP64 the result equals the FP32+FP64-triangulation restatement bit for
-    bit and its mean EPE against the FP64 oracle is <= 0.05 px. (With the reference's FP32
-    triangulation the FP32-vs-FP64 gap is a property of the arithmetic, not of this code: the
-    FP32 restatement itself misses the bar on some seeds; the bit-exact check covers it.)
+  * joint flow/depth (640x480, 5 levels, 5 warps, 50 iterations): in the FP64 mode (binary64
+    triangulation, projection blend and data blend) the result equals the oracle's restatement of
+    that mode bit for bit, its mean EPE against the FP64 oracle is <= 1e-3 px and its depth is
+    within 1e-4 relative of the FP64 oracle's at the 99th percentile. (The reference's all-FP32
+    arithmetic misses these bars by itself: the FP32 restatement of the reference order is ~0.01 px
+    from the FP64 oracle at 640x480; the bit-exact check covers that mode.)
 """
 import ctypes as C
 
@@ -156,31 +157,53 @@ def test_recon_640x480_bitexact(orc, joint_640, tri):
     assert _same(got, ref, 640)
 
 
-def test_recon_640x480_fp64_triangulation_epe(orc, joint_640):
-    print("case: joint 640x480 tri_f64 vs FP64 oracle", flush=True)
+def _depth_rel(Z, Z64, w):
+    zr = np.abs(Z[:, :w] - Z64[:, :w]) / np.abs(Z64[:, :w])
+    return zr[np.isfinite(zr)]
+
+
+# Accuracy against the FP64 oracle at 640x480 (seed 2), FP64 mode. Measured on seeds 2, 0, 1:
+#   FP32 storage:             mean EPE 5.8e-4 / 5.1e-4 / 4.5e-4 px, depth rel. median ~1e-6,
+#                             p99 3.5e-5 / 2.2e-5 / 7.9e-5;
+#   FP16 derivatives:         mean EPE 0.028 / 0.020 / 0.019 px, depth median 5e-5, p99 <= 2e-3;
+#   FP16 images+derivatives:  mean EPE 0.164 / 0.167 / 0.126 px, depth median 3e-4, p99 <= 1.7e-2
+#                             (misses the 0.05 px target: the joint solution follows the rounding
+#                             of It = I1w - I0 in FP16 images; bounded at its measured level).
+# Bounds: the target where a mode meets it, else the measured value plus ~25%.
+ACC = {rf.STORAGE_F32: (1e-3, 1e-5, 1e-4), rf.STORAGE_F16_DERIV: (0.05, 1e-4, 3e-3),
+       rf.STORAGE_F16: (0.2, 5e-4, 2.5e-2)}
+
+
+def _check_accuracy(u, v, Z, ref, storage):
+    e = _epe(u, v, ref[0], ref[1], 640)
+    zr = _depth_rel(Z, ref[4], 640)
+    b_epe, b_zmed, b_z99 = ACC[storage]
+    print(f"joint storage={storage}: mean EPE vs FP64 {e.mean():.3e}, p99 {np.percentile(e, 99):.3e}, "
+          f"depth rel err median {np.median(zr):.3e} p99 {np.percentile(zr, 99):.3e}", flush=True)
+    assert e.mean() <= b_epe, e.mean()
+    assert np.median(zr) <= b_zmed, np.median(zr)
+    assert np.percentile(zr, 99) <= b_z99, np.percentile(zr, 99)
+
+
+@pytest.fixture(scope="module")
+def joint_640_f64(orc, joint_640):
+    return _oracle_recon(orc, joint_640, 640, 480, 5, "f64")
+
+
+def test_recon_640x480_fp64_mode_accuracy(joint_640, joint_640_f64):
+    print("case: joint 640x480 plain FP64 mode vs FP64 oracle", flush=True)
     r = _recon(640, 480, 5, tri=1)
     u, v, X, Y, Z = _run_recon(r, joint_640)
-    u64, v64, X64, Y64, Z64 = _oracle_recon(orc, joint_640, 640, 480, 5, "f64")
-    e = _epe(u, v, u64, v64, 640)
-    assert e.mean() <= 0.05, e.mean()
+    _check_accuracy(u, v, Z, joint_640_f64, rf.STORAGE_F32)
 
 
-# Accuracy of the storage modes against the FP64 oracle (joint, 640x480). FP16 images move the
-# flow by ~0.15 px mean (the joint solution is sensitive to the rounding of It = I1w - I0), so
-# the joint solver's FP16 mode stores only the derivatives; the full FP16 mode is measured and
-# bounded at its observed level, not at the 0.05 px target.
-@pytest.mark.parametrize("storage,bound", [(rf.STORAGE_F32, 0.05), (rf.STORAGE_F16_DERIV, 0.05),
-                                           (rf.STORAGE_F16, 0.25)], ids=["f32", "f16-deriv", "f16"])
-def test_recon_640x480_fused_epe(orc, joint_640, storage, bound):
-    print(f"case: joint 640x480 fused graph tri_f64 storage={storage}", flush=True)
+@pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16_DERIV, rf.STORAGE_F16],
+                         ids=["f32", "f16-deriv", "f16"])
+def test_recon_640x480_fused_accuracy(joint_640, joint_640_f64, storage):
+    print(f"case: joint 640x480 fused graph FP64 mode storage={storage}", flush=True)
     r = _recon(640, 480, 5, path=rf.PATH_FUSED, graph=True, tri=1, storage=storage)
     u, v, X, Y, Z = _run_recon(r, joint_640)
-    u64, v64, X64, Y64, Z64 = _oracle_recon(orc, joint_640, 640, 480, 5, "f64")
-    e = _epe(u, v, u64, v64, 640)
-    zr = np.abs(Z[:, :640] - Z64[:, :640]) / np.abs(Z64[:, :640]).clip(1e-6)
-    print(f"joint storage={storage}: mean EPE vs FP64 {e.mean():.3e}, p99 {np.percentile(e, 99):.3e}, "
-          f"depth rel err median {np.median(zr):.3e}", flush=True)
-    assert e.mean() <= bound
+    _check_accuracy(u, v, Z, joint_640_f64, storage)
 
 
 def _hsv_ref(orc, u, v, w, h, s, fs):
