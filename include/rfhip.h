@@ -213,6 +213,21 @@ int rf_solve_scene_flow(rf_context* ctx, float fx, float fy, float cx, float cy,
 /* Blocks until the context's stream is idle. */
 int rf_synchronize(rf_context* ctx);
 
+/* Allocation-free TV-L1 solve. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It gives the bits of rf_initialize + rf_copy_images_device + rf_solve_flow on a fresh
+ * context (METHOD_TVL1, float gray input; params->path, storage, use_graph and with_visualization
+ * are ignored: plain FP32 schedule, no colour map) and calls no allocating routine. d_i0, d_i1:
+ * stride x height float device images; d_u, d_v: stride x height device outputs; enqueued on
+ * `stream` (hipStream_t as void*, NULL = the null stream). The working fields (2 * n_levels + 20
+ * buffers of stride x (height + 1) floats, rounded up to 64) live in a static device array of
+ * RF_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load on that GPU until the process ends;
+ * a solve that does not fit (e.g. above 960x540 with 5 levels) is RF_ERR_ARG. Every call on a GPU
+ * shares that array and zeroes the part it uses: calls that overlap on one GPU race. */
+#define RF_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int rf_solve_flow_noalloc(const rf_params* params, const float* d_i0, const float* d_i1, float* d_u, float* d_v,
+                          void* stream);
+
 /* ---- outputs ---- */
 /* CudaFlow::copyOpticalFlowToHost(u, v[, uvrgb]) (Cudaflow.cu:313-324): stride x height
  * floats each; uvrgb (3 floats per pixel, BGR) may be NULL. A non-NULL uvrgb after a solve that

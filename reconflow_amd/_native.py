@@ -52,6 +52,7 @@ _PROTOS = {
     "rf_copy_warm_start": (_I, [_P, _P, _P]),
     "rf_warm_start_in_use": (_I, [_P]),
     "rf_solve_flow": (_I, [_P, _F]),
+    "rf_solve_flow_noalloc": (_I, [C.POINTER(RFParams), _P, _P, _P, _P, _P]),
     "rf_copy_masks": (_I, [_P, _P, _P]),
     "rf_copy_depth": (_I, [_P, _P, _P]),
     "rf_solve_inpainting": (_I, [_P, _F]),

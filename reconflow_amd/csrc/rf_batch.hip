@@ -1313,6 +1313,11 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
             set_error("rf_batch_create: pair " + std::to_string(i) + " has no pixels");
             return RF_ERR_ARG;
         }
+        if (nw >= kMaxSide || nh >= kMaxSide) {  // 32-bit texel-coordinate products in the resampling
+            set_error("rf_batch_create: pair " + std::to_string(i) + ": width and height must be below " +
+                      std::to_string(kMaxSide));
+            return RF_ERR_ARG;
+        }
         for (int l = 0; l < p.n_levels; ++l) {
             if (nw < 1 || nh < 1) {
                 set_error("rf_batch_create: pair " + std::to_string(i) + " runs out of pixels at level " +

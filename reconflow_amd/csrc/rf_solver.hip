@@ -76,13 +76,46 @@ int Solver::release() {
     return RF_OK;
 }
 
-// CudaFlow::initialize (Cudaflow.cu:99-269) and ReconFlow::_initializeR (Reconflow.cu:27-63).
-int Solver::initialize(const rf_params& p) {
+// Argument checks shared by rf_initialize and rf_solve_flow_noalloc.
+int Solver::check_params(const rf_params& p, const char* who) {
     if (p.width <= 0 || p.height <= 0 || p.n_levels <= 0 || p.n_levels > kMaxLevels || !(p.scale > 1.0f) ||
         p.n_warp_iters < 0 || p.n_solver_iters < 0) {
-        set_error("rf_initialize: invalid size, level count or scale");
+        set_error(std::string(who) + ": invalid size, level count or scale");
         return RF_ERR_ARG;
     }
+    if (p.width >= kMaxSide || p.height >= kMaxSide) {
+        set_error(std::string(who) + ": width and height must be below " + std::to_string(kMaxSide));
+        return RF_ERR_ARG;
+    }
+    return RF_OK;
+}
+
+// Pyramid geometry (Cudaflow.cu:146-171): w_{l+1} = (int)(w_l / fScale) in float.
+int Solver::set_geometry(const rf_params& p, int align, const char* who) {
+    Geometry g;
+    g.L = p.n_levels;
+    int nh = p.height, nw = p.width, ns = align_up(p.width, align);
+    for (int l = 0; l < p.n_levels; ++l) {
+        if (nw < 1 || nh < 1) {
+            set_error(std::string(who) + ": pyramid level " + std::to_string(l) + " has no pixels");
+            return RF_ERR_ARG;
+        }
+        g.W[l] = nw;
+        g.H[l] = nh;
+        g.S[l] = ns;
+        nh = (int)((float)nh / p.scale);
+        nw = (int)((float)nw / p.scale);
+        ns = align_up(nw, align);
+    }
+    g.n = (long)g.S[0] * g.H[0];
+    g.bufn = ((g.n + g.S[0]) + 63) / 64 * 64;
+    g_ = g;
+    return RF_OK;
+}
+
+// CudaFlow::initialize (Cudaflow.cu:99-269) and ReconFlow::_initializeR (Reconflow.cu:27-63).
+int Solver::initialize(const rf_params& p) {
+    if (int rc = check_params(p, "rf_initialize")) return rc;
     if (p.input_type < RF_INPUT_32F || p.input_type > RF_INPUT_16U) {
         set_error("rf_initialize: unknown input type");
         return RF_ERR_ARG;
@@ -112,23 +145,10 @@ int Solver::initialize(const rf_params& p) {
     release();
     g_ = Geometry();
     const int align = p.stride_alignment > 0 ? p.stride_alignment : align_;
-    // pyramid geometry (Cudaflow.cu:146-171): w_{l+1} = (int)(w_l / fScale) in float
-    Geometry g;
-    g.L = p.n_levels;
-    int nh = p.height, nw = p.width, ns = align_up(p.width, align);
-    for (int l = 0; l < p.n_levels; ++l) {
-        if (nw < 1 || nh < 1) {
-            set_error("rf_initialize: pyramid level " + std::to_string(l) + " has no pixels");
-            return RF_ERR_ARG;
-        }
-        g.W.push_back(nw);
-        g.H.push_back(nh);
-        g.S.push_back(ns);
-        nh = (int)((float)nh / p.scale);
-        nw = (int)((float)nw / p.scale);
-        ns = align_up(nw, align);
+    if (int rc = set_geometry(p, align, "rf_initialize")) {
+        g_ = Geometry();
+        return rc;
     }
-    g_ = g;
     p_ = p;
     align_ = align;
     if (p.path == RF_PATH_FUSED) {
@@ -139,16 +159,11 @@ int Solver::initialize(const rf_params& p) {
             g_ = Geometry();
             return rc;
         }
-        g_.n = (long)g_.S[0] * g_.H[0];
         inited_ = true;
         prior_ = false;
         camera_ = false;
         return RF_OK;
     }
-    const long s0 = g_.S[0], h0 = g_.H[0];
-    g_.n = s0 * h0;
-    g_.bufn = ((g_.n + s0) + 63) / 64 * 64;
-
     if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return RF_ERR_HIP;
 
     // field buffers: 2 per level (image pyramids), the fixed set, the uvrgb triple, the raw inputs,
@@ -823,6 +838,67 @@ int Solver::solve_recon(const double* R0, const double* t0, const double* R1, co
     int rc = run_schedule(1);
     uvrgb_valid_ = rc == RF_OK && p_.with_visualization;
     return rc;
+}
+
+// rf_solve_flow_noalloc: the TV-L1 solve of enqueue_flow_plain on fields carved from the caller's
+// scratch in the order rf_initialize carves them, every one zeroed first (as a fresh slab is), the
+// images read in place and the flow copied out. No allocation: no slab, parameter block or stream.
+int Solver::solve_flow_noalloc(const rf_params& p, const float* i0, const float* i1, float* u, float* v,
+                               hipStream_t st, float* scratch, size_t nfloats) {
+    const char* who = "rf_solve_flow_noalloc";
+    if (inited_ || st_) {
+        set_error(std::string(who) + ": internal: the solver must be uninitialized");
+        return RF_ERR_STATE;
+    }
+    if (int rc = check_params(p, who)) return rc;
+    if (!i0 || !i1 || !u || !v) {
+        set_error(std::string(who) + ": null image or flow pointer");
+        return RF_ERR_ARG;
+    }
+    if (p.method != RF_METHOD_TVL1 || p.input_type != RF_INPUT_32F) {
+        set_error(std::string(who) + ": runs METHOD_TVL1 on float gray images only");
+        return RF_ERR_UNSUPPORTED;
+    }
+    const int align = p.stride_alignment > 0 ? p.stride_alignment : align_;
+    if (int rc = set_geometry(p, align, who)) return rc;
+    // the fields enqueue_flow_plain uses for METHOD_TVL1: two image pyramids and 20 state fields
+    const long per = (g_.bufn + 63) / 64 * 64;
+    const long nfields = 2L * g_.L + 20;
+    if ((size_t)(nfields * per) > nfloats) {
+        set_error(std::string(who) + ": " + std::to_string(p.width) + "x" + std::to_string(p.height) + " with " +
+                  std::to_string(p.n_levels) + " levels needs " + std::to_string(nfields * per * 4) +
+                  " bytes of scratch; the scratch holds " + std::to_string(nfloats * 4));
+        return RF_ERR_ARG;
+    }
+    p_ = p;
+    p_.with_visualization = 0;
+    p_.path = RF_PATH_PLAIN;
+    p_.storage = RF_STORAGE_F32;
+    p_.use_graph = 0;
+    float* cursor = scratch;
+    auto take = [&]() {
+        float* r = cursor;
+        cursor += per;
+        return r;
+    };
+    Fields f{};
+    for (int l = 0; l < g_.L; ++l) {
+        f.pI0[l] = take();
+        f.pI1[l] = take();
+    }
+    for (float** pp : {&f.i1warp, &f.dumed, &f.dvmed, &f.dumeds, &f.dvmeds, &f.pu1, &f.pu2, &f.pv1, &f.pv2, &f.pu1s,
+                       &f.pu2s, &f.pv1s, &f.pv2s, &f.Ix, &f.Iy, &f.Iz, &f.u, &f.v, &f.us, &f.vs})
+        *pp = take();
+    f.in0 = const_cast<float*>(i0);  // read only (Cv32fToGray)
+    f.in1 = const_cast<float*>(i1);
+    st_ = st;
+    float* all[] = {scratch};
+    hipError_t e = launch_fill_zero(all, 1, nfields * per, st_);
+    if (e == hipSuccess) e = enqueue_flow_plain(f);
+    if (e == hipSuccess) e = hipMemcpyAsync(u, f.u, (size_t)g_.n * 4, hipMemcpyDeviceToDevice, st_);
+    if (e == hipSuccess) e = hipMemcpyAsync(v, f.v, (size_t)g_.n * 4, hipMemcpyDeviceToDevice, st_);
+    st_ = nullptr;  // the caller's stream: not this solver's to destroy
+    return check(e, who);
 }
 
 int Solver::sync() { return check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }

@@ -79,10 +79,14 @@ struct Profiler {
 
 struct Geometry {
     int L = 0;
-    std::vector<int> W, H, S;
+    int W[kMaxLevels] = {}, H[kMaxLevels] = {}, S[kMaxLevels] = {};
     long n = 0;     // s0 * h0
     long bufn = 0;  // floats per field buffer (n + guard row, rounded to 64)
 };
+
+// Widths and heights at or above this are rejected: the resampling kernels form texel coordinates
+// as 32-bit products such as (4 x + 2) * W.
+constexpr int kMaxSide = 32768;
 
 class Batch;
 
@@ -113,6 +117,11 @@ class Solver {
     int profile_arm(int kid, int level);
     int profile_read(double* total_ms, int* launches);
     int tiling(bool recon, int* tiling, int n);  // joint (recon) or TV-L1 tilings per level
+    // rf_solve_flow_noalloc: the plain TV-L1 solve of params on `st`, its fields carved from the
+    // caller's device scratch (nfloats floats), which it zeroes first; allocates nothing. For a
+    // Solver that was never initialized.
+    int solve_flow_noalloc(const rf_params& p, const float* i0, const float* i1, float* u, float* v,
+                           hipStream_t st, float* scratch, size_t nfloats);
     Profiler& prof() { return prof_; }
 
     const Geometry& geom() const { return g_; }
@@ -122,6 +131,8 @@ class Solver {
 
   private:
     int release();
+    int check_params(const rf_params& p, const char* who);
+    int set_geometry(const rf_params& p, int align, const char* who);
     int run_schedule(int kind);  // 0: flow, 1: joint, 2: masked flow
     hipError_t enqueue_flow_plain(Fields& f);
     hipError_t enqueue_recon_plain(Fields& f);

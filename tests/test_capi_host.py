@@ -55,6 +55,19 @@ def test_validation_errors_without_device(lib):
     assert lib.rf_initialize(ctx, C.byref(p)) == -1
     p.input_type, p.path = 0, 7
     assert lib.rf_initialize(ctx, C.byref(p)) == -1
+    # sides of 32768 or more: the resampling's 32-bit texel-coordinate products would overflow
+    for w, h in ((32768, 8), (64, 32768)):
+        lib.rf_params_default(C.byref(p), w, h)
+        assert lib.rf_initialize(ctx, C.byref(p)) == -1
+        assert b"below 32768" in lib.rf_last_error()
+    lib.rf_params_default(C.byref(p), 64, 48)
+    b = C.c_void_p()
+    W, H = (C.c_int * 2)(64, 40000), (C.c_int * 2)(48, 16)
+    assert lib.rf_batch_create(C.byref(b), C.byref(p), 2, W, H) == -1
+    assert b"below 32768" in lib.rf_last_error()
+    # the allocation-free solve validates before it looks up its scratch or launches anything
+    lib.rf_params_default(C.byref(p), 40000, 8)
+    assert lib.rf_solve_flow_noalloc(None, None, None, None, None, None) == -1
     # calls before a successful initialize report a state error
     assert lib.rf_get_stride(ctx) == -2
     assert lib.rf_solve_flow(ctx, 5.0) == -2
