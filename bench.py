@@ -17,8 +17,9 @@ GPU flow against that FP64 result, and the per-step (plain) GPU path's frames/s 
 
 --workload tvl1-320: the 320x240 TV-L1 case (fused, fused FP16, plain; EPE and time of the FP64
 oracle). --workload methods: each CudaFlow data term at 320x240 on the per-step path.
---workload stream720: consecutive pairs of a 1280x720 synthetic camera trajectory, one and eight
-pairs per batched launch: frames/s and p50/p99 latency.
+--workload stream720: consecutive pairs of a 300-frame 1280x720 synthetic camera trajectory: one
+pair per launch cold and warm-started from the previous pair's flow, eight pairs per batched launch:
+frames/s and p50/p99 latency.
 --workload batch64: 64 seeded pairs of uneven sizes (W in 160..1280, H in 120..720, multiples of
 16), even seeds TV-L1 (4 levels, 3 warps, 30 iterations; FP16 images, derivatives and duals),
 odd seeds joint (as above; FP16 derivatives, FP32 images); the two batches run on their own
@@ -58,17 +59,26 @@ FLOW_TILING = {0: "auto", 1: "64x16-rpt4", 2: "16x16", 3: "32x16", 4: "32x16-k4"
                7: "32x16-k6"}
 
 
-def make_solver(rf, path, graph, tri, joint_tile=0):
+def make_solver(rf, path, graph, tri, joint_tile=0, fix=0):
+    """ReconFlow with the demo's parameters; every solve ends with the flow's colour map, as the
+    reference's solveReconFlow does (Reconflow.cu:460)."""
     r = rf.ReconFlow()
     r._initializeR(W, H, 1, LEVELS, 2.0, rf.ReconFlow.METHODR_TVL1_MS, 50.0, 0.0, 0.1, 100.0, 33.3, 60.0, 0.125,
-                   WARPS, ITERS, cvType=rf.INPUT_32F, path=path, use_graph=graph)
+                   WARPS, ITERS, cvType=rf.INPUT_32F, path=path, use_graph=graph, withVisualization=True)
     r.params.tri_f64 = int(tri)
     r.params.joint_tile = int(joint_tile)
+    r.params.fix_coarse_k = int(fix)
     r._init_params(r.params)
     return r
 
 
 REF64 = {}  # the FP64 oracle's flow of the single-pair workload, kept for the batched EPE
+
+
+def depth_rel_err(Z, Z64):
+    zr = np.abs(Z[:, :W] - Z64[:, :W]) / np.abs(Z64[:, :W])
+    zr = zr[np.isfinite(zr)]
+    return {"median": float(f"{np.median(zr):.3g}"), "p99": float(f"{np.percentile(zr, 99):.3g}")}
 
 
 def cpu_baseline(d, u, v):
@@ -115,6 +125,14 @@ def _fetch(dptr, n):
     return out
 
 
+def _copy_async(dst, src, nbytes, stream):
+    """hipMemcpyAsync device to device on `stream` (raw addresses)."""
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMemcpyAsync.restype = ctypes.c_int
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    assert hip.hipMemcpyAsync(dst, src, nbytes, 3, stream) == 0
+
+
 def run_joint_batched(args, rf, torch, n, storage):
     """n seeded 640x480 pairs (pair 0 is the single-pair workload's scene) in one batch: every
     launch covers the tiles of all n pairs. Returns frames/s and pair 0's flow."""
@@ -144,6 +162,60 @@ def run_joint_batched(args, rf, torch, n, storage):
     tiling = [TILING.get(c, str(c)) for c in b.tiling(LEVELS)]
     del b, keep
     return n * steps / dt, u, v, tiling
+
+
+def run_drop_in_cycle(args, rf, d):
+    """The reference demo's call cycle from host memory (test_reconflow/main.cpp:100-106):
+    copyImagesToDevice from pageable host images, solveR, copyOpticalFlowToHost(u, v, uvrgb) and
+    copy3dToHost into host buffers, per frame."""
+    r = make_solver(rf, rf.PATH_FUSED if args.path == "fused" else rf.PATH_PLAIN, not args.no_graph, args.tri_f64)
+    r.setCameraMatrices(d["K"], d["K"])
+
+    def cycle():
+        r.copyImagesToDevice(d["i0"], d["i1"])
+        r.solveR(d["R"], d["t"], 50.0)
+        r.copyOpticalFlowToHost(with_rgb=True)
+        r.copy3dToHost()
+
+    for _ in range(max(1, args.warmup)):
+        cycle()
+    steps = max(5, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cycle()
+    dt = (time.perf_counter() - t0) / steps
+    return {"frames_per_s": round(1.0 / dt, 2), "ms_per_frame": round(dt * 1e3, 3), "frames": steps,
+            "cycle": "copyImagesToDevice (host) + solveR + copyOpticalFlowToHost(u, v, uvrgb) + copy3dToHost"}
+
+
+def run_fixed_k(args, rf, d, path):
+    """The same solve with K[2][2] = 1 on the coarse levels (the reference's setCameraMatrices leaves
+    it 0 there, Reconflow.cu:99-110, which makes the coarse triangulation degenerate): frames/s, its
+    mean EPE against the FP64 oracle of that mode, and both modes' median EPE against the generator's
+    ground-truth flow."""
+    r = make_solver(rf, path, not args.no_graph, args.tri_f64, fix=1)
+    r.setCameraMatrices(d["K"], d["K"])
+    r.copyImagesToDevice(d["i0"], d["i1"])
+    dt = _time(lambda: r.solveR(d["R"], d["t"], 50.0), r.synchronize, max(5, args.steps // 2), max(1, args.warmup))
+    u, v = r.copyOpticalFlowToHost()
+    out = {"frames_per_s": round(1.0 / dt, 2)}
+    gt = lambda a, b: float(np.median(np.hypot(a[:, :W] - d["gt_u"][:, :W], b[:, :W] - d["gt_v"][:, :W])))  # noqa
+    out["median_epe_vs_ground_truth"] = round(gt(u, v), 4)
+    if "joint" in REF64:
+        out["reference_semantics_median_epe_vs_ground_truth"] = round(gt(REF64["joint"][0], REF64["joint"][1]), 3)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        ref = orc.solve_recon(orc.recon_params(W, H, LEVELS, fix_coarse_k=1), d["K"], d["K"], np.eye(3), np.zeros(3),
+                              d["R"], d["t"], d["i0"], d["i1"], "f64")
+        e = np.hypot(u[:, :W] - ref[0][:, :W], v[:, :W] - ref[1][:, :W])
+        out["epe_vs_fp64_oracle"] = round(float(e.mean()), 5)
+        out["median_epe_vs_fp64_oracle"] = round(float(np.median(e)), 6)
+    return out
+
+
+def _sub(line, keys):
+    """A workload's line as a sub-object of the default line."""
+    return {k: line[k] for k in keys if k in line}
 
 
 def run_joint(args, rf, torch):
@@ -220,7 +292,11 @@ def run_joint(args, rf, torch):
         plain_fps = n_plain / (time.perf_counter() - t1)
         del rp
 
+    X, Y, Z = r.copy3dToHost()
     cpu, epe = (None, None) if args.no_cpu_baseline else cpu_baseline(d, u, v)
+    zerr = depth_rel_err(Z, REF64["joint"][4]) if "joint" in REF64 else None
+    cycle = None if args.no_sub else run_drop_in_cycle(args, rf, d)
+    fixed = None if args.no_sub else run_fixed_k(args, rf, d, path)
     batched = None
     if path == rf.PATH_FUSED and args.batch_pairs > 0:
         batched = {"pairs_per_launch": args.batch_pairs}
@@ -256,7 +332,10 @@ def run_joint(args, rf, torch):
                      "algorithmic_bytes_per_launch": bytes_launch, "avg_iteration_us_per_level": per_level_us},
         "cpu_baseline": cpu,
         "epe_vs_fp64_oracle": None if epe is None else round(epe, 5),
+        "depth_rel_err_vs_fp64_oracle": zerr,
         "accuracy_ok": None if epe is None else bool(epe <= 0.05),
+        "drop_in_cycle": cycle,
+        "fixed_coarse_k": fixed,
         "plain_fp32_path_fps": None if plain_fps is None else round(plain_fps, 3),
         "speedup_vs_plain_path": None if plain_fps is None else round(fps / plain_fps, 3),
         "mpix_iter_per_s": round(fps * px_iter / 1e6, 1),
@@ -404,10 +483,13 @@ def run_tvl1(args, rf, torch):
 
 def run_stream(args, rf, torch):
     """Streaming 1280x720 joint flow+depth: consecutive pairs of a synthetic camera trajectory
-    (per-frame rotation <= 0.5 deg, camera displacement <= 0.02, 16 planes at depths 2-20), each
-    solved cold (5 levels x 5 warps x 50 iterations), B consecutive pairs per batched launch. A
-    frame's result is ready when its batch completes: its latency is the batch's time from
-    submission to completion (waiting for the batch's later frames to arrive is not counted)."""
+    (per-frame rotation <= 0.5 deg, camera displacement <= 0.02, 16 planes at depths 2-20), 5 levels x
+    5 warps x 50 iterations, FP64 mode, K[2][2] = 1 on the coarse levels (with the reference's
+    degenerate coarse cameras the joint flow is not a usable warm start). Runs: one pair per launch
+    cold; one pair per launch, each pair warm-started from a device copy of the previous pair's flow
+    (rf_batch_set_warm_start_device; the rule falls back to a cold start when that flow is not usable);
+    eight consecutive pairs per batched launch, cold. A frame's latency is its launch's time from
+    submission to completion (waiting for a batch's later frames is not counted)."""
     w, h, n = 1280, 720, args.stream_frames
     t_gen = time.perf_counter()
     seq = rf.make_sequence(w, h, n, args.seed)
@@ -415,26 +497,35 @@ def run_stream(args, rf, torch):
     frames = torch.from_numpy(seq["frames"]).cuda()
     K, R, t = seq["K"], seq["R"], seq["t"]
     runs, pair0 = {}, None
-    for name, B, storage in (("1_pair_f32", 1, rf.STORAGE_F32), ("8_pairs_f32", 8, rf.STORAGE_F32),
-                             ("8_pairs_f16_derivatives", 8, rf.STORAGE_F16_DERIV)):
+    for name, B, storage, warm in (("1_pair_cold_f16_derivatives", 1, rf.STORAGE_F16_DERIV, False),
+                                   ("1_pair_warm_started_f16_derivatives", 1, rf.STORAGE_F16_DERIV, True),
+                                   ("8_pairs_cold_f16_derivatives", 8, rf.STORAGE_F16_DERIV, False)):
         p = rf.batch_params(14, LEVELS, WARPS, ITERS, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0,
-                            lambdaf=0.1, lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage)
+                            lambdaf=0.1, lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage, fix_coarse_k=1)
         b = rf.Batch(p, [w] * B, [h] * B)
+        st = b.stride(0)
+        wsu = torch.zeros(h * st, dtype=torch.float32, device="cuda")
+        wsv = torch.zeros_like(wsu)
 
         def submit(k0):
             for i in range(B):
                 b.set_images_device(i, frames[k0 + i].data_ptr(), frames[k0 + i + 1].data_ptr())
                 b.set_camera(i, K, K, R[k0 + i], t[k0 + i])
             b.solve()
+            if warm:  # the next pair's warm start: this pair's flow, copied on the batch stream
+                u_ptr, v_ptr = b.result_ptrs(0)[:2]
+                _copy_async(wsu.data_ptr(), u_ptr, h * st * 4, b.stream())
+                _copy_async(wsv.data_ptr(), v_ptr, h * st * 4, b.stream())
 
         submit(0)  # the first solve of a batch includes the tiling tuning solves
         b.synchronize()
         if pair0 is None:
-            st = b.stride(0)
             u_ptr, v_ptr = b.result_ptrs(0)[:2]
             pair0 = (_fetch(u_ptr, h * st).reshape(h, st), _fetch(v_ptr, h * st).reshape(h, st))
+        if warm:
+            b.set_warm_start_device(0, wsu.data_ptr(), wsv.data_ptr())
         npairs = ((n - 1) // B) * B
-        lat = []
+        lat, used, nonfinite = [], 0, 0
         t0 = time.perf_counter()
         for k0 in range(0, npairs, B):
             ts = time.perf_counter()
@@ -442,27 +533,40 @@ def run_stream(args, rf, torch):
             b.synchronize()
             lat.append(time.perf_counter() - ts)
         dt = time.perf_counter() - t0
+        if warm:  # one more pass, untimed, for the rule's decisions and the results' finiteness
+            submit(0)
+            b.synchronize()
+            for k0 in range(1, npairs):
+                submit(k0)
+                b.synchronize()
+                used += int(b.warm_start_in_use(0))
+                if not np.isfinite(_fetch(b.result_ptrs(0)[0], h * st)).all():
+                    nonfinite += 1
         runs[name] = {"frames_per_s": round(npairs / dt, 2), "pairs": npairs,
                       "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                       "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
                       "joint_tiling_per_level": [TILING.get(c, str(c)) for c in b.tiling(LEVELS)]}
+        if warm:
+            runs[name].update({"warm_start_used": used, "pairs_with_nonfinite_flow": nonfinite})
         del b
     epe = None
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
         f = seq["frames"]
-        ref = orc.solve_recon(orc.recon_params(w, h, LEVELS), K, K, np.eye(3), np.zeros(3), R[0], t[0], f[0], f[1],
-                              "f64")
-        epe = float(np.hypot(pair0[0][:, :w] - ref[0][:, :w], pair0[1][:, :w] - ref[1][:, :w]).mean())
-    best = runs["8_pairs_f16_derivatives"]
+        ref = orc.solve_recon(orc.recon_params(w, h, LEVELS, fix_coarse_k=1), K, K, np.eye(3), np.zeros(3), R[0],
+                              t[0], f[0], f[1], "f64")
+        e = np.hypot(pair0[0][:, :w] - ref[0][:, :w], pair0[1][:, :w] - ref[1][:, :w])
+        epe = {"mean": round(float(e.mean()), 5), "median": round(float(np.median(e)), 6)}
+    best = runs["1_pair_warm_started_f16_derivatives"]
     return {
-        "metric": "1280x720 streaming joint flow+depth frames/s (consecutive pairs, 8 per batched launch)",
-        "value": best["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "metric": "1280x720 streaming joint flow+depth frames/s (consecutive pairs, warm-started from the previous "
+                  "pair, one pair per launch)",
+        "value": best["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": best["pairs"], "warmup": 1,
         "ms_per_step": round(1e3 / best["frames_per_s"], 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 updates, f16 derivatives" + (", f64 triangulation" if args.tri_f64 else ""),
+        "vs_baseline": None, "dtype": "f32 updates, f16 derivatives" + (", f64 mode" if args.tri_f64 else ""),
         "data": f"synthetic {n}-frame camera trajectory over a piecewise-planar scene (generated in {t_gen:.1f} s)",
-        "config": {"workload": "stream 1280x720, 5 levels x 5 warps x 50 iterations, cold start per pair"},
-        "runs": runs, "epe_pair0_vs_fp64_oracle": None if epe is None else round(epe, 5), "cpu_baseline": None,
+        "config": {"workload": "stream 1280x720, 5 levels x 5 warps x 50 iterations, K[2][2] = 1 on coarse levels"},
+        "runs": runs, "epe_pair0_vs_fp64_oracle": epe, "cpu_baseline": None,
     }
 
 
@@ -507,7 +611,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["joint640", "batch64", "tvl1-320", "methods", "stream720"],
                     default="joint640")
-    ap.add_argument("--stream-frames", type=int, default=49, help="stream720: frames of the sequence")
+    ap.add_argument("--stream-frames", type=int, default=300, help="stream720: frames of the sequence")
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--batch-pairs", type=int, default=8,
                     help="joint640: also time this many 640x480 pairs per batched launch (0: skip)")
@@ -519,6 +623,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain-compare", action="store_true")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="joint640: skip the drop-in cycle, fixed-intrinsics, TV-L1 320x240 and 64-pair sub-objects")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (scales the FETCH_SIZE / WRITE_SIZE "
                          "counters to bytes) and exit")
@@ -544,6 +650,13 @@ def main() -> int:
     run = {"batch64": run_batch, "tvl1-320": run_tvl1, "methods": run_methods, "stream720": run_stream}.get(
         args.workload, run_joint)
     line = run(args, rf, torch)
+    if run is run_joint and not args.no_sub:
+        t = run_tvl1(args, rf, torch)
+        line["tvl1_320"] = _sub(t, ("metric", "value", "unit", "ms_per_step", "frames_per_s", "mpix_iter_per_s",
+                                    "epe_vs_fp64_oracle", "flow_tiling_per_level", "cpu_baseline"))
+        b = run_batch(args, rf, torch)
+        line["batch64"] = _sub(b, ("metric", "value", "unit", "ms_per_step", "dtype", "config", "mpix_iter_per_s",
+                                   "algorithmic_gb_per_s_fp32_equivalent", "tiling_per_level", "roofline"))
     print(json.dumps(line))
     return 0
 

@@ -300,7 +300,8 @@ int rf_batch_get_result_device(const rf_batch* b, int i, const float** u, const 
 void* rf_batch_get_stream(const rf_batch* b);
 /* Warm-start input of pair i (no counterpart in the reference; the rule of rf_copy_warm_start):
  * device flow (stride_i x height_i floats each), read by every solve until replaced, for example
- * the previous frame's result of a stream; NULL clears. rf_batch_warm_start_in_use: 1 when the
+ * a copy of the previous frame's result in a stream; NULL clears. It may not point into the
+ * batch's own fields (a solve clears them before reading it): RF_ERR_ARG. rf_batch_warm_start_in_use: 1 when the
  * last solve used it, 0 otherwise (synchronises the batch stream). */
 int rf_batch_set_warm_start_device(rf_batch* b, int i, const float* d_u, const float* d_v);
 int rf_batch_warm_start_in_use(rf_batch* b, int i);

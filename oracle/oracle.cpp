@@ -1173,7 +1173,36 @@ struct Params {
     //               arithmetic carries the FP32 solve's distance from the FP64 one.
     int f32_steps;
     int proj_f64;  // PREC_* bits: the precise-projection mode (for FP32 fields)
+    // FP16 storage of the fused GPU path (rfhip.h RF_STORAGE_*): 1 = images, derivatives and the
+    // TV-L1 duals in binary16; 2 = derivatives and the TV-L1 duals. Every store of such a field is
+    // rounded to nearest-even binary16 and widened exactly on load; updates stay in T.
+    int storage;
 };
+
+// Round to the nearest binary16 value (ties to even), returned as a float: quantum q of the value's
+// binade (2^-24 below the smallest normal 2^-14), rounding by the float addition of C = q 2^23 (whose
+// float ulp is q) and its exact subtraction; 65520 and above (the tie above the largest finite
+// binary16, 65504) go to infinity; NaN and infinities pass through.
+static inline float round_half(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return f;
+    const float a = std::fabs(f);
+    if (a >= 65520.0f) return std::copysign(std::numeric_limits<float>::infinity(), f);
+    const int e = (int)(ax >> 23) - 127;
+    const float C = std::ldexp(1.0f, (e < -14 ? -24 : e - 10) + 23);
+    volatile float t = a + C;  // one float rounding, to a multiple of q
+    const float r = t - C;
+    return std::copysign(r, f);
+}
+
+template <typename T>
+static void round_f16(int n_or_zero, long n, std::initializer_list<T*> bufs) {
+    if (!n_or_zero) return;
+    for (T* b : bufs)
+        for (long i = 0; i < n; ++i) b[i] = (T)round_half((float)b[i]);
+}
 
 // Steps of Params::f32_steps (joint solve).
 enum {
@@ -1379,6 +1408,8 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
     gray_input(P, i1, pI1[0], s0);
     round_group<T>(P, RND_IMG, n, {pI0[0], pI1[0]});
     const int mut = P.mutation;
+    const int img16 = P.storage == 1, der16 = P.storage == 1 || P.storage == 2;  // the TV-L1 duals: der16
+    round_f16<T>(img16, n, {pI0[0], pI1[0]});
     if (cg) {
         deriv_single(pI0[0], pW[0], pH[0], s0, pIx0[0], pIy0[0]);
         deriv_single(pI1[0], pW[0], pH[0], s0, pIx1[0], pIy1[0]);
@@ -1391,6 +1422,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
         down(pI0);
         down(pI1);
         round_group<T>(P, RND_IMG, n, {pI0[l], pI1[l]});
+        round_f16<T>(img16, n, {pI0[l], pI1[l]});
         if (cg) {
             down(pIx0);
             down(pIx1);
@@ -1413,6 +1445,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
             else
                 warp(pI1[level], w, h, s, u, v, i1warp);
             round_group<T>(P, RND_IMG, n, {i1warp});
+            round_f16<T>(img16, n, {i1warp});
             if (cg) {
                 warp(pIx1[level], w, h, s, u, v, ix1warp);
                 warp(pIy1[level], w, h, s, u, v, iy1warp);
@@ -1422,6 +1455,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
             else
                 derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
             round_group<T>(P, RND_DER, n, {Ix, Iy, Iz});
+            round_f16<T>(der16, n, {Ix, Iy, Iz});
             if (cg) {
                 derivatives(pIx0[level], ix1warp, w, h, s, Ixx, Ixy, Ixz);
                 derivatives(pIy0[level], iy1warp, w, h, s, Iyx, Iyy, Iyz);
@@ -1451,6 +1485,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
                     std::swap(pv2, pv2s);
                 }
                 round_group<T>(P, RND_DUAL, n, {pu1, pu2, pv1, pv2});
+                round_f16<T>(der16, n, {pu1, pu2, pv1, pv2});
             }
             if (mut != MUT_FLOW_NO_MEDIAN) {
                 median5(dumed, dvmed, w, h, s, dumeds, dvmeds);
@@ -1556,16 +1591,19 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
     k_pyramid(K1, P.n_levels, P.scale, P.fix_coarse_k, pK1);
 
     const int mut = P.mutation;
+    const int img16 = P.storage == 1, der16 = P.storage == 1 || P.storage == 2;  // joint duals stay in T
     // binary64 blends: the FP64 mode (tri_f64) takes the projection and data blends along with the
     // triangulation; proj_f64 selects them one by one. The FP64 solve computes in binary64 anyway.
     const int prec = sizeof(T) == sizeof(float) ? (P.proj_f64 | (P.tri_f64 ? PREC_PROJ | PREC_BLEND : 0)) : 0;
     gray_input(P, i0, pI0[0], s0);
     gray_input(P, i1, pI1[0], s0);
     round_group<T>(P, RND_IMG, n, {pI0[0], pI1[0]});
+    round_f16<T>(img16, n, {pI0[0], pI1[0]});
     for (int l = 1; l < P.n_levels; ++l) {
         downscale(pI0[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI0[l]);
         downscale(pI1[l - 1], pW[l - 1], pH[l - 1], pS[l - 1], pW[l], pH[l], pS[l], T(1), false, pI1[l]);
         round_group<T>(P, RND_IMG, n, {pI0[l], pI1[l]});
+        round_f16<T>(img16, n, {pI0[l], pI1[l]});
     }
     for (T* b : {u, v, uproj, vproj, uc, vc, X, Y, Z, sku, skv}) zero(b, n);
     apply_warm_start(wsu, wsv, pW, pH, pS, pool, u, v);
@@ -1602,6 +1640,7 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                 warp(pI1[level], w, h, s, u, v, i1warp);
             }
             round_group<T>(P, RND_IMG, n, {i1warp});
+            round_f16<T>(img16, n, {i1warp});
             if (a32(STEP_DERIV)) {
                 derivatives<float>(q.in(0, pI0[level]), q.in(1, i1warp), w, h, s, q.in(2, Ix), q.in(3, Iy),
                                    q.in(4, Iz));
@@ -1612,6 +1651,7 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                 derivatives(pI0[level], i1warp, w, h, s, Ix, Iy, Iz);
             }
             round_group<T>(P, RND_DER, n, {Ix, Iy, Iz});
+            round_f16<T>(der16, n, {Ix, Iy, Iz});
             for (int it = 0; it < P.n_solver_iters; ++it) {
                 if (fn && (prec & PREC_BLEND))
                     recon_data_l1_fn_wide(u, v, dumed, dvmed, uproj, vproj, ufn_l, vfn_l, sku, skv, pu1, pu2, pv1,
@@ -1970,6 +2010,11 @@ void orc_pyramid(const orc::Params* P, int* W, int* H, int* S) {
 }
 
 int orc_params_size(void) { return (int)sizeof(orc::Params); }
+
+// The binary16 rounding of the FP16 storage modes, elementwise (for its pin).
+void orc_round_half(const float* in, float* out, long n) {
+    for (long i = 0; i < n; ++i) out[i] = orc::round_half(in[i]);
+}
 
 // The per-level intrinsics and camera matrices the joint solve uses (setCameraMatrices,
 // Reconflow.cu:78-116, and K*[R|t], Reconflow.cu:225-228): Kl (n_levels x 9) and Pl (n_levels x 12).

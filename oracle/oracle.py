@@ -26,7 +26,7 @@ class Params(C.Structure):
         ("lambdams", C.c_float), ("alpha_tv", C.c_float), ("alpha_proj", C.c_float), ("alpha_fn", C.c_float),
         ("stride_alignment", C.c_int), ("input_type", C.c_int), ("fix_coarse_k", C.c_int),
         ("tri_f64", C.c_int), ("mutation", C.c_int), ("round_mask", C.c_int),
-        ("f32_steps", C.c_int), ("proj_f64", C.c_int),
+        ("f32_steps", C.c_int), ("proj_f64", C.c_int), ("storage", C.c_int),
     ]
 
 
@@ -178,6 +178,7 @@ _SOLVE_SIGS = {
     "orc_pyramid": ([_PP, _VP, _VP, _VP], None),
     "orc_warm_start_usable": ([_VP, _VP, _I, _I, _I], _I),
     "orc_camera_pyramid": ([_VP, _VP, _VP, _I, _F, _I, _VP, _VP], None),
+    "orc_round_half": ([_VP, _VP, C.c_long], None),
     "orc_solve_flow_f64": ([_PP] + [_VP] * 6, _I), "orc_solve_flow_f32": ([_PP] + [_VP] * 6, _I),
     "orc_solve_flow_masked_f64": ([_PP] + [_VP] * 5, _I), "orc_solve_flow_masked_f32": ([_PP] + [_VP] * 5, _I),
     "orc_solve_recon_f64": ([_PP] + [_VP] * 18, _I), "orc_solve_recon_f32": ([_PP] + [_VP] * 18, _I),
@@ -260,6 +261,15 @@ def _ws(ws, p: Params, s0: int):
     return [_p(a) for a in keep], keep
 
 
+def round_half(a) -> np.ndarray:
+    """The FP16 storage modes' rounding: float32 values to the nearest binary16 (ties to even), as
+    float32."""
+    x = np.ascontiguousarray(a, np.float32)
+    out = np.empty_like(x)
+    _sym("orc_round_half")(_p(x), _p(out), x.size)
+    return out
+
+
 def warm_start_usable(wsu, wsv, w, h) -> bool:
     """The warm-start rule: every component over the w x h pixels finite and at most max(w, h)."""
     a, b = (np.ascontiguousarray(x, np.float32) for x in (wsu, wsv))
@@ -314,22 +324,23 @@ def solve_recon(p: Params, K0, K1, R0, t0, R1, t1, i0, i1, kind="f64", ufn=None,
 
 
 def flow_params(w, h, n_levels, scale=2.0, n_warp=3, n_iter=30, lam=50.0, theta=0.3, tau=0.25, input_type=0,
-                method=0, lambdagrad=400.0):
+                method=0, lambdagrad=400.0, storage=0):
     p = Params()
     p.width, p.height, p.n_levels, p.scale, p.method = w, h, n_levels, scale, method
     p.lambda_, p.lambdagrad, p.theta, p.tau = lam, lambdagrad, theta, tau
     p.n_warp_iters, p.n_solver_iters = n_warp, n_iter
-    p.stride_alignment, p.input_type = 32, input_type
+    p.stride_alignment, p.input_type, p.storage = 32, input_type, storage
     return p
 
 
 def recon_params(w, h, n_levels, scale=2.0, n_warp=5, n_iter=50, lam=50.0, tau=0.125, alpha_tv=33.3,
                  alpha_proj=60.0, lambdaf=0.1, lambdams=100.0, alpha_fn=100.0, method=14, input_type=0,
-                 fix_coarse_k=0, tri_f64=0):
+                 fix_coarse_k=0, tri_f64=0, storage=0):
     p = Params()
     p.width, p.height, p.n_levels, p.scale, p.method = w, h, n_levels, scale, method
     p.lambda_, p.lambdagrad, p.theta, p.tau = lam, 0.0, float(np.float32(1.0) / np.float32(alpha_tv)), tau
     p.n_warp_iters, p.n_solver_iters = n_warp, n_iter
     p.lambdaf, p.lambdams, p.alpha_tv, p.alpha_proj, p.alpha_fn = lambdaf, lambdams, alpha_tv, alpha_proj, alpha_fn
     p.stride_alignment, p.input_type, p.fix_coarse_k, p.tri_f64 = 32, input_type, fix_coarse_k, tri_f64
+    p.storage = storage
     return p

@@ -381,7 +381,8 @@ class Batch:
 
 def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50.0, theta=0.3, tau=0.25,
                  alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1, lambdams=100.0, use_graph=True, tri_f64=0,
-                 storage=STORAGE_F32, joint_tile=0, flow_tile=0, with_visualization=False) -> RFParams:
+                 storage=STORAGE_F32, joint_tile=0, flow_tile=0, with_visualization=False,
+                 fix_coarse_k=0) -> RFParams:
     """Parameters for a Batch: TV-L1 flow (method 0) or the joint flow/depth solve (method 14),
     float gray input, scale 2 pyramid."""
     lib = _native.lib()
@@ -394,6 +395,7 @@ def batch_params(method: int, n_levels: int, n_warp: int, n_iter: int, *, lam=50
     p.path, p.use_graph, p.tri_f64, p.storage = PATH_FUSED, int(use_graph), tri_f64, storage
     p.joint_tile, p.flow_tile = joint_tile, flow_tile
     p.with_visualization = int(bool(with_visualization))
+    p.fix_coarse_k = int(fix_coarse_k)
     return p
 
 

@@ -1351,6 +1351,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
     bool same = n > 1 && (long long)per_fields * geo_[0].bufn * n < (1LL << 30);
     for (int i = 1; i < n && same; ++i) same = geo_[i].W == geo_[0].W && geo_[i].H == geo_[0].H && geo_[i].S == geo_[0].S;
     if (same) pair_elems_ = (unsigned)(per_fields * geo_[0].bufn);
+    slab_bytes_ = total;
     if (hipMalloc(&slab_, total) != hipSuccess) {
         slab_ = nullptr;
         set_error("rf_batch_create: device allocation of " + std::to_string(total) + " bytes failed");
@@ -1712,6 +1713,15 @@ int Batch::set_warm_start(int i, const float* u, const float* v) {
         return RF_ERR_ARG;
     }
     if (!u || !v) u = v = nullptr;
+    // a solve clears the batch's state fields before it reads the warm start, so the warm start
+    // may not live in them (a result handed straight back, for one): copy it out first
+    const char *b0 = (const char*)slab_, *b1 = b0 + slab_bytes_;
+    for (const float* q : {u, v})
+        if (q && (const char*)q >= b0 && (const char*)q < b1) {
+            set_error("rf_batch_set_warm_start_device: the warm start lies in the batch's own fields, which a solve "
+                      "clears first; copy it to a buffer of its own");
+            return RF_ERR_ARG;
+        }
     if ((u != nullptr) != (wsu_[i] != nullptr) || u != wsu_[i] || v != wsv_[i]) drop_graphs();
     wsu_[i] = u;
     wsv_[i] = v;

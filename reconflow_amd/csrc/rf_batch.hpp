@@ -124,6 +124,7 @@ class Batch {
     std::vector<PairFields> hp_;
     PairFields* dp_ = nullptr;
     void* slab_ = nullptr;
+    size_t slab_bytes_ = 0;
     hipStream_t st_ = nullptr;
     bool prior_ = false, sparse_ = false;
     float flow_scale_ = 5.0f;

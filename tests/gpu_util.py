@@ -53,6 +53,15 @@ def fetch(dptr_value, n):
     return out
 
 
+def copy_async(dst, src, nbytes, stream):
+    """hipMemcpyAsync device to device of nbytes on `stream` (raw addresses)."""
+    if _hip is None:
+        fetch(0, 0)  # binds the runtime
+    _hip.hipMemcpyAsync.restype = C.c_int
+    _hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    return _hip.hipMemcpyAsync(dst, src, nbytes, 3, stream)  # 3 = device to device
+
+
 def dptr(a):
     a = np.ascontiguousarray(np.asarray(a, np.float64).ravel())
     return a.ctypes.data_as(C.POINTER(C.c_double)), a

@@ -406,3 +406,49 @@ def test_composition_check_sees_every_planted_mistake(orc, name):
     else:
         p, d = _recon_case(orc, orc.MUT[name])
         assert not _eq(_solve_recon(orc, p, d), compose_recon(orc, p, d["K"], d["R"], d["t"], d["i0"], d["i1"])), name
+
+
+# ------------------------------------------------------------------ FP16 storage rounding
+def test_round_half_is_numpy_float16(orc):
+    """The FP16 storage modes round every stored value to the nearest binary16 (ties to even,
+    subnormals, overflow to infinity): numpy's float32 -> float16 cast on random bit patterns, a
+    dense range and the binade, subnormal and overflow boundaries with their neighbours."""
+    rng = np.random.default_rng(16)
+    x = rng.integers(0, 2 ** 32, 500_000, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    edges = np.array([65504, 65520, 65536, 2.0 ** -14, 2.0 ** -24, 2.0 ** -25, 1.0, 1.0 + 2.0 ** -11, 0.0, -0.0,
+                      np.inf, -np.inf], np.float32)
+    x = np.concatenate([x, edges, np.nextafter(edges, np.float32(np.inf)), np.nextafter(edges, np.float32(-np.inf)),
+                        np.linspace(-70000, 70000, 200_001, dtype=np.float32)])
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).astype(np.float32)
+    got = orc.round_half(x)
+    same = (got == ref) & (np.signbit(got) == np.signbit(ref)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), x[~same][:5]
+
+
+def test_fp16_derivative_storage_rounds_the_derivatives(orc):
+    """One level, one warp, one iteration in the derivatives-and-duals FP16 mode: the solve equals
+    the per-kernel chain with the derivatives rounded to binary16 before the data step (warp,
+    derivatives, rounding, data step from zero state, 5x5 median, add), and differs from the FP32
+    mode's solve."""
+    w, h = 64, 48
+    i0, i1, _, _ = rf.make_flow_pair(w, h, 3)
+    s = i0.shape[1]
+    n = (h + 1) * s
+    f = lambda a: np.concatenate([a.ravel(), np.zeros(s, np.float32)])  # noqa: E731
+    I0, I1 = f(i0), f(i1)
+    z = np.zeros(n, np.float32)
+    warped, Ix, Iy, Iz, du, dv, mu, mv = (np.zeros(n, np.float32) for _ in range(8))
+    orc.call("warp", "f32", I1, w, h, s, z, z, warped)
+    orc.call("derivatives", "f32", I0, warped, w, h, s, Ix, Iy, Iz)
+    Ix, Iy, Iz = (orc.round_half(a) for a in (Ix, Iy, Iz))
+    orc.call("data_l1", "f32", z, z, z, z, z, z, Ix, Iy, Iz, w, h, s, 50.0, 0.3, du, dv)
+    orc.call("median5", "f32", du, dv, w, h, s, mu, mv)
+    u, v = z.copy(), z.copy()
+    orc.call("add", "f32", u, mu, h * s, u)
+    orc.call("add", "f32", v, mv, h * s, v)
+    got = orc.solve_flow(orc.flow_params(w, h, 1, n_warp=1, n_iter=1, storage=2), i0, i1, "f32")
+    assert np.array_equal(got[0][:, :w], u[:h * s].reshape(h, s)[:, :w])
+    assert np.array_equal(got[1][:, :w], v[:h * s].reshape(h, s)[:, :w])
+    plain = orc.solve_flow(orc.flow_params(w, h, 1, n_warp=1, n_iter=1), i0, i1, "f32")
+    assert not np.array_equal(plain[0][:, :w], got[0][:, :w])
