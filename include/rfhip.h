@@ -309,6 +309,41 @@ int rf_batch_warm_start_in_use(rf_batch* b, int i);
  * RF_FLOW_TILE_* for a TV-L1 batch (AUTO until the tuning solves ran). */
 int rf_batch_tiling(const rf_batch* b, int* tiling, int n);
 
+/* ---- learned warm start (no counterpart in the reference) ----
+ * A six-layer 3x3 convolutional encoder-decoder (32 channels, BF16 weights and activations on the
+ * matrix cores, FP32 accumulation) that predicts the flow at pyramid level 3 (1/8 of the image) from
+ * levels 3 and 4 of both frames; its prediction, scaled up to level 0, is a warm start for
+ * rf_copy_warm_start / rf_batch_set_warm_start_device, which lets a solve start at a finer level.
+ * The reference's own prior is a flow computed outside (the FlowNet input of test_reconflow/main.cpp:
+ * 82-101, consumed by SolveReconDataL1Fn); this network is trained on the joint solver's results.
+ * Parameters are FP32 masters, flat: per layer the weights [9][cout][cin] then the bias [cout]. */
+typedef struct rf_warmnet rf_warmnet;
+/* height x width: the level-3 size (even, at most 4096); max_batch: samples per forward / step. */
+int rf_warmnet_create(rf_warmnet** out, int max_batch, int height, int width);
+int rf_warmnet_destroy(rf_warmnet* net);
+int rf_warmnet_param_count(const rf_warmnet* net);
+/* Host FP32 parameters in / out (setting them resets the optimizer state). Synchronous. */
+int rf_warmnet_set_params(rf_warmnet* net, const float* params, int count);
+int rf_warmnet_get_params(rf_warmnet* net, float* params, int count);
+/* The gradients of the last rf_warmnet_train_step (FP32, the parameter layout). Synchronous. */
+int rf_warmnet_get_grads(rf_warmnet* net, float* grads, int count);
+/* The network input of n samples from device level images: level 3 (height x width, stride s3,
+ * nstride3 floats between samples) and level 4 (stride s4, nstride4) of frames 0 and 1. */
+int rf_warmnet_input(rf_warmnet* net, const float* d_i0_l3, const float* d_i1_l3, int s3, long nstride3,
+                     const float* d_i0_l4, const float* d_i1_l4, int s4, long nstride4, int n, void* stream);
+/* Forward pass of the last input: the predicted level-3 flow (n x height x width x 2 floats, u v
+ * interleaved, level-3 pixels) into d_flow (may be NULL). */
+int rf_warmnet_forward(rf_warmnet* net, int n, float* d_flow, void* stream);
+/* One training step on the last input against d_target (n x height x width x 2; non-finite entries
+ * are ignored): Huber loss (delta 1 px, mean), gradients, and with lr > 0 an Adam step (beta 0.9,
+ * 0.999). *loss (may be NULL) receives the loss; then the call synchronises the stream. */
+int rf_warmnet_train_step(rf_warmnet* net, int n, const float* d_target, float lr, float* loss, void* stream);
+/* The warm start of one pair: level-0 float images (w0 x h0, stride s0) -> pyramid (the solver's
+ * downscale) -> the network's level-3 flow -> the solver's upscale to level 0 (d_u, d_v, stride s0).
+ * The network's size must be the images' level 3. */
+int rf_warmnet_warm_start(rf_warmnet* net, const float* d_i0, const float* d_i1, int w0, int h0, int s0, float* d_u,
+                          float* d_v, void* stream);
+
 /* ---- per-kernel entry points on device pointers (the public kernel wrappers of
  * Cudaflow.h:101-193 and Reconflow.h:86-158), enqueued on `stream` ---- */
 int rf_k_rgb_to_gray(const uint8_t* rgb, float* gray, int w, int h, int s, void* stream);       /* rgbToGray.cu:80-86 */

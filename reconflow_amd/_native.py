@@ -53,6 +53,16 @@ _PROTOS = {
     "rf_warm_start_in_use": (_I, [_P]),
     "rf_solve_flow": (_I, [_P, _F]),
     "rf_solve_flow_noalloc": (_I, [C.POINTER(RFParams), _P, _P, _P, _P, _P]),
+    "rf_warmnet_create": (_I, [C.POINTER(_P), _I, _I, _I]),
+    "rf_warmnet_destroy": (_I, [_P]),
+    "rf_warmnet_param_count": (_I, [_P]),
+    "rf_warmnet_set_params": (_I, [_P, _P, _I]),
+    "rf_warmnet_get_params": (_I, [_P, _P, _I]),
+    "rf_warmnet_get_grads": (_I, [_P, _P, _I]),
+    "rf_warmnet_input": (_I, [_P, _P, _P, _I, C.c_long, _P, _P, _I, C.c_long, _I, _P]),
+    "rf_warmnet_forward": (_I, [_P, _I, _P, _P]),
+    "rf_warmnet_train_step": (_I, [_P, _I, _P, _F, C.POINTER(_F), _P]),
+    "rf_warmnet_warm_start": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
     "rf_copy_masks": (_I, [_P, _P, _P]),
     "rf_copy_depth": (_I, [_P, _P, _P]),
     "rf_solve_inpainting": (_I, [_P, _F]),

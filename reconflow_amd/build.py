@@ -19,7 +19,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 
-HIP_SOURCES = ["rf_kernels.hip", "rf_batch.hip", "rf_solver.hip", "rf_capi_kernels.hip", "rf_noalloc.hip"]
+HIP_SOURCES = ["rf_kernels.hip", "rf_batch.hip", "rf_solver.hip", "rf_capi_kernels.hip", "rf_noalloc.hip",
+               "rf_warmnet.hip"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

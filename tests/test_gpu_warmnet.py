@@ -1,0 +1,226 @@
+"""The learned warm start (reconflow_amd.warmnet, rf_warmnet_*) against a plain PyTorch FP32
+reference of the same network, and the warm-started joint solve against the FP64 oracle.
+
+* forward: the BF16/MFMA network equals PyTorch conv2d in FP32 on the same BF16-rounded weights
+  and activations (rounded after every layer, as the kernels store them), up to accumulation order;
+* backward: every parameter gradient of the Huber loss equals PyTorch autograd's (the kernels feed
+  BF16-rounded gradients of the pre-activations to the matrix cores);
+* Adam: the first step moves each parameter by lr g / (|g| + 1e-8) (bias-corrected moments);
+* training lowers the loss on a fixed synthetic task;
+* warm start: with the trained weights, the warm-started three-level solve of a held-out pair, and
+  the fallback rule: a network whose prediction is zero gives the cold solve bit for bit, and one
+  whose prediction is not finite is refused (the cold solve again).
+"""
+import numpy as np
+import pytest
+
+import reconflow_amd as rf
+from reconflow_amd import warmnet as wn
+from tests.gpu_util import fetch, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+H3, W3 = 12, 16  # small level-3 grid for the numerics tests
+
+
+def _bf(t):
+    torch = torch_cuda()
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def _torch_params(p):
+    """Flat parameters -> [(weight (cout, cin, 3, 3) BF16-rounded, bias)] as leaf tensors."""
+    torch = torch_cuda()
+    out = []
+    for w, b in wn.split_params(p):
+        wt = torch.from_numpy(np.ascontiguousarray(w.reshape(3, 3, w.shape[1], w.shape[2]).transpose(2, 3, 0, 1)))
+        out.append((_bf(wt.cuda()).requires_grad_(True), torch.from_numpy(b.copy()).cuda().requires_grad_(True)))
+    return out
+
+
+def _ref_forward(tp, x):
+    """x: (n, 4, H, W) FP32 input channels (already minus 0.5)."""
+    torch = torch_cuda()
+    F = torch.nn.functional
+    # BF16 rounding in the forward pass, identity in the backward pass (as the kernels' stores)
+    bf = lambda t: t + (_bf(t) - t).detach()  # noqa: E731
+    a = bf(x)
+    a1 = bf(F.relu(F.conv2d(a, tp[0][0], tp[0][1], padding=1)))
+    a2 = bf(F.relu(F.conv2d(a1, tp[1][0], tp[1][1], stride=2, padding=1)))
+    a3 = bf(F.relu(F.conv2d(a2, tp[2][0], tp[2][1], padding=1)))
+    a4 = bf(F.relu(F.conv2d(a3, tp[3][0], tp[3][1], padding=1)))
+    u = bf(a4.repeat_interleave(2, dim=2).repeat_interleave(2, dim=3) + a1)
+    a5 = bf(F.relu(F.conv2d(u, tp[4][0], tp[4][1], padding=1)))
+    return F.conv2d(a5, tp[5][0], tp[5][1], padding=1)  # (n, 2, H, W) FP32
+
+
+def _inputs(n, seed):
+    """Level-3 and level-4 images of n samples (device tensors laid out as the network reads them)
+    and the matching (n, 4, H, W) reference input."""
+    torch = torch_cuda()
+    rng = np.random.default_rng(seed)
+    s3, s4 = 32, 32
+    x3 = np.zeros((n, 2, H3, s3), np.float32)
+    x4 = np.zeros((n, 2, H3 // 2, s4), np.float32)
+    x3[..., :W3] = rng.random((n, 2, H3, W3))
+    x4[..., :W3 // 2] = rng.random((n, 2, H3 // 2, W3 // 2))
+    ref = np.concatenate([x3[..., :W3], x4[..., :W3 // 2].repeat(2, axis=2).repeat(2, axis=3)], axis=1) - 0.5
+    return torch.from_numpy(x3).cuda(), torch.from_numpy(x4).cuda(), torch.from_numpy(ref).cuda(), s3, s4
+
+
+def _set_input(net, x3, x4, s3, s4, n):
+    net.set_input(x3[:, 0].data_ptr(), x3[:, 1].data_ptr(), s3, 2 * H3 * s3, x4[:, 0].data_ptr(),
+                  x4[:, 1].data_ptr(), s4, 2 * (H3 // 2) * s4, n)
+
+
+def _relerr(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_equals_torch_fp32():
+    torch = torch_cuda()
+    n = 3
+    p = wn.init_params(1)
+    p[-2:] = [0.3, -0.2]  # nonzero output bias
+    net = wn.WarmStartNet(H3, W3, n, p)
+    x3, x4, xr, s3, s4 = _inputs(n, 2)
+    _set_input(net, x3, x4, s3, s4, n)
+    out = torch.zeros((n, H3, W3, 2), dtype=torch.float32, device="cuda")
+    net.forward(n, out.data_ptr())
+    torch.cuda.synchronize()
+    ref = _ref_forward(_torch_params(p), xr).permute(0, 2, 3, 1).detach()
+    err = _relerr(out.cpu().numpy(), ref.cpu().numpy())
+    print(f"forward: normwise relative error {err:.2e}", flush=True)
+    assert err < 5e-3
+    assert float(out.abs().max()) > 0.1
+
+
+def test_backward_equals_torch_autograd():
+    torch = torch_cuda()
+    n = 2
+    p = wn.init_params(3)
+    p = p * 2.0  # larger last layer so every gradient is well above the rounding
+    net = wn.WarmStartNet(H3, W3, n, p)
+    x3, x4, xr, s3, s4 = _inputs(n, 4)
+    target = torch.from_numpy(np.random.default_rng(5).normal(0, 1.5, (n, H3, W3, 2)).astype(np.float32)).cuda()
+    _set_input(net, x3, x4, s3, s4, n)
+    loss = net.train_step(n, target.data_ptr(), 0.0)
+    g = net.get_grads()
+    tp = _torch_params(p)
+    pred = _ref_forward(tp, xr).permute(0, 2, 3, 1)
+    ref_loss = torch.nn.functional.huber_loss(pred, target, delta=1.0, reduction="mean")
+    ref_loss.backward()
+    assert abs(loss - float(ref_loss)) <= 1e-3 * abs(float(ref_loss)), (loss, float(ref_loss))
+    o = 0
+    for k, ((w, b), (ci, co, _)) in enumerate(zip(tp, wn.LAYERS)):
+        gw = w.grad.permute(2, 3, 0, 1).reshape(9, co, ci).cpu().numpy().ravel()
+        ew = _relerr(g[o:o + gw.size], gw)
+        o += gw.size
+        eb = _relerr(g[o:o + co], b.grad.cpu().numpy())
+        o += co
+        print(f"layer {k + 1}: weight gradient error {ew:.2e}, bias gradient error {eb:.2e}", flush=True)
+        assert ew < 3e-2 and eb < 3e-2, k
+    assert o == g.size
+
+
+def test_adam_first_step():
+    torch = torch_cuda()
+    n = 2
+    p = wn.init_params(6)
+    net = wn.WarmStartNet(H3, W3, n, p)
+    x3, x4, _, s3, s4 = _inputs(n, 7)
+    target = torch.ones((n, H3, W3, 2), dtype=torch.float32, device="cuda")
+    _set_input(net, x3, x4, s3, s4, n)
+    lr = 1e-3
+    net.train_step(n, target.data_ptr(), lr)
+    g = net.get_grads()
+    q = net.get_params()
+    c1 = np.float32(1.0) / (np.float32(1.0) - np.float32(0.9))
+    c2 = np.float32(1.0) / (np.float32(1.0) - np.float32(0.999))
+    m, v = np.float32(0.1) * g, np.float32(0.001) * g * g
+    exp = p - np.float32(lr) * (m * c1) / (np.sqrt(v * c2) + np.float32(1e-8))
+    np.testing.assert_allclose(q, exp, rtol=1e-5, atol=1e-9)
+
+
+def test_training_lowers_the_loss():
+    torch = torch_cuda()
+    n = 16
+    net = wn.WarmStartNet(H3, W3, n, wn.init_params(8))
+    x3, x4, xr, s3, s4 = _inputs(n, 9)
+    # a target the network can represent: a smooth function of the image difference
+    d = (xr[:, 1] - xr[:, 0]).cpu().numpy()
+    target = torch.from_numpy(np.stack([3.0 * d, -2.0 * d], -1).astype(np.float32)).cuda()
+    _set_input(net, x3, x4, s3, s4, n)
+    losses = [net.train_step(n, target.data_ptr(), 3e-3) for _ in range(150)]
+    print(f"training: loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+    assert losses[-1] < 0.3 * losses[0]
+
+
+WEIGHTS = wn.WEIGHTS / "warmnet_640x480.npz"
+
+
+def _joint_batch(L, fix=1):
+    return rf.batch_params(14, L, 5, 50, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1,
+                           lambdams=100.0, tri_f64=1, fix_coarse_k=fix)
+
+
+def _solve(d, L, ws=None):
+    torch = torch_cuda()
+    b = rf.Batch(_joint_batch(L), [640], [480])
+    i0, i1 = torch.from_numpy(d["i0"]).cuda(), torch.from_numpy(d["i1"]).cuda()
+    b.set_images_device(0, i0.data_ptr(), i1.data_ptr())
+    b.set_camera(0, d["K"], d["K"], d["R"], d["t"])
+    if ws is not None:
+        b.set_warm_start_device(0, ws[0].data_ptr(), ws[1].data_ptr())
+    b.solve()
+    b.synchronize()
+    out = [fetch(ptr, 480 * 640).reshape(480, 640) for ptr in b.result_ptrs(0)[:2]]
+    used = b.warm_start_in_use(0) if ws is not None else False
+    return out, used
+
+
+def _predict(net, d):
+    torch = torch_cuda()
+    i0, i1 = torch.from_numpy(d["i0"]).cuda(), torch.from_numpy(d["i1"]).cuda()
+    u = torch.zeros(480 * 640, dtype=torch.float32, device="cuda")
+    v = torch.zeros_like(u)
+    net.warm_start(i0.data_ptr(), i1.data_ptr(), 640, 480, 640, u.data_ptr(), v.data_ptr())
+    torch.cuda.synchronize()
+    return u, v
+
+
+def test_warm_start_fallbacks_give_the_cold_solve():
+    d = rf.make_recon_pair(640, 480, 7)
+    cold, _ = _solve(d, 3)
+    zero = wn.init_params(0) * 0.0  # predicts exactly zero flow: the reference's cold start
+    net = wn.WarmStartNet(60, 80, 1, zero)
+    got, used = _solve(d, 3, _predict(net, d))
+    assert used and all(np.array_equal(a, b) for a, b in zip(got, cold))
+    bad = zero.copy()
+    bad[-1] = np.inf  # the last bias: every prediction is infinite, so the rule refuses it
+    net.set_params(bad)
+    got, used = _solve(d, 3, _predict(net, d))
+    assert not used and all(np.array_equal(a, b) for a, b in zip(got, cold))
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="no trained weights (python -m reconflow_amd.train_warmnet)")
+def test_trained_warm_start_on_held_out_pairs(orc):
+    """Held-out seeds (the training pairs start at seed 100000): the warm-started solve from level 2
+    (three levels) against the full cold solve (five levels) of the FP64 oracle, beside the cold
+    three-level solve, which has no coarse levels to find the large displacements."""
+    net = wn.WarmStartNet.load(WEIGHTS)
+    rows = []
+    for seed in (2, 3, 4):
+        d = rf.make_recon_pair(640, 480, seed)
+        p = orc.recon_params(640, 480, 5, fix_coarse_k=1)
+        ref = orc.solve_recon(p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f64")
+        warm, used = _solve(d, 3, _predict(net, d))
+        cold3, _ = _solve(d, 3)
+        e = lambda o: np.hypot(o[0] - ref[0][:, :640], o[1] - ref[1][:, :640])  # noqa: E731
+        ew, ec = e(warm), e(cold3)
+        rows.append((np.median(ew), np.median(ec)))
+        print(f"seed {seed}: warm-started 3 levels median EPE {np.median(ew):.4f} mean {ew.mean():.3f}; "
+              f"cold 3 levels median {np.median(ec):.4f} mean {ec.mean():.3f}", flush=True)
+        assert used
+    assert np.mean([r[0] for r in rows]) < np.mean([r[1] for r in rows])
