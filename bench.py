@@ -207,9 +207,62 @@ def run_fixed_k(args, rf, d, path):
         from oracle import oracle as orc
         ref = orc.solve_recon(orc.recon_params(W, H, LEVELS, fix_coarse_k=1), d["K"], d["K"], np.eye(3), np.zeros(3),
                               d["R"], d["t"], d["i0"], d["i1"], "f64")
+        REF64["joint_fixed_k"] = ref
         e = np.hypot(u[:, :W] - ref[0][:, :W], v[:, :W] - ref[1][:, :W])
         out["epe_vs_fp64_oracle"] = round(float(e.mean()), 5)
         out["median_epe_vs_fp64_oracle"] = round(float(np.median(e)), 6)
+    return out
+
+
+def run_warm_started(args, rf, torch, d, ref_fixed):
+    """The learned warm start (reconflow_amd.warmnet, trained weights in reconflow_amd/weights):
+    per frame the network's prediction (pyramid, six BF16 MFMA convolutions, upscale to level 0) and
+    a solve that starts from it at level 3 (4 levels, the network's resolution) or level 2 (3 levels),
+    5 warps x 50 iterations, K[2][2] = 1, FP64 mode. One pair per step (FP32 storage) and 8 pairs per
+    batched launch (FP16 derivatives, 4 levels); the network runs on the solver's stream inside the
+    timed step. Accuracy against the FP64 oracle's full cold solve (5 levels) of the same mode."""
+    from reconflow_amd import warmnet as wn
+    path = wn.WEIGHTS / "warmnet_640x480.npz"
+    if not path.exists():
+        return None
+    out = {"weights": str(path.relative_to(ROOT))}
+    for name, B, storage, L in (("1_pair_4_levels_f32", 1, rf.STORAGE_F32, 4),
+                                ("1_pair_3_levels_f32", 1, rf.STORAGE_F32, 3),
+                                ("8_pairs_4_levels_f16_derivatives", 8, rf.STORAGE_F16_DERIV, 4)):
+        net = wn.WarmStartNet.load(path)
+        p = rf.batch_params(14, L, WARPS, ITERS, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1,
+                            lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage, fix_coarse_k=1)
+        b = rf.Batch(p, [W] * B, [H] * B)
+        keep, ws = [], []
+        for i in range(B):
+            di = d if i == 0 else rf.make_recon_pair(W, H, args.seed + i)
+            a, c = torch.from_numpy(di["i0"]).cuda(), torch.from_numpy(di["i1"]).cuda()
+            u = torch.zeros(H * W, dtype=torch.float32, device="cuda")
+            v = torch.zeros_like(u)
+            keep += [a, c, u, v]
+            ws.append((a, c, u, v))
+            b.set_images_device(i, a.data_ptr(), c.data_ptr())
+            b.set_camera(i, di["K"], di["K"], di["R"], di["t"])
+            b.set_warm_start_device(i, u.data_ptr(), v.data_ptr())
+        stream = b.stream()
+
+        def step():
+            for a, c, u, v in ws:
+                net.warm_start(a.data_ptr(), c.data_ptr(), W, H, W, u.data_ptr(), v.data_ptr(), stream)
+            b.solve()
+
+        torch.cuda.synchronize()
+        dt = _time(step, b.synchronize, max(4, args.steps // 2), max(1, args.warmup))
+        ent = {"frames_per_s": round(B / dt, 2), "ms_per_step": round(dt * 1e3, 3),
+               "warm_start_used": bool(b.warm_start_in_use(0))}
+        if ref_fixed is not None:
+            u_ptr, v_ptr = b.result_ptrs(0)[:2]
+            uu, vv = _fetch(u_ptr, H * W).reshape(H, W), _fetch(v_ptr, H * W).reshape(H, W)
+            e = np.hypot(uu - ref_fixed[0][:, :W], vv - ref_fixed[1][:, :W])
+            ent["epe_vs_fp64_oracle_5_levels"] = {"mean": round(float(e.mean()), 4),
+                                                  "median": round(float(np.median(e)), 5)}
+        out[name] = ent
+        del b, net
     return out
 
 
@@ -297,6 +350,7 @@ def run_joint(args, rf, torch):
     zerr = depth_rel_err(Z, REF64["joint"][4]) if "joint" in REF64 else None
     cycle = None if args.no_sub else run_drop_in_cycle(args, rf, d)
     fixed = None if args.no_sub else run_fixed_k(args, rf, d, path)
+    warm = None if args.no_sub else run_warm_started(args, rf, torch, d, REF64.get("joint_fixed_k"))
     batched = None
     if path == rf.PATH_FUSED and args.batch_pairs > 0:
         batched = {"pairs_per_launch": args.batch_pairs}
@@ -336,6 +390,7 @@ def run_joint(args, rf, torch):
         "accuracy_ok": None if epe is None else bool(epe <= 0.05),
         "drop_in_cycle": cycle,
         "fixed_coarse_k": fixed,
+        "learned_warm_start": warm,
         "plain_fp32_path_fps": None if plain_fps is None else round(plain_fps, 3),
         "speedup_vs_plain_path": None if plain_fps is None else round(fps / plain_fps, 3),
         "mpix_iter_per_s": round(fps * px_iter / 1e6, 1),

@@ -7,9 +7,10 @@ reference of the same network, and the warm-started joint solve against the FP64
   BF16-rounded gradients of the pre-activations to the matrix cores);
 * Adam: the first step moves each parameter by lr g / (|g| + 1e-8) (bias-corrected moments);
 * training lowers the loss on a fixed synthetic task;
-* warm start: with the trained weights, the warm-started three-level solve of a held-out pair, and
-  the fallback rule: a network whose prediction is zero gives the cold solve bit for bit, and one
-  whose prediction is not finite is refused (the cold solve again).
+* warm start: with the trained weights, the warm-started four-level solve of held-out pairs against
+  the full cold solve of the FP64 oracle; and the fallback rule: a network whose prediction is zero
+  gives the cold solve bit for bit, and one whose prediction is not finite is refused (the cold
+  solve again).
 """
 import numpy as np
 import pytest
@@ -206,21 +207,27 @@ def test_warm_start_fallbacks_give_the_cold_solve():
 
 @pytest.mark.skipif(not WEIGHTS.exists(), reason="no trained weights (python -m reconflow_amd.train_warmnet)")
 def test_trained_warm_start_on_held_out_pairs(orc):
-    """Held-out seeds (the training pairs start at seed 100000): the warm-started solve from level 2
-    (three levels) against the full cold solve (five levels) of the FP64 oracle, beside the cold
-    three-level solve, which has no coarse levels to find the large displacements."""
+    """Held-out seeds 2-9 (the training pairs start at seed 100000): the solve that starts at level 3
+    (4 levels) from the network's prediction against the full cold solve (5 levels) of the FP64
+    oracle, beside the cold 4-level solve, which has too few levels to find the large displacements.
+    Measured: the warm-started solve is within 0.014 px (median) of the full solve on 5 of the 8
+    pairs; on the other 3 the prediction is 5-8 px off (large rotations) and the solve 1-3.3 px,
+    against 9-11 px for the cold 4-level solve. Asserted: at least 5 of 8 within 0.05 px, and on every
+    pair the warm-started median below the cold one."""
     net = wn.WarmStartNet.load(WEIGHTS)
-    rows = []
-    for seed in (2, 3, 4):
+    warm_med, cold_med = [], []
+    for seed in range(2, 10):
         d = rf.make_recon_pair(640, 480, seed)
         p = orc.recon_params(640, 480, 5, fix_coarse_k=1)
         ref = orc.solve_recon(p, d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"], "f64")
-        warm, used = _solve(d, 3, _predict(net, d))
-        cold3, _ = _solve(d, 3)
+        warm, used = _solve(d, 4, _predict(net, d))
+        cold4, _ = _solve(d, 4)
         e = lambda o: np.hypot(o[0] - ref[0][:, :640], o[1] - ref[1][:, :640])  # noqa: E731
-        ew, ec = e(warm), e(cold3)
-        rows.append((np.median(ew), np.median(ec)))
-        print(f"seed {seed}: warm-started 3 levels median EPE {np.median(ew):.4f} mean {ew.mean():.3f}; "
-              f"cold 3 levels median {np.median(ec):.4f} mean {ec.mean():.3f}", flush=True)
+        ew, ec = e(warm), e(cold4)
+        warm_med.append(np.median(ew))
+        cold_med.append(np.median(ec))
+        print(f"seed {seed}: warm-started 4 levels median EPE {np.median(ew):.4f} mean {ew.mean():.3f}; "
+              f"cold 4 levels median {np.median(ec):.4f} mean {ec.mean():.3f}", flush=True)
         assert used
-    assert np.mean([r[0] for r in rows]) < np.mean([r[1] for r in rows])
+    assert sum(m < 0.05 for m in warm_med) >= 5
+    assert all(w < c for w, c in zip(warm_med, cold_med))
