@@ -106,10 +106,10 @@ typedef struct rf_params {
 
 /* Tilings of the fused joint iteration (all give the same results). AUTO: the first joint solves
  * time 16x16, 32x16, 16x16_K2, 16x16_OCC5 and 16x16_K3 on alternating warps of every pyramid
- * level (as many solves as it takes to time each at least once) and keep the fastest per level.
- * Those solves synchronise. A K2/K3 launch runs two/three solver
+ * level (as many solves as it takes to time each on at least two warps) and keep, per level, the
+ * one with the shortest warp. Those solves synchronise. A K2/K3 launch runs two/three solver
  * iterations on an extended tile that recomputes a ring of 3/5 pixels (for latency-bound small
- * levels); an iteration count that K does not divide ends with one-iteration launches. */
+ * levels); an iteration count that K does not divide ends with launches of fewer iterations. */
 #define RF_JOINT_TILE_AUTO 0
 #define RF_JOINT_TILE_64x8_RPT2 1  /* 64x4 lanes, two rows per lane plus a ring pass */
 #define RF_JOINT_TILE_32x16 2      /* one pixel per lane, 30x14 pixels per workgroup */
@@ -119,11 +119,11 @@ typedef struct rf_params {
 #define RF_JOINT_TILE_16x16_K3 6   /* three iterations per launch, 6x6 pixels per 16x16 workgroup */
 
 /* Tilings of the fused TV-L1 iteration (all give the same results). AUTO: the first solves time
- * 16x16, 32x16_K4, 32x16, 32x32_K8 and 64x16_RPT4 on alternating warps of every level (as many
- * solves as it takes to time each at least once) and keep the fastest per level. Those solves
- * synchronise. A K4/K8 launch runs four/eight solver iterations on an extended tile that
- * recomputes a ring of pixels; an iteration count that K does not divide ends with
- * one-iteration launches. */
+ * 16x16, 32x16_K4, 32x16, 32x16_K6, 32x32_K8 and 64x16_RPT4 on alternating warps of every level
+ * (as many solves as it takes to time each on at least two warps) and keep, per level, the one with
+ * the shortest warp. Those solves synchronise. A K2/K4/K6/K8 launch runs that many solver
+ * iterations on an extended tile that recomputes a ring of pixels; an iteration count that K does
+ * not divide ends with launches of fewer iterations. */
 #define RF_FLOW_TILE_AUTO 0
 #define RF_FLOW_TILE_64x16_RPT4 1 /* 64x4 lanes, four rows per lane plus a ring pass */
 #define RF_FLOW_TILE_16x16 2      /* one pixel per lane, 15x15 pixels per workgroup */
@@ -275,7 +275,7 @@ int rf_profile_arm(rf_context* ctx, int kernel_id, int level);
 int rf_profile_read(rf_context* ctx, double* total_ms, int* launches);
 
 /* The fused joint tiling (RF_JOINT_TILE_*) each pyramid level uses, for levels 0..n-1: the
- * fixed choice, or what the tuning solve picked (RF_JOINT_TILE_AUTO before it ran). Fused path
+ * fixed choice, or what the tuning solves picked (RF_JOINT_TILE_AUTO before it ran). Fused path
  * only (no counterpart in the reference). */
 int rf_joint_tiling(rf_context* ctx, int* tiling, int n);
 /* The same for the fused TV-L1 iteration (RF_FLOW_TILE_*). */

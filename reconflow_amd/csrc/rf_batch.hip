@@ -852,8 +852,9 @@ __device__ __forceinline__ void joint_store_deriv(const JointView& V, unsigned q
 // covers one pair whose tiles are the dispatch indices (no tile list read) and whose field
 // table is in the kernel arguments.
 // OCC: minimum waves per SIMD the register allocation must allow (1: no constraint, the
-// schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs with
-// the FP64 triangulation, met without spilling, for occupancy on large levels).
+// schedule keeps more loads in flight, for latency-bound small levels; 5: at most 96 VGPRs, five
+// 4-wave workgroups per CU, for occupancy on large levels; 4 for the 8-wave 32x16 workgroup, of
+// which a CU holds two at four or five waves per SIMD alike, so the tighter cap would only spill).
 // The body of kb_joint_px for the tile at (x0, y0). INNER: every lane of the extended tile is a pixel
 // at least one pixel inside the image, so no coordinate is clamped and no border rule applies.
 // TI (not void): the lanes compute the derivatives (joint_load) and interior lanes store them.
@@ -1539,12 +1540,14 @@ int Batch::flow_tiling(int level) const {
     }
 }
 
-// The tuning solve: an eager solve whose warps alternate between the candidate tilings, timed
-// per (level, warp) with events; each level keeps its faster candidate. Its results are those of
-// any other solve.
+// The tuning solves: eager solves whose warps alternate between the candidate tilings, timed
+// per (level, warp) with events; every candidate is timed on at least two warps of each level and
+// the level keeps the candidate with the shortest warp (candidates within a few percent of each
+// other are common, and one sample per candidate picked the slower one on some runs). Their results
+// are those of any other solve.
 int Batch::tune_tiling(bool recon) {
     const int L = p_.n_levels, nw = p_.n_warp_iters, k = recon ? 1 : 0;
-    const int nc = kCandCount[k], rounds = (nc + nw - 1) / nw;  // every candidate timed at least once
+    const int nc = kCandCount[k], rounds = 2 * ((nc + nw - 1) / nw);  // every candidate timed twice or more
     tev_.assign(2 * L * nw, nullptr);
     int rc = RF_OK;
     for (auto& ev : tev_)
@@ -1566,9 +1569,9 @@ int Batch::tune_tiling(bool recon) {
                     rc = RF_ERR_HIP;
                     break;
                 }
-                const int c = (r * nw + wi) % nc;
-                t[(size_t)l * nc + c] += ms;
-                ++cnt[(size_t)l * nc + c];
+                const size_t i = (size_t)l * nc + (r * nw + wi) % nc;
+                t[i] = cnt[i] ? std::min(t[i], (double)ms) : (double)ms;
+                ++cnt[i];
             }
     }
     cls_[k].assign(L, kTuneCand[k][0]);
@@ -1577,8 +1580,8 @@ int Batch::tune_tiling(bool recon) {
         bool any = false;
         for (int c = 0; c < nc; ++c) {
             const int i = l * nc + c;
-            if (cnt[i] && (!any || t[i] / cnt[i] < best)) {
-                best = t[i] / cnt[i];
+            if (cnt[i] && (!any || t[i] < best)) {
+                best = t[i];
                 cls_[k][l] = kTuneCand[k][c];
                 any = true;
             }
@@ -1750,7 +1753,7 @@ static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
             hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, F64, PRIOR, TS>), grid, dim3(TXW, TRW), 0, st, a);
             break;
         case TC_J32:
-            hipLaunchKernelGGL((kb_joint_px<32, 16, 5, SM, F64, PRIOR, TS, TK>), grid, dim3(32, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<32, 16, 4, SM, F64, PRIOR, TS, TK>), grid, dim3(32, 16), 0, st, a);
             break;
         case TC_J16O:
             hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
