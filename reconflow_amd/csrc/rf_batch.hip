@@ -1325,6 +1325,11 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
                           std::to_string(l));
                 return RF_ERR_ARG;
             }
+            if (l > 0 && !(resample_fits(geo[i].W[l - 1], nw) && resample_fits(geo[i].H[l - 1], nh))) {
+                set_error("rf_batch_create: pair " + std::to_string(i) + ": levels " + std::to_string(l - 1) +
+                          " and " + std::to_string(l) + " are too large for 32-bit texel coordinates at this scale");
+                return RF_ERR_ARG;
+            }
             geo[i].W.push_back(nw);
             geo[i].H.push_back(nh);
             geo[i].S.push_back(align_up_b(nw, align));

@@ -100,6 +100,11 @@ int Solver::set_geometry(const rf_params& p, int align, const char* who) {
             set_error(std::string(who) + ": pyramid level " + std::to_string(l) + " has no pixels");
             return RF_ERR_ARG;
         }
+        if (l > 0 && !(resample_fits(g.W[l - 1], nw) && resample_fits(g.H[l - 1], nh))) {
+            set_error(std::string(who) + ": levels " + std::to_string(l - 1) + " and " + std::to_string(l) +
+                      " are too large for 32-bit texel coordinates at this scale");
+            return RF_ERR_ARG;
+        }
         g.W[l] = nw;
         g.H[l] = nh;
         g.S[l] = ns;

@@ -88,6 +88,14 @@ struct Geometry {
 // as 32-bit products such as (4 x + 2) * W.
 constexpr int kMaxSide = 32768;
 
+// The same products between a level of side `fine` and the next coarser one of side `coarse`:
+// downscale forms (4 x + 2) * fine + fine with x < coarse, upscale (2 x + 1) * coarse with
+// x < fine. Below kMaxSide they fit for a scale of 2 or more; a smaller scale needs this check.
+inline bool resample_fits(int fine, int coarse) {
+    const long long lim = 2147483647LL;
+    return 4LL * coarse * fine + fine <= lim && 2LL * fine * coarse <= lim;
+}
+
 class Batch;
 
 // One image pair. The per-step path runs here; the fused path runs as a batch of one pair.

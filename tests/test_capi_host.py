@@ -60,8 +60,16 @@ def test_validation_errors_without_device(lib):
         lib.rf_params_default(C.byref(p), w, h)
         assert lib.rf_initialize(ctx, C.byref(p)) == -1
         assert b"below 32768" in lib.rf_last_error()
-    lib.rf_params_default(C.byref(p), 64, 48)
+    # below 32768, a scale under 2 leaves a coarse level wide enough for the products to overflow
+    lib.rf_params_default(C.byref(p), 30000, 8)
+    p.n_levels, p.scale = 2, 1.5
+    assert lib.rf_initialize(ctx, C.byref(p)) == -1
+    assert b"32-bit texel coordinates" in lib.rf_last_error()
     b = C.c_void_p()
+    W, H = (C.c_int * 1)(30000), (C.c_int * 1)(8)
+    assert lib.rf_batch_create(C.byref(b), C.byref(p), 1, W, H) == -1
+    assert b"32-bit texel coordinates" in lib.rf_last_error()
+    lib.rf_params_default(C.byref(p), 64, 48)
     W, H = (C.c_int * 2)(64, 40000), (C.c_int * 2)(48, 16)
     assert lib.rf_batch_create(C.byref(b), C.byref(p), 2, W, H) == -1
     assert b"below 32768" in lib.rf_last_error()
