@@ -11,6 +11,9 @@
   previous pair's result through rf_batch_set_warm_start_device: bit for bit the oracle's
   warm-started solves at 160x120; at 1280x720, finite results that use the warm start and stay
   close to the cold solve of the same pair.
+* A stream split into sequence chunks that share batched launches (one pair of each chunk per
+  launch, each pair warm-started from the previous pair of its chunk): every chunk equals the
+  oracle's warm-started chain over that chunk bit for bit.
 """
 import numpy as np
 import pytest
@@ -228,3 +231,65 @@ def test_warm_started_stream_1280x720_properties(orc):
             d = dict(i0=seq["frames"][k], i1=seq["frames"][k + 1], K=seq["K"], R=seq["R"][k], t=seq["t"][k])
             cold = _solve_single(p, w, h, _setup_for([d], True)(0))
             assert np.array_equal(got[k][0][:, :w], cold[0][:, :w], equal_nan=True), k
+
+
+def _chunked_stream(seq, w, h, chunks, steps, L, nw, ni, storage=rf.STORAGE_F32):
+    """A sequence cut into `chunks` runs of `steps` consecutive pairs; launch j solves pair j of every
+    run in one batch (slot c: pair c * steps + j), each slot warm-started from a device copy of its
+    own previous result (the first pair of a run starts cold: its warm-start buffers are zero).
+    Returns flows[c][j] and whether each solve used its warm start."""
+    p = rf.batch_params(14, L, nw, ni, tau=0.125, tri_f64=1, fix_coarse_k=1, storage=storage)
+    b = rf.Batch(p, [w] * chunks, [h] * chunks)
+    s = rf.align_up(w)
+    frames = [_dev_img(seq["frames"][k]) for k in range(chunks * steps + 1)]
+    ws = [(dzeros(h * s), dzeros(h * s)) for _ in range(chunks)]
+    for c in range(chunks):
+        b.set_warm_start_device(c, ws[c][0].data_ptr(), ws[c][1].data_ptr())
+    out = [[] for _ in range(chunks)]
+    used = [[] for _ in range(chunks)]
+    for j in range(steps):
+        for c in range(chunks):
+            k = c * steps + j
+            b.set_images_device(c, frames[k].data_ptr(), frames[k + 1].data_ptr())
+            b.set_camera(c, seq["K"], seq["K"], seq["R"][k], seq["t"][k])
+        b.solve()
+        for c in range(chunks):
+            u_ptr, v_ptr = b.result_ptrs(c)[:2]
+            assert copy_async(ws[c][0].data_ptr(), u_ptr, h * s * 4, b.stream()) == 0
+            assert copy_async(ws[c][1].data_ptr(), v_ptr, h * s * 4, b.stream()) == 0
+        b.synchronize()
+        for c in range(chunks):
+            u_ptr, v_ptr = b.result_ptrs(c)[:2]
+            used[c].append(b.warm_start_in_use(c))
+            out[c].append((fetch(u_ptr, h * s).reshape(h, s), fetch(v_ptr, h * s).reshape(h, s)))
+    return out, used
+
+
+def test_chunked_stream_batch_equals_oracle_small(orc):
+    """Three chunks of three consecutive 160x120 pairs share each batched launch; every chunk is the
+    oracle's warm-started chain over its pairs bit for bit, the rule's decisions included. The first
+    pair of a chunk reads its zero warm-start buffers, which the rule accepts and which give the cold
+    solve bit for bit."""
+    w, h, chunks, steps, L, nw, ni = 160, 120, 3, 3, 3, 2, 10
+    print(f"case: chunked warm-started stream {chunks} x {steps} pairs {w}x{h}", flush=True)
+    seq = rf.make_sequence(w, h, chunks * steps + 1, 33)
+    got, used = _chunked_stream(seq, w, h, chunks, steps, L, nw, ni)
+    f, K = seq["frames"], seq["K"]
+    s = rf.align_up(w)
+    pr = orc.recon_params(w, h, L, n_warp=nw, n_iter=ni, tri_f64=1, fix_coarse_k=1)
+    for c in range(chunks):
+        prev = (np.zeros((h, s), np.float32), np.zeros((h, s), np.float32))
+        for j in range(steps):
+            k = c * steps + j
+            exp = orc.warm_start_usable(prev[0], prev[1], w, h)
+            ref = orc.solve_recon(pr, K, K, np.eye(3), np.zeros(3), seq["R"][k], seq["t"][k], f[k], f[k + 1], "f32",
+                                  ws=prev)
+            if j == 0:
+                cold = orc.solve_recon(pr, K, K, np.eye(3), np.zeros(3), seq["R"][k], seq["t"][k], f[k], f[k + 1],
+                                       "f32")
+                assert all(np.array_equal(a[:, :w], b[:, :w], equal_nan=True) for a, b in zip(ref, cold)), c
+            print(f"  chunk {c} pair {k}: warm start used {used[c][j]}, rule {exp}", flush=True)
+            assert bool(used[c][j]) == exp, (c, j)
+            assert np.array_equal(got[c][j][0][:, :w], ref[0][:, :w], equal_nan=True), (c, j)
+            assert np.array_equal(got[c][j][1][:, :w], ref[1][:, :w], equal_nan=True), (c, j)
+            prev = (ref[0], ref[1])
