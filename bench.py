@@ -43,6 +43,8 @@ sys.path.insert(0, str(ROOT))
 
 W, H, LEVELS, WARPS, ITERS = 640, 480, 5, 5, 50
 PEAK_HBM_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 VALU instruction per SIMD every 4 cycles
+PEAK_VALU_GINST = 256 * 4 * 2.4 / 4
 
 # Algorithmic bytes per pixel of one launch (each distinct element read once and written
 # once, FP32).
@@ -330,6 +332,16 @@ def run_joint(args, rf, torch):
     if tfile.exists():
         key = f"{dom}_{tiling[0] if tiling else args.path}_{W}x{H}"
         traffic = json.loads(tfile.read_text()).get(key)
+    # the VALU work of the same launches (rocprofv3 SQ_INSTS_VALU per launch, profiles/valu.json) over
+    # the same measured time: the issue rate the level-0 launch is bound by
+    valu = None
+    vfile = ROOT / "profiles" / "valu.json"
+    if vfile.exists() and tiling:
+        n_valu = json.loads(vfile.read_text()).get(f"{dom}_{tiling[0]}_{W}x{H}")
+        if n_valu:
+            rate = n_valu / avg_s / 1e9
+            valu = {"wave_instructions_per_launch": n_valu, "achieved": round(rate, 1), "peak": PEAK_VALU_GINST,
+                    "unit": "G wave-instructions/s", "frac": round(rate / PEAK_VALU_GINST, 4)}
 
     plain_fps = None
     if path == rf.PATH_FUSED and not args.no_plain_compare:
@@ -384,7 +396,8 @@ def run_joint(args, rf, torch):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "level": 0, "avg_launch_us": round(avg_s * 1e6, 3), "iterations_per_launch": k0,
-                     "algorithmic_bytes_per_launch": bytes_launch, "avg_iteration_us_per_level": per_level_us},
+                     "algorithmic_bytes_per_launch": bytes_launch, "avg_iteration_us_per_level": per_level_us,
+                     "valu_issue": valu},
         "cpu_baseline": cpu,
         "epe_vs_fp64_oracle": None if epe is None else round(epe, 5),
         "depth_rel_err_vs_fp64_oracle": zerr,
