@@ -367,7 +367,8 @@ def run_joint(args, rf, torch):
     batched = None
     if path == rf.PATH_FUSED and args.batch_pairs > 0:
         batched = {"pairs_per_launch": args.batch_pairs}
-        for name, storage in (("f32", rf.STORAGE_F32), ("f16_derivatives", rf.STORAGE_F16_DERIV)):
+        for name, storage in (("f32", rf.STORAGE_F32), ("f16_derivatives", rf.STORAGE_F16_DERIV),
+                              ("f16_derivatives_duals", rf.STORAGE_F16_DERIV_DUALS)):
             bfps, bu, bv, btl = run_joint_batched(args, rf, torch, args.batch_pairs, storage)
             ent = {"frames_per_s": round(bfps, 2), "joint_tiling_per_level": btl}
             if plain_fps:

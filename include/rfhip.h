@@ -63,6 +63,7 @@ extern "C" {
 #define RF_STORAGE_F32 0
 #define RF_STORAGE_F16 1       /* FP16 images, derivatives (and TV-L1 duals); updates in FP32 */
 #define RF_STORAGE_F16_DERIV 2 /* FP32 images; FP16 derivatives (and TV-L1 duals); updates in FP32 */
+#define RF_STORAGE_F16_DERIV_DUALS 3 /* FP32 images; FP16 derivatives and duals (TV-L1 and joint); updates in FP32 */
 
 /* Status codes. */
 #define RF_OK 0

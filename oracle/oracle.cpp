@@ -1174,8 +1174,10 @@ struct Params {
     int f32_steps;
     int proj_f64;  // PREC_* bits: the precise-projection mode (for FP32 fields)
     // FP16 storage of the fused GPU path (rfhip.h RF_STORAGE_*): 1 = images, derivatives and the
-    // TV-L1 duals in binary16; 2 = derivatives and the TV-L1 duals. Every store of such a field is
-    // rounded to nearest-even binary16 and widened exactly on load; updates stay in T.
+    // TV-L1 duals in binary16; 2 = derivatives and the TV-L1 duals; 3 = derivatives and the duals of
+    // both solvers; 4 (a measurement mode with no GPU counterpart) = images, derivatives and the
+    // duals of both solvers. Every store of such a field is rounded to nearest-even binary16 and
+    // widened exactly on load; updates stay in T.
     int storage;
 };
 
@@ -1408,7 +1410,7 @@ int solve_flow(const Params& P, const void* i0, const void* i1, float* u_out, fl
     gray_input(P, i1, pI1[0], s0);
     round_group<T>(P, RND_IMG, n, {pI0[0], pI1[0]});
     const int mut = P.mutation;
-    const int img16 = P.storage == 1, der16 = P.storage == 1 || P.storage == 2;  // the TV-L1 duals: der16
+    const int img16 = P.storage == 1 || P.storage == 4, der16 = P.storage >= 1;  // the TV-L1 duals: der16
     round_f16<T>(img16, n, {pI0[0], pI1[0]});
     if (cg) {
         deriv_single(pI0[0], pW[0], pH[0], s0, pIx0[0], pIy0[0]);
@@ -1591,7 +1593,8 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
     k_pyramid(K1, P.n_levels, P.scale, P.fix_coarse_k, pK1);
 
     const int mut = P.mutation;
-    const int img16 = P.storage == 1, der16 = P.storage == 1 || P.storage == 2;  // joint duals stay in T
+    const int img16 = P.storage == 1 || P.storage == 4, der16 = P.storage >= 1;
+    const int dual16 = P.storage >= 3;  // the joint duals in binary16: storage 3 and 4 (measurement modes)
     // binary64 blends: the FP64 mode (tri_f64) takes the projection and data blends along with the
     // triangulation; proj_f64 selects them one by one. The FP64 solve computes in binary64 anyway.
     const int prec = sizeof(T) == sizeof(float) ? (P.proj_f64 | (P.tri_f64 ? PREC_PROJ | PREC_BLEND : 0)) : 0;
@@ -1694,6 +1697,7 @@ int solve_recon(const Params& P, const double* K0, const double* K1, const doubl
                 std::swap(pv1, pv1s);
                 std::swap(pv2, pv2s);
                 round_group<T>(P, RND_DUAL, n, {pu1, pu2, pv1, pv2});
+                round_f16<T>(dual16, n, {pu1, pu2, pv1, pv2});
                 if (a32(STEP_PROJ)) {
                     uvproj<float>(q.in(0, u), q.in(1, v), q.in(2, dumed), q.in(3, dvmed), q.in(4, uc), q.in(5, vc),
                                   q.in(6, sku), q.in(7, skv), w, h, s, (float)lambdaf, (float)alphaProj,

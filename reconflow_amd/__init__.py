@@ -20,11 +20,11 @@ from ._native import RFError, RFParams, check
 
 __all__ = ["CudaFlow", "ReconFlow", "RFError", "make_flow_pair", "make_recon_pair", "align_up",
            "INPUT_32F", "INPUT_8UC3", "INPUT_8U", "INPUT_16U", "PATH_PLAIN", "PATH_FUSED",
-           "STORAGE_F32", "STORAGE_F16", "STORAGE_F16_DERIV"]
+           "STORAGE_F32", "STORAGE_F16", "STORAGE_F16_DERIV", "STORAGE_F16_DERIV_DUALS"]
 
 INPUT_32F, INPUT_8UC3, INPUT_8U, INPUT_16U = 0, 1, 2, 3
 PATH_PLAIN, PATH_FUSED = 0, 1
-STORAGE_F32, STORAGE_F16, STORAGE_F16_DERIV = 0, 1, 2
+STORAGE_F32, STORAGE_F16, STORAGE_F16_DERIV, STORAGE_F16_DERIV_DUALS = 0, 1, 2, 3
 
 _DTYPE = {INPUT_32F: np.float32, INPUT_8UC3: np.uint8, INPUT_8U: np.uint8, INPUT_16U: np.uint16}
 

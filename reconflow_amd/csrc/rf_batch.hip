@@ -72,7 +72,8 @@ struct Launch {
 
 // Storage types: TI for the images (pyramid, warped image), TS for the derivatives and the TV-L1
 // dual fields; float, or __half with every update in FP32 (values rounded to nearest on store,
-// widened exactly on load). The joint solver keeps its dual fields in FP32.
+// widened exactly on load). The joint solver's dual fields are FP32, or binary16 with
+// RF_STORAGE_F16_DERIV_DUALS (JDual).
 template <typename TS>
 __device__ __forceinline__ TS st(float v) {
     return v;
@@ -100,6 +101,18 @@ template <typename T>
 __device__ __forceinline__ const T& el(const T* p, unsigned i) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T));
 }
+
+// The element of a T field that holds element q of a view whose pair sits qb floats further on (the
+// field slots are float-sized, so a 16-bit field's pair sits 2 qb elements further on).
+template <typename T>
+__device__ __forceinline__ unsigned typed_el(unsigned q, unsigned qb) {
+    return q + qb * (unsigned)(sizeof(float) / sizeof(T) - 1);
+}
+
+// The joint solver's dual fields: FP32, or binary16 (RF_STORAGE_F16_DERIV_DUALS: rounded to nearest
+// on store, widened exactly on load, every update in FP32).
+template <bool D16>
+using JDual = typename std::conditional<D16, __half, float>::type;
 
 template <typename TS>
 __global__ void __launch_bounds__(256) kb_gray(Launch L, int type) {
@@ -449,8 +462,9 @@ __device__ __forceinline__ int2 tile_xy(int t, int tiles_x, float inv) {
 }
 
 // One joint iteration (see the file comment). TY = 8.
-template <int RPT, bool F64TRI, bool PRIOR, typename TS>
+template <int RPT, bool F64TRI, bool PRIOR, typename TS, bool D16 = false>
 __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
+    using TD = JDual<D16>;
     constexpr int TY = TRW * RPT, EX = TXW + 2, EY = TY + 2, NE = EX * EY, NHALO = 2 * EX + 2 * TY;
     static_assert(NHALO <= 256, "ring larger than the workgroup");
     __shared__ float s_du[NE], s_dv[NE], s_up[NE], s_vp[NE], s_X[NE], s_Y[NE], s_Z[NE];
@@ -463,7 +477,8 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
     const int tid = threadIdx.y * TXW + threadIdx.x;
     const int pd = a.pd, pf = a.pf, px = a.px;
     const float *du0 = F.du[pd], *dv0 = F.dv[pd];
-    const float *p1u0 = F.p1u[pd], *p2u0 = F.p2u[pd], *p1v0 = F.p1v[pd], *p2v0 = F.p2v[pd];
+    const TD *p1u0 = as<TD>(F.p1u[pd]), *p2u0 = as<TD>(F.p2u[pd]), *p1v0 = as<TD>(F.p1v[pd]),
+             *p2v0 = as<TD>(F.p2v[pd]);
     const float *up0 = F.up[pf], *vp0 = F.vp[pf], *sku0 = F.sku[pf], *skv0 = F.skv[pf], *uc0 = F.uc[pf],
                 *vc0 = F.vc[pf];
     const float *X0 = F.X[px], *Y0 = F.Y[px], *Z0 = F.Z[px];
@@ -491,12 +506,12 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
         if (!a.first) {
             duh = du0[q];
             dvh = dv0[q];
-            p1u = p1u0[q];
-            p2u = p2u0[q];
-            p1v = p1v0[q];
-            p2v = p2v0[q];
-            if (hl) p1ul = p1u0[q - 1], p1vl = p1v0[q - 1];
-            if (hu) p2ud = p2u0[q - s], p2vd = p2v0[q - s];
+            p1u = (float)p1u0[q];
+            p2u = (float)p2u0[q];
+            p1v = (float)p1v0[q];
+            p2v = (float)p2v0[q];
+            if (hl) p1ul = (float)p1u0[q - 1], p1vl = (float)p1v0[q - 1];
+            if (hu) p2ud = (float)p2u0[q - s], p2vd = (float)p2v0[q - s];
         }
         const float uu = U[q], vv = V[q], sku = sku0[q], skv = skv0[q];
         float du, dv, nduh, ndvh;
@@ -565,7 +580,8 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
         s_Z[e] = oZ;
     };
     const float fac = a.tau / a.theta;
-    float *p1u1 = F.p1u[pd ^ 1], *p2u1 = F.p2u[pd ^ 1], *p1v1 = F.p1v[pd ^ 1], *p2v1 = F.p2v[pd ^ 1];
+    TD *p1u1 = as<TD>(F.p1u[pd ^ 1]), *p2u1 = as<TD>(F.p2u[pd ^ 1]), *p1v1 = as<TD>(F.p1v[pd ^ 1]),
+       *p2v1 = as<TD>(F.p2v[pd ^ 1]);
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int ex = threadIdx.x + 1, ey = threadIdx.y + TRW * r + 1;
@@ -585,11 +601,11 @@ __global__ void __launch_bounds__(256) kb_joint_iter(IterArgs a) {
             }
             float o1, o2;
             dual_px(dux, duy, rp[r][0], rp[r][1], fac, o1, o2);
-            p1u1[q] = o1;
-            p2u1[q] = o2;
+            p1u1[q] = st<TD>(o1);
+            p2u1[q] = st<TD>(o2);
             dual_px(dvx, dvy, rp[r][2], rp[r][3], fac, o1, o2);
-            p1v1[q] = o1;
-            p2v1[q] = o2;
+            p1v1[q] = st<TD>(o1);
+            p2v1[q] = st<TD>(o2);
         }
     }
     if (hex >= 0) tri(hex, hey);
@@ -670,8 +686,9 @@ struct JointIn {
 // INNER: the pixel has neighbours on every side (not on column 0 or row 0), known at compile time.
 // TI (not void): the derivatives are computed at the pixel from the level image and the warped
 // image (element type TI) as kb_deriv does and rounded to TS, instead of loaded.
-template <bool PRIOR, typename TS, bool INNER = false, typename TI = void>
+template <bool PRIOR, typename TS, bool INNER = false, typename TI = void, bool D16 = false>
 __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView& V, int xc, int yc, unsigned qb) {
+    using TD = JDual<D16>;
     const int s = V.s;
     const unsigned q = qb + (unsigned)xc + (unsigned)yc * (unsigned)s;
     const bool hl = INNER || xc > 0, hu = INNER || yc > 0;
@@ -680,8 +697,13 @@ __device__ __forceinline__ JointIn joint_load(const IterArgs& a, const JointView
     // the zero field
     JointIn in;
     in.duh = el(V.du0, q), in.dvh = el(V.dv0, q);
-    in.p1u = el(V.p1u0, q), in.p2u = el(V.p2u0, q), in.p1v = el(V.p1v0, q), in.p2v = el(V.p2v0, q);
-    in.p1ul = el(V.p1u0, ql), in.p1vl = el(V.p1v0, ql), in.p2ud = el(V.p2u0, qu), in.p2vd = el(V.p2v0, qu);
+    {
+        const unsigned qd = typed_el<TD>(q, qb), qdl = typed_el<TD>(ql, qb), qdu = typed_el<TD>(qu, qb);
+        in.p1u = (float)el(as<TD>(V.p1u0), qd), in.p2u = (float)el(as<TD>(V.p2u0), qd);
+        in.p1v = (float)el(as<TD>(V.p1v0), qd), in.p2v = (float)el(as<TD>(V.p2v0), qd);
+        in.p1ul = (float)el(as<TD>(V.p1u0), qdl), in.p1vl = (float)el(as<TD>(V.p1v0), qdl);
+        in.p2ud = (float)el(as<TD>(V.p2u0), qdu), in.p2vd = (float)el(as<TD>(V.p2v0), qdu);
+    }
     in.uu = el(V.U, q), in.vv = el(V.V, q);
     in.sku = el(V.sku0, q), in.skv = el(V.skv0, q), in.up0 = el(V.up0, q), in.vp0 = el(V.vp0, q);
     in.uc0 = el(V.uc0, q), in.vc0 = el(V.vc0, q);
@@ -812,12 +834,16 @@ __device__ __forceinline__ JointBack joint_back_calc(const IterArgs& a, const Uc
     return B;
 }
 
-// Stores state n of a pixel (element q).
-__device__ __forceinline__ void joint_store(const JointView& V, unsigned q, const JointFront& J, const JointBack& B) {
-    el(V.p1u1, q) = B.p1u;
-    el(V.p2u1, q) = B.p2u;
-    el(V.p1v1, q) = B.p1v;
-    el(V.p2v1, q) = B.p2v;
+// Stores state n of a pixel (element q of the view, whose pair sits qb floats further on).
+template <bool D16 = false>
+__device__ __forceinline__ void joint_store(const JointView& V, unsigned q, unsigned qb, const JointFront& J,
+                                            const JointBack& B) {
+    using TD = JDual<D16>;
+    const unsigned qd = typed_el<TD>(q, qb);
+    el(as<TD>(V.p1u1), qd) = st<TD>(B.p1u);
+    el(as<TD>(V.p2u1), qd) = st<TD>(B.p2u);
+    el(as<TD>(V.p1v1), qd) = st<TD>(B.p1v);
+    el(as<TD>(V.p2v1), qd) = st<TD>(B.p2v);
     el(V.X1, q) = B.X;
     el(V.Y1, q) = B.Y;
     el(V.Z1, q) = B.Z;
@@ -858,7 +884,7 @@ __device__ __forceinline__ void joint_store_deriv(const JointView& V, unsigned q
 // The body of kb_joint_px for the tile at (x0, y0). INNER: every lane of the extended tile is a pixel
 // at least one pixel inside the image, so no coordinate is clamped and no border rule applies.
 // TI (not void): the lanes compute the derivatives (joint_load) and interior lanes store them.
-template <int EXW, int EXH, bool INNER, bool F64TRI, bool PRIOR, typename TS, typename TI>
+template <int EXW, int EXH, bool INNER, bool F64TRI, bool PRIOR, typename TS, typename TI, bool D16>
 __device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView& V, const PairOff& po, int x0,
                                               int y0, float* s_du, float* s_dv, float* s_X, float* s_Y, float* s_Z) {
     constexpr int NE = EXW * EXH;
@@ -866,7 +892,7 @@ __device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView
     const int ex = threadIdx.x, ey = threadIdx.y, e = ex + ey * EXW;
     const int x = x0 - 1 + ex, y = y0 - 1 + ey;
     const int xc = INNER ? x : min(max(x, 0), w - 1), yc = INNER ? y : min(max(y, 0), h - 1);
-    const JointIn in = joint_load<PRIOR, TS, INNER, TI>(a, V, xc, yc, po.qb);
+    const JointIn in = joint_load<PRIOR, TS, INNER, TI, D16>(a, V, xc, yc, po.qb);
     const JointFront J = joint_compute<F64TRI, PRIOR, INNER>(a, V, po, xc, yc, in);
     s_du[e] = J.nduh;
     s_dv[e] = J.ndvh;
@@ -880,14 +906,15 @@ __device__ __forceinline__ void joint_px_tile(const IterArgs& a, const JointView
     // unsigned 32-bit element offsets: every field access becomes a scalar base plus one vector
     // offset
     const unsigned q = po.qb + (unsigned)x + (unsigned)y * (unsigned)s;
-    joint_store(V, q, J,
-                joint_back_calc<EXW, NE, false, INNER>(a, ucp, J, x, y, w, h, e, s_du, s_dv, s_X, s_Y, s_Z));
+    joint_store<D16>(V, q, po.qb, J,
+                     joint_back_calc<EXW, NE, false, INNER>(a, ucp, J, x, y, w, h, e, s_du, s_dv, s_X, s_Y, s_Z));
     if constexpr (!std::is_void<TI>::value) joint_store_deriv<TS>(V, q, po.qb, in);
 }
 
 // TI (not void): the first launch of a warp, which computes the derivatives at its lanes' pixels
 // and stores them for the warp's later launches (kb_deriv is not launched).
-template <int EXW, int EXH, int OCC, int SM, bool F64TRI, bool PRIOR, typename TS, typename TI = void>
+// D16: the dual fields in binary16 (RF_STORAGE_F16_DERIV_DUALS).
+template <int EXW, int EXH, int OCC, int SM, bool F64TRI, bool PRIOR, typename TS, typename TI = void, bool D16 = false>
 __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(OCC))) kb_joint_px(IterArgs a) {
     constexpr int TW = EXW - 2, TH = EXH - 2, NE = EXW * EXH;
     __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];
@@ -900,9 +927,9 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
     // lanes cover pixels x0 - 1 .. x0 + TW and y0 - 1 .. y0 + TH; the batched kernels keep one path
     // (with more scalars live, a second path makes the registers spill)
     if (SM == 1 && x0 >= 2 && y0 >= 2 && x0 + TW < V.w - 1 && y0 + TH < V.h - 1)
-        joint_px_tile<EXW, EXH, true, F64TRI, PRIOR, TS, TI>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
+        joint_px_tile<EXW, EXH, true, F64TRI, PRIOR, TS, TI, D16>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
     else
-        joint_px_tile<EXW, EXH, false, F64TRI, PRIOR, TS, TI>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
+        joint_px_tile<EXW, EXH, false, F64TRI, PRIOR, TS, TI, D16>(a, V, po, x0, y0, s_du, s_dv, s_X, s_Y, s_Z);
 }
 
 // K joint iterations per launch (temporal blocking) on an EXW x EXH extended tile indexed by flat
@@ -918,8 +945,11 @@ __global__ void __launch_bounds__(EXW* EXH) __attribute__((amdgpu_waves_per_eu(O
 // y0..: state n + K into the other parity, one flip per launch. Same per-pixel arithmetic as
 // kb_joint_px. TI (not void): as for kb_joint_px, the warp's first launch computes and stores the
 // derivatives.
-template <int EXW, int EXH, int K, int SM, bool F64TRI, bool PRIOR, typename TS, typename TI = void>
+// D16: the dual fields in binary16; the duals carried from one in-kernel iteration to the next are
+// rounded to binary16 as a store and reload would round them.
+template <int EXW, int EXH, int K, int SM, bool F64TRI, bool PRIOR, typename TS, typename TI = void, bool D16 = false>
 __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
+    using TD = JDual<D16>;
     constexpr int R = 2 * K - 1, TW = EXW - 2 * R, TH = EXH - 2 * R, NE = EXW * EXH;
     static_assert(TW > 0 && TH > 0, "tile too small for K iterations");
     __shared__ float s_du[NE], s_dv[NE], s_X[NE], s_Y[NE], s_Z[NE];                              // uhat_n, X'_n
@@ -944,7 +974,7 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
     const unsigned Ec = qb + (unsigned)min(max(cx + ry * s, 0), (h + 1) * s - 1);
     const float Xk = el(V.X0, Ec), Yk = el(V.Y0, Ec), Zk = el(V.Z0, Ec);
     const UcParams ucp = cam(V.ucp, po);  // before any store: scalar loads
-    JointIn in = joint_load<PRIOR, TS, false, TI>(a, V, xc, yc, qb);
+    JointIn in = joint_load<PRIOR, TS, false, TI, D16>(a, V, xc, yc, qb);
     const bool hl = xc > 0, hu = yc > 0;
     const int e_l = max(e - 1, 0), e_u = max(e - EXW, 0), e_r = min(e + 1, NE - 1), e_d = min(e + EXW, NE - 1);
 #pragma unroll 1
@@ -960,25 +990,28 @@ __global__ void __launch_bounds__(EXW* EXH) kb_joint_multi(IterArgs a) {
         if (k == K - 1) {
             if (lx >= R && lx < EXW - R && ly >= R && ly < EXH - R && cx < w && ry < h) {
                 const unsigned q = qb + (unsigned)cx + (unsigned)ry * (unsigned)s;
-                joint_store(V, q, J, B);
+                joint_store<D16>(V, q, qb, J, B);
                 if constexpr (!std::is_void<TI>::value) joint_store_deriv<TS>(V, q, qb, in);
             }
             return;
         }
         const float Xn = pix ? B.X : Xk, Yn = pix ? B.Y : Yk, Zn = pix ? B.Z : Zk;
-        s_p1u[e] = B.p1u;
-        s_p2u[e] = B.p2u;
-        s_p1v[e] = B.p1v;
-        s_p2v[e] = B.p2v;
+        // the duals as the next iteration reads them (rounded to the field type)
+        const float p1u = (float)st<TD>(B.p1u), p2u = (float)st<TD>(B.p2u), p1v = (float)st<TD>(B.p1v),
+                    p2v = (float)st<TD>(B.p2v);
+        s_p1u[e] = p1u;
+        s_p2u[e] = p2u;
+        s_p1v[e] = p1v;
+        s_p2v[e] = p2v;
         s_Xn[e] = Xn;
         s_Yn[e] = Yn;
         s_Zn[e] = Zn;
         __syncthreads();
         // state n as the next iteration's inputs (what joint_load reads from memory)
         in.duh = J.nduh, in.dvh = J.ndvh;
-        in.p1u = B.p1u, in.p2u = B.p2u, in.p1v = B.p1v, in.p2v = B.p2v;
-        in.p1ul = hl ? s_p1u[e_l] : B.p1u, in.p1vl = hl ? s_p1v[e_l] : B.p1v;
-        in.p2ud = hu ? s_p2u[e_u] : B.p2u, in.p2vd = hu ? s_p2v[e_u] : B.p2v;
+        in.p1u = p1u, in.p2u = p2u, in.p1v = p1v, in.p2v = p2v;
+        in.p1ul = hl ? s_p1u[e_l] : p1u, in.p1vl = hl ? s_p1v[e_l] : p1v;
+        in.p2ud = hu ? s_p2u[e_u] : p2u, in.p2vd = hu ? s_p2v[e_u] : p2v;
         in.sku = B.sku, in.skv = B.skv, in.up0 = J.upn, in.vp0 = J.vpn, in.uc0 = B.uc, in.vc0 = B.vc;
         in.Xc = Xn, in.Yc = Yn, in.Zc = Zn;
         in.Xr = s_Xn[e_r], in.Yr = s_Yn[e_r], in.Zr = s_Zn[e_r];
@@ -1302,7 +1335,7 @@ int Batch::create(const rf_params& p, int n, const int* widths, const int* heigh
         set_error("rf_batch_create: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
     }
-    if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV) {
+    if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV_DUALS) {
         set_error("rf_batch_create: unknown storage");
         return RF_ERR_ARG;
     }
@@ -1750,27 +1783,27 @@ int Batch::warm_start_in_use(int i) {
 }
 
 // TK: the image type when the launch also computes the warp's derivatives, else void
-template <typename TK, typename TS, int SM, bool F64, bool PRIOR>
+template <typename TK, typename TS, int SM, bool F64, bool PRIOR, bool D16>
 static void launch_joint_t(int cls, hipStream_t st, const IterArgs& a) {
     const dim3 grid(a.L.ntiles);
     switch (cls) {
         case TC_J64:  // never asked to compute the derivatives (fuses_deriv)
-            hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, F64, PRIOR, TS>), grid, dim3(TXW, TRW), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_iter<RPT_JOINT, F64, PRIOR, TS, D16>), grid, dim3(TXW, TRW), 0, st, a);
             break;
         case TC_J32:
-            hipLaunchKernelGGL((kb_joint_px<32, 16, 4, SM, F64, PRIOR, TS, TK>), grid, dim3(32, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<32, 16, 4, SM, F64, PRIOR, TS, TK, D16>), grid, dim3(32, 16), 0, st, a);
             break;
         case TC_J16O:
-            hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 5, SM, F64, PRIOR, TS, TK, D16>), grid, dim3(16, 16), 0, st, a);
             break;
         case TC_JK2:
-            hipLaunchKernelGGL((kb_joint_multi<16, 16, 2, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 2, SM, F64, PRIOR, TS, TK, D16>), grid, dim3(16, 16), 0, st, a);
             break;
         case TC_JK3:
-            hipLaunchKernelGGL((kb_joint_multi<16, 16, 3, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_multi<16, 16, 3, SM, F64, PRIOR, TS, TK, D16>), grid, dim3(16, 16), 0, st, a);
             break;
         default:
-            hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SM, F64, PRIOR, TS, TK>), grid, dim3(16, 16), 0, st, a);
+            hipLaunchKernelGGL((kb_joint_px<16, 16, 1, SM, F64, PRIOR, TS, TK, D16>), grid, dim3(16, 16), 0, st, a);
             break;
     }
 }
@@ -1816,30 +1849,41 @@ static bool fuses_deriv(int cls) {
 }
 
 // sm: how the kernels find the fields (see tile_view); deriv: the launch also computes the warp's
-// derivatives (the caller then skips kb_deriv)
+// derivatives (the caller then skips kb_deriv); d16: the dual fields in binary16 (with 16-bit
+// derivative fields only)
 template <typename TI, typename TS>
-static void launch_joint(int cls, int sm, bool f64, bool prior, bool deriv, hipStream_t st, const IterArgs& a) {
+static void launch_joint(int cls, int sm, bool f64, bool prior, bool deriv, bool d16, hipStream_t st,
+                         const IterArgs& a) {
     const int k = (f64 ? 2 : 0) | (prior ? 1 : 0);
-#define RF_J(TK, SM)                                                            \
-    switch (k) {                                                                \
-        case 0: launch_joint_t<TK, TS, SM, false, false>(cls, st, a); break;    \
-        case 1: launch_joint_t<TK, TS, SM, false, true>(cls, st, a); break;     \
-        case 2: launch_joint_t<TK, TS, SM, true, false>(cls, st, a); break;     \
-        default: launch_joint_t<TK, TS, SM, true, true>(cls, st, a); break;     \
+#define RF_J(TK, SM, D)                                                            \
+    switch (k) {                                                                   \
+        case 0: launch_joint_t<TK, TS, SM, false, false, D>(cls, st, a); break;    \
+        case 1: launch_joint_t<TK, TS, SM, false, true, D>(cls, st, a); break;     \
+        case 2: launch_joint_t<TK, TS, SM, true, false, D>(cls, st, a); break;     \
+        default: launch_joint_t<TK, TS, SM, true, true, D>(cls, st, a); break;     \
     }
-#define RF_JS(TK)       \
-    if (sm == 1) {      \
-        RF_J(TK, 1)     \
-    } else if (sm == 2) { \
-        RF_J(TK, 2)     \
-    } else {            \
-        RF_J(TK, 0)     \
+#define RF_JS(TK, D)       \
+    if (sm == 1) {         \
+        RF_J(TK, 1, D)     \
+    } else if (sm == 2) {  \
+        RF_J(TK, 2, D)     \
+    } else {               \
+        RF_J(TK, 0, D)     \
     }
-    if (deriv) {
-        RF_JS(TI)
-    } else {
-        RF_JS(void)
+#define RF_JD(D)          \
+    if (deriv) {          \
+        RF_JS(TI, D)      \
+    } else {              \
+        RF_JS(void, D)    \
     }
+    if constexpr (std::is_same<TS, __half>::value) {
+        if (d16) {
+            RF_JD(true)
+            return;
+        }
+    }
+    RF_JD(false)
+#undef RF_JD
 #undef RF_JS
 #undef RF_J
 }
@@ -1863,7 +1907,8 @@ static void launch_joint(int cls, int sm, bool f64, bool prior, bool deriv, hipS
 hipError_t Batch::enqueue(bool recon) {
     switch (p_.storage) {
         case RF_STORAGE_F16: return enqueue_t<__half, __half>(recon);
-        case RF_STORAGE_F16_DERIV: return enqueue_t<float, __half>(recon);
+        case RF_STORAGE_F16_DERIV:
+        case RF_STORAGE_F16_DERIV_DUALS: return enqueue_t<float, __half>(recon);
         default: return enqueue_t<float, float>(recon);
     }
 }
@@ -1925,7 +1970,8 @@ hipError_t Batch::enqueue_t(bool recon) {
                         launch_flow<TI, TS>(cls, single, fused_deriv && it == 0, st_, a);
                     else
                         launch_joint<TI, TS>(cls, single ? (n() == 1 ? 1 : 2) : 0, p_.tri_f64 != 0, fn,
-                                             fused_deriv && it == 0, st_, a);
+                                             fused_deriv && it == 0, p_.storage == RF_STORAGE_F16_DERIV_DUALS, st_,
+                                             a);
                     RF_TRY(hipGetLastError());
                 }
                 pr.d ^= 1;  // one flip per launch, whatever its iteration count

@@ -134,7 +134,7 @@ int Solver::initialize(const rf_params& p) {
         set_error("rf_initialize: unknown joint or TV-L1 tiling");
         return RF_ERR_ARG;
     }
-    if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV) {
+    if (p.storage < RF_STORAGE_F32 || p.storage > RF_STORAGE_F16_DERIV_DUALS) {
         set_error("rf_initialize: unknown storage");
         return RF_ERR_ARG;
     }

@@ -138,7 +138,7 @@ def _solve_single(p, w, h, setup):
     return out
 
 
-@pytest.mark.parametrize("kind", ["joint-f16-derivatives", "tvl1-f16"])
+@pytest.mark.parametrize("kind", ["joint-f16-derivatives", "joint-f16-derivatives-duals", "tvl1-f16"])
 def test_batch_same_size_fp16_pairs_equal_single(kind):
     # 16-bit fields of later pairs sit at the pair offset counted in 16-bit elements: every pair of
     # a same-size FP16 batch equals the same pair solved alone, bit for bit
@@ -146,7 +146,8 @@ def test_batch_same_size_fp16_pairs_equal_single(kind):
     print(f"case: batch {kind} {n} pairs of {w}x{h}", flush=True)
     joint = kind.startswith("joint")
     if joint:
-        p = rf.batch_params(14, 3, 2, 7, tau=0.125, use_graph=True, tri_f64=1, storage=rf.STORAGE_F16_DERIV)
+        storage = rf.STORAGE_F16_DERIV_DUALS if kind.endswith("duals") else rf.STORAGE_F16_DERIV
+        p = rf.batch_params(14, 3, 2, 7, tau=0.125, use_graph=True, tri_f64=1, storage=storage)
         data = [rf.make_recon_pair(w, h, 500 + i) for i in range(n)]
     else:
         p = rf.batch_params(0, 3, 2, 9, use_graph=True, storage=rf.STORAGE_F16)

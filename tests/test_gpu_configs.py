@@ -2,8 +2,8 @@
 
 * Uneven-size FP16 batches (pairs wider than 256 px, the descriptor path with 16-bit fields):
   every pair equals the same pair solved alone and the oracle's restatement of the FP16 storage
-  mode bit for bit (TV-L1 with FP16 images, derivatives and duals; joint with FP16 derivatives,
-  FP64 mode); TV-L1 pairs stay within 0.02 px of the FP64 oracle.
+  mode bit for bit (TV-L1 with FP16 images, derivatives and duals; joint with FP16 derivatives, and
+  with FP16 derivatives and duals, FP64 mode); TV-L1 pairs stay within 0.02 px of the FP64 oracle.
 * The 64-pair uneven batch (W 160-1280, H 120-720 in steps of 16; even pairs TV-L1 with 4 levels x
   3 warps x 30 iterations, odd pairs joint with 5 levels x 5 warps x 50 iterations): every output
   is finite, and three sampled pairs equal their single-pair solves bit for bit.
@@ -64,7 +64,7 @@ UNEVEN = [(320, 240), (272, 160), (400, 208), (288, 304)]
 TVL1_F16_BOUND = 0.02
 
 
-@pytest.mark.parametrize("kind", ["tvl1-f16", "joint-f16-derivatives"])
+@pytest.mark.parametrize("kind", ["tvl1-f16", "joint-f16-derivatives", "joint-f16-derivatives-duals"])
 def test_uneven_fp16_batch_pairs_equal_single_and_oracle(orc, kind):
     print(f"case: uneven {kind} batch {UNEVEN}", flush=True)
     joint = kind.startswith("joint")
@@ -73,7 +73,7 @@ def test_uneven_fp16_batch_pairs_equal_single_and_oracle(orc, kind):
         # K[2][2] = 1 on the coarse levels: with the reference's degenerate coarse cameras the joint
         # flow at these sizes runs to hundreds of pixels, where any rounding grows
         p = rf.batch_params(14, L, nw, ni, tau=0.125, use_graph=True, tri_f64=1, fix_coarse_k=1,
-                            storage=rf.STORAGE_F16_DERIV)
+                            storage=rf.STORAGE_F16_DERIV_DUALS if kind.endswith("duals") else rf.STORAGE_F16_DERIV)
         data = [rf.make_recon_pair(w, h, 800 + i) for i, (w, h) in enumerate(UNEVEN)]
     else:
         L, nw, ni = 4, 3, 30

@@ -96,6 +96,24 @@ def test_flow_tilings_equal(orc, storage, ni):
     assert np.array_equal(base[0][:, :200], ru[:, :200]) and np.array_equal(base[1][:, :200], rv[:, :200])
 
 
+def test_flow_f16_derivatives_duals_mode_is_f16_derivatives(orc):
+    # RF_STORAGE_F16_DERIV_DUALS adds the joint solver's duals to RF_STORAGE_F16_DERIV; TV-L1 already
+    # stores its duals with its derivatives, so the mode solves TV-L1 as RF_STORAGE_F16_DERIV does
+    print("case: TV-L1 200x152 fused graph storage F16_DERIV vs F16_DERIV_DUALS", flush=True)
+    i0, i1, _, _ = rf.make_flow_pair(200, 152, 6)
+    out = {}
+    for storage in (rf.STORAGE_F16_DERIV, rf.STORAGE_F16_DERIV_DUALS):
+        f = _flow(path=rf.PATH_FUSED, graph=True, w=200, h=152, L=3, nw=3, ni=8, storage=storage)
+        f.copyImagesToDevice(i0, i1)
+        f.solveOpticalFlow(5.0)
+        out[storage] = f.copyOpticalFlowToHost()
+    a, b = out[rf.STORAGE_F16_DERIV], out[rf.STORAGE_F16_DERIV_DUALS]
+    assert np.array_equal(a[0][:, :200], b[0][:, :200]) and np.array_equal(a[1][:, :200], b[1][:, :200])
+    ru, rv = orc.solve_flow(orc.flow_params(200, 152, 3, n_warp=3, n_iter=8, storage=rf.STORAGE_F16_DERIV_DUALS),
+                            i0, i1, "f32")
+    assert np.array_equal(b[0][:, :200], ru[:, :200]) and np.array_equal(b[1][:, :200], rv[:, :200])
+
+
 def test_flow_320x240_fp16_storage_epe(orc, tvl1_320):
     # FP16 images / derivatives / duals, FP32 updates: the oracle's restatement of the mode bit for
     # bit, and its accuracy against the FP64 oracle (measured 0.0035 px mean; bound 0.01)
@@ -168,11 +186,14 @@ def _depth_rel(Z, Z64, w):
 #   FP32 storage:             mean EPE 5.8e-4 / 5.1e-4 / 4.5e-4 px, depth rel. median ~1e-6,
 #                             p99 3.5e-5 / 2.2e-5 / 7.9e-5;
 #   FP16 derivatives:         mean EPE 0.028 / 0.020 / 0.019 px, depth median 5e-5, p99 <= 2e-3;
+#   FP16 derivatives+duals:   mean EPE 0.030 / 0.023 / 0.018 px, depth median 5e-5,
+#                             p99 2.6e-3 / 9.4e-4 / 2.3e-3;
 #   FP16 images+derivatives:  mean EPE 0.164 / 0.167 / 0.126 px, depth median 3e-4, p99 <= 1.7e-2
 #                             (misses the 0.05 px target: the joint solution follows the rounding
 #                             of It = I1w - I0 in FP16 images; bounded at its measured level).
 # Bounds: the target where a mode meets it, else the measured value plus ~25%.
 ACC = {rf.STORAGE_F32: (1e-3, 1e-5, 1e-4), rf.STORAGE_F16_DERIV: (0.05, 1e-4, 3e-3),
+       rf.STORAGE_F16_DERIV_DUALS: (0.05, 1e-4, 3.5e-3),
        rf.STORAGE_F16: (0.2, 5e-4, 2.5e-2)}
 
 
@@ -199,8 +220,8 @@ def test_recon_640x480_fp64_mode_accuracy(joint_640, joint_640_f64):
     _check_accuracy(u, v, Z, joint_640_f64, rf.STORAGE_F32)
 
 
-@pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16_DERIV, rf.STORAGE_F16],
-                         ids=["f32", "f16-deriv", "f16"])
+@pytest.mark.parametrize("storage", [rf.STORAGE_F32, rf.STORAGE_F16_DERIV, rf.STORAGE_F16_DERIV_DUALS,
+                                     rf.STORAGE_F16], ids=["f32", "f16-deriv", "f16-deriv-duals", "f16"])
 def test_recon_640x480_fused_accuracy(orc, joint_640, joint_640_f64, storage):
     print(f"case: joint 640x480 fused graph FP64 mode storage={storage}", flush=True)
     r = _recon(640, 480, 5, path=rf.PATH_FUSED, graph=True, tri=1, storage=storage)
@@ -296,7 +317,8 @@ def test_recon_multi_iteration_launches_equal_oracle(orc, tile, w, h):
         assert _same(_run_recon(r, d), ref, w), rep
 
 
-@pytest.mark.parametrize("storage", [rf.STORAGE_F16, rf.STORAGE_F16_DERIV], ids=["f16", "f16-deriv"])
+@pytest.mark.parametrize("storage", [rf.STORAGE_F16, rf.STORAGE_F16_DERIV, rf.STORAGE_F16_DERIV_DUALS],
+                         ids=["f16", "f16-deriv", "f16-deriv-duals"])
 def test_recon_fp16_tilings_equal(orc, storage):
     # FP16 storage: every joint tiling gives the bits of the oracle's restatement of the mode (the
     # 64x8 tiling's warps compute their derivatives in a launch of their own, the other tilings in a

@@ -452,3 +452,21 @@ def test_fp16_derivative_storage_rounds_the_derivatives(orc):
     assert np.array_equal(got[1][:, :w], v[:h * s].reshape(h, s)[:, :w])
     plain = orc.solve_flow(orc.flow_params(w, h, 1, n_warp=1, n_iter=1), i0, i1, "f32")
     assert not np.array_equal(plain[0][:, :w], got[0][:, :w])
+
+
+def test_fp16_joint_dual_storage_rounds_only_the_duals(orc):
+    """Storage 3 (RF_STORAGE_F16_DERIV_DUALS) adds the joint solver's duals to storage 2: the duals are
+    reset at every warp and read only by the next iteration's data step, so with one iteration per
+    warp the rounded duals are never read and the two modes give the same bits, while two iterations
+    per warp read them and differ. TV-L1 already keeps its duals with its derivatives, so storage 3
+    solves TV-L1 as storage 2 does."""
+    d = rf.make_recon_pair(96, 72, 7)
+    args = (d["K"], d["K"], np.eye(3), np.zeros(3), d["R"], d["t"], d["i0"], d["i1"])
+    for ni, same in ((1, True), (2, False)):
+        a = orc.solve_recon(orc.recon_params(96, 72, 2, n_warp=2, n_iter=ni, tri_f64=1, storage=2), *args, "f32")
+        b = orc.solve_recon(orc.recon_params(96, 72, 2, n_warp=2, n_iter=ni, tri_f64=1, storage=3), *args, "f32")
+        assert all(np.array_equal(x, y, equal_nan=True) for x, y in zip(a, b)) == same, ni
+    i0, i1, _, _ = rf.make_flow_pair(96, 72, 8)
+    a = orc.solve_flow(orc.flow_params(96, 72, 2, n_warp=2, n_iter=5, storage=2), i0, i1, "f32")
+    b = orc.solve_flow(orc.flow_params(96, 72, 2, n_warp=2, n_iter=5, storage=3), i0, i1, "f32")
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
