@@ -13,7 +13,8 @@ launch over its average launch time at level 0, timed with HIP events on the sol
 the FP64 CPU oracle's time on the same frame (the reported CPU baseline), the mean EPE of the
 GPU flow against that FP64 result, and the per-step (plain) GPU path's frames/s beside it.
 `batched_640x480` adds the same solve over --batch-pairs seeded 640x480 pairs per batched launch
-(FP32 storage, and FP16 derivatives), frames/s and pair 0's EPE against the FP64 result.
+(FP32 storage, FP16 derivatives, FP16 derivatives and duals), frames/s and pair 0's EPE against the
+FP64 result.
 
 --workload tvl1-320: the 320x240 TV-L1 case (fused, fused FP16, plain; EPE and time of the FP64
 oracle). --workload methods: each CudaFlow data term at 320x240 on the per-step path.
@@ -222,7 +223,7 @@ def run_warm_started(args, rf, torch, d, ref_fixed):
     per frame the network's prediction (pyramid, six BF16 MFMA convolutions, upscale to level 0) and
     a solve that starts from it at level 3 (4 levels, the network's resolution) or level 2 (3 levels),
     5 warps x 50 iterations, K[2][2] = 1, FP64 mode. One pair per step (FP32 storage) and 8 pairs per
-    batched launch (FP16 derivatives, 4 levels); the network runs on the solver's stream inside the
+    batched launch (FP16 derivatives and duals, 4 levels); the network runs on the solver's stream inside the
     timed step. Accuracy against the FP64 oracle's full cold solve (5 levels) of the same mode."""
     from reconflow_amd import warmnet as wn
     path = wn.WEIGHTS / "warmnet_640x480.npz"
@@ -231,7 +232,7 @@ def run_warm_started(args, rf, torch, d, ref_fixed):
     out = {"weights": str(path.relative_to(ROOT))}
     for name, B, storage, L in (("1_pair_4_levels_f32", 1, rf.STORAGE_F32, 4),
                                 ("1_pair_3_levels_f32", 1, rf.STORAGE_F32, 3),
-                                ("8_pairs_4_levels_f16_derivatives", 8, rf.STORAGE_F16_DERIV, 4)):
+                                ("8_pairs_4_levels_f16_derivatives_duals", 8, rf.STORAGE_F16_DERIV_DUALS, 4)):
         net = wn.WarmStartNet.load(path)
         p = rf.batch_params(14, L, WARPS, ITERS, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0, lambdaf=0.1,
                             lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage, fix_coarse_k=1)
@@ -570,10 +571,11 @@ def run_stream(args, rf, torch):
     frames = torch.from_numpy(seq["frames"]).cuda()
     K, R, t = seq["K"], seq["R"], seq["t"]
     runs, pair0 = {}, None
-    for name, B, storage, mode in (("1_pair_cold_f16_derivatives", 1, rf.STORAGE_F16_DERIV, "cold"),
-                                   ("1_pair_warm_started_f16_derivatives", 1, rf.STORAGE_F16_DERIV, "warm"),
-                                   ("8_pairs_cold_f16_derivatives", 8, rf.STORAGE_F16_DERIV, "cold"),
-                                   ("8_chunks_warm_started_f16_derivatives", 8, rf.STORAGE_F16_DERIV, "chunks")):
+    f16 = rf.STORAGE_F16_DERIV_DUALS
+    for name, B, storage, mode in (("1_pair_cold_f16_derivatives_duals", 1, f16, "cold"),
+                                   ("1_pair_warm_started_f16_derivatives_duals", 1, f16, "warm"),
+                                   ("8_pairs_cold_f16_derivatives_duals", 8, f16, "cold"),
+                                   ("8_chunks_warm_started_f16_derivatives_duals", 8, f16, "chunks")):
         p = rf.batch_params(14, LEVELS, WARPS, ITERS, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0,
                             lambdaf=0.1, lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage, fix_coarse_k=1)
         b = rf.Batch(p, [w] * B, [h] * B)
@@ -659,13 +661,13 @@ def run_stream(args, rf, torch):
                               t[0], f[0], f[1], "f64")
         e = np.hypot(pair0[0][:, :w] - ref[0][:, :w], pair0[1][:, :w] - ref[1][:, :w])
         epe = {"mean": round(float(e.mean()), 5), "median": round(float(np.median(e)), 6)}
-    best = runs["8_chunks_warm_started_f16_derivatives"]
+    best = runs["8_chunks_warm_started_f16_derivatives_duals"]
     return {
         "metric": "1280x720 streaming joint flow+depth frames/s (the sequence in 8 chunks of consecutive pairs, one "
                   "pair of each chunk per batched launch, each pair warm-started from the previous pair of its chunk)",
         "value": best["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": best["pairs"], "warmup": 1,
         "ms_per_step": round(1e3 / best["frames_per_s"], 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 updates, f16 derivatives" + (", f64 mode" if args.tri_f64 else ""),
+        "vs_baseline": None, "dtype": "f32 updates, f16 derivatives and duals" + (", f64 mode" if args.tri_f64 else ""),
         "data": f"synthetic {n}-frame camera trajectory over a piecewise-planar scene (generated in {t_gen:.1f} s)",
         "config": {"workload": "stream 1280x720, 5 levels x 5 warps x 50 iterations, K[2][2] = 1 on coarse levels"},
         "runs": runs, "epe_pair0_vs_fp64_oracle": epe, "cpu_baseline": None,
