@@ -717,7 +717,7 @@ def main() -> int:
                     default="joint640")
     ap.add_argument("--stream-frames", type=int, default=300, help="stream720: frames of the sequence")
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
-    ap.add_argument("--batch-pairs", type=int, default=8,
+    ap.add_argument("--batch-pairs", type=int, default=32,
                     help="joint640: also time this many 640x480 pairs per batched launch (0: skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
