@@ -1517,6 +1517,14 @@ static const int kTuneCand[2][kNumCand] = {{TC_F16, TC_FK4, TC_F32, TC_FK6, TC_F
                                            {TC_J16, TC_J32, TC_JK2, TC_J16O, TC_JK3, TC_JK3}};
 static const int kCandCount[2] = {6, 5};
 
+// The candidate a tuning solve times on a warp: round r covers candidates r * nw .. r * nw + nw - 1
+// (mod the candidate count), so every candidate is timed in at least two rounds, and the warps take
+// them in an order rotated by r, so a candidate's samples come from different warps of the level.
+int Batch::tune_slot(int k, int round, int warp) const {
+    const int nw = p_.n_warp_iters;
+    return (round * nw + (warp + round) % nw) % kCandCount[k];
+}
+
 // The class of the last `rem` iterations of a warp when the level's class runs more per launch:
 // the most iterations per launch that do not pass rem.
 static int remainder_class(bool recon, int rem) {
@@ -1548,7 +1556,7 @@ int Batch::tile_class(bool recon, int level, int warp) const {
         }
     }
     const int k = recon ? 1 : 0;
-    if (tuning_) return kTuneCand[k][(tune_round_ * p_.n_warp_iters + warp) % kCandCount[k]];
+    if (tuning_) return kTuneCand[k][tune_slot(k, tune_round_, warp)];
     return tuned_[k] ? cls_[k][level] : kTuneCand[k][0];
 }
 
@@ -1607,7 +1615,7 @@ int Batch::tune_tiling(bool recon) {
                     rc = RF_ERR_HIP;
                     break;
                 }
-                const size_t i = (size_t)l * nc + (r * nw + wi) % nc;
+                const size_t i = (size_t)l * nc + tune_slot(k, r, wi);
                 t[i] = cnt[i] ? std::min(t[i], (double)ms) : (double)ms;
                 ++cnt[i];
             }

@@ -116,6 +116,7 @@ class Batch {
     int upload_pairs();
     TileSet& tiles(int level, int cls);
     int tile_class(bool recon, int level, int warp) const;
+    int tune_slot(int k, int round, int warp) const;
     int tune_tiling(bool recon);
     void release();
 
