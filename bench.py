@@ -141,7 +141,8 @@ def run_joint_batched(args, rf, torch, n, storage):
     """n seeded 640x480 pairs (pair 0 is the single-pair workload's scene) in one batch: every
     launch covers the tiles of all n pairs. Returns frames/s and pair 0's flow."""
     p = rf.batch_params(14, LEVELS, WARPS, ITERS, lam=50.0, tau=0.125, alpha_tv=33.3, alpha_proj=60.0,
-                        lambdaf=0.1, lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage)
+                        lambdaf=0.1, lambdams=100.0, tri_f64=int(args.tri_f64), storage=storage,
+                        joint_tile=args.joint_tile)
     b = rf.Batch(p, [W] * n, [H] * n)
     keep = []
     for i in range(n):
@@ -725,7 +726,7 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--tri-f64", type=int, default=1, help="FP64 triangulation (1) or the reference FP32 (0)")
     ap.add_argument("--joint-tile", type=int, default=0, choices=sorted(TILING),
-                    help="fused joint tiling: 0 autotuned per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16, "
+                    help="fused joint tiling of the one-pair and batched 640x480 runs: 0 autotuned per level, 1 64x8 two rows per lane, 2 32x16, 3 16x16, "
                          "4 16x16 capped at 96 VGPRs")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
